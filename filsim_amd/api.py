@@ -1,0 +1,288 @@
+"""Thin Python handles over the C-ABI (include/filsim.h).
+
+Device arrays are torch.cuda float64 tensors (torch is only the allocator);
+host arrays are numpy. Every call goes through libfilsim_amd.so and raises
+FsError on a non-zero status.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+def _ptr(t):
+    """Device pointer of a contiguous torch tensor (or None)."""
+    if t is None:
+        return None
+    assert t.is_contiguous(), "tensor must be contiguous"
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(s):
+    if s is None:
+        return None
+    return C.c_void_p(s.cuda_stream if hasattr(s, "cuda_stream") else int(s))
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Fcm:
+    """FCM mobility solver on one GPU (struct Fcm, FCMfunctions.hpp:7-59)."""
+
+    def __init__(self, fp, max_np, precision=abi.FS_FP64):
+        self.lib = abi.load()
+        self.fp = fp
+        self.max_np = max_np
+        self.precision = precision
+        self.h = C.c_void_p()
+        abi.check(self.lib.fs_fcm_create(C.byref(fp), max_np, precision,
+                                         C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.lib.fs_fcm_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def mobility(self, Y, F, T, V, W, stream=None):
+        n = Y.shape[0]
+        abi.check(self.lib.fs_fcm_mobility(self.h, n, _ptr(Y), _ptr(F),
+                                           _ptr(T), _ptr(V), _ptr(W),
+                                           _stream(stream)))
+
+    def mobility_host(self, Y, F, T):
+        Y, F, T = (_f64(a, (-1, 3)) for a in (Y, F, T))
+        n = Y.shape[0]
+        V = np.zeros((n, 3))
+        W = np.zeros((n, 3))
+        abi.check(self.lib.fs_fcm_mobility_host(
+            self.h, n, abi.dptr(Y), abi.dptr(F), abi.dptr(T), abi.dptr(V),
+            abi.dptr(W)))
+        return V, W
+
+    def friction(self, F, T, V, W, stream=None):
+        abi.check(self.lib.fs_fcm_friction(self.h, F.shape[0], _ptr(F),
+                                           _ptr(T), _ptr(V), _ptr(W),
+                                           _stream(stream)))
+
+    def set_timing(self, on=True):
+        abi.check(self.lib.fs_fcm_set_timing(self.h, 1 if on else 0))
+
+    def stage_times(self):
+        t = np.zeros(7, dtype=np.float32)
+        abi.check(self.lib.fs_fcm_stage_times(self.h, abi.fptr(t)))
+        return t
+
+    def get_tuning(self):
+        t = abi.FcmTuning()
+        abi.check(self.lib.fs_fcm_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, fft_batch=-1, spread_tile=-1, interp_wave=-1):
+        t = abi.FcmTuning(fft_batch, spread_tile, interp_wave)
+        abi.check(self.lib.fs_fcm_set_tuning(self.h, C.byref(t)))
+
+
+class Colbar:
+    """Collision barrier (struct LinkedList,
+    CollisionBarrierFilament.hpp:19-57)."""
+
+    def __init__(self, cp, seg_fil, seg_idx):
+        self.lib = abi.load()
+        self.sf = np.ascontiguousarray(seg_fil, dtype=np.int32)
+        self.si = np.ascontiguousarray(seg_idx, dtype=np.int32)
+        self.np = len(self.sf)
+        self.h = C.c_void_p()
+        abi.check(self.lib.fs_colbar_create(C.byref(cp), self.np,
+                                            abi.iptr(self.sf),
+                                            abi.iptr(self.si),
+                                            C.byref(self.h)))
+        m = np.zeros(3, dtype=np.int32)
+        abi.check(self.lib.fs_colbar_dims(self.h, abi.iptr(m)))
+        self.M = tuple(int(v) for v in m)
+
+    def close(self):
+        if self.h:
+            self.lib.fs_colbar_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def link(self, Xs, stream=None):
+        abi.check(self.lib.fs_colbar_link(self.h, _ptr(Xs), _stream(stream)))
+
+    def apply(self, Y, nt, F, stream=None):
+        md = C.c_double()
+        abi.check(self.lib.fs_colbar_apply(self.h, _ptr(Y), nt, _ptr(F),
+                                           C.byref(md), _stream(stream)))
+        return md.value
+
+    def lists(self):
+        nc = self.M[0] * self.M[1] * self.M[2]
+        head = np.zeros(nc, dtype=np.int32)
+        lst = np.zeros(max(self.np, 1), dtype=np.int32)
+        abi.check(self.lib.fs_colbar_get_lists(self.h, abi.iptr(head),
+                                               abi.iptr(lst)))
+        return head, lst[:self.np]
+
+    def pairs(self, Y, nt, max_pairs=1 << 20):
+        out = np.zeros(2 * max_pairs, dtype=np.int32)
+        n = C.c_int()
+        abi.check(self.lib.fs_colbar_pairs(self.h, _ptr(Y), nt, max_pairs,
+                                           abi.iptr(out), C.byref(n)))
+        assert n.value <= max_pairs
+        return out[:2 * n.value].reshape(n.value, 2)
+
+
+STATE_FIELDS = ("X", "Xs", "Xt", "U", "Us", "Ut", "q", "qs", "qt", "lam",
+                "lam1", "lam2", "F", "TAU")
+
+
+def _width(name):
+    return 4 if name in ("q", "qs", "qt") else 3
+
+
+class System:
+    """Filament system with its FCM solver and barrier, resident on the GPU
+    (class Filament + filamentJsolver + main.cpp time loop)."""
+
+    def __init__(self, sp, nw, props):
+        self.lib = abi.load()
+        self.sp = sp
+        self.nw = np.ascontiguousarray(nw, dtype=np.int32)
+        self.props = props
+        self.h = C.c_void_p()
+        abi.check(self.lib.fs_system_create(C.byref(sp), len(self.nw),
+                                            abi.iptr(self.nw), props,
+                                            C.byref(self.h)))
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        abi.check(self.lib.fs_system_sizes(self.h, C.byref(a), C.byref(b),
+                                           C.byref(c)))
+        self.nfil, self.np, self.nlam = a.value, b.value, c.value
+
+    def close(self):
+        if self.h:
+            self.lib.fs_system_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def _rows(self, k):
+        return self.nlam if k.startswith("lam") else self.np
+
+    def set_state(self, **kw):
+        s = abi.StatePtrs()
+        keep = []
+        for k, v in kw.items():
+            a = _f64(v)
+            assert a.size == self._rows(k) * _width(k), k
+            keep.append(a)
+            setattr(s, k, abi.dptr(a))
+        abi.check(self.lib.fs_system_set_state(self.h, C.byref(s)))
+
+    def get_state(self):
+        s = abi.StatePtrs()
+        out = {}
+        for k in STATE_FIELDS:
+            a = np.zeros((self._rows(k), _width(k)))
+            out[k] = a
+            setattr(s, k, abi.dptr(a))
+        abi.check(self.lib.fs_system_get_state(self.h, C.byref(s)))
+        return out
+
+    def fcm_ptrs(self):
+        ps = [C.c_void_p() for _ in range(5)]
+        abi.check(self.lib.fs_system_fcm_arrays(self.h,
+                                                *[C.byref(p) for p in ps]))
+        return ps
+
+    def timestep(self, nt):
+        it = C.c_int()
+        abi.check(self.lib.fs_system_timestep(self.h, nt, C.byref(it)))
+        return it.value
+
+    def initial_guess(self):
+        abi.check(self.lib.fs_fil_initial_guess(self.h))
+
+    def forces(self, nt):
+        abi.check(self.lib.fs_fil_forces(self.h, nt))
+
+    def build_jacobian(self, nt):
+        abi.check(self.lib.fs_fil_build_jacobian(self.h, nt))
+
+    def jacobian(self, f):
+        n = 6 * int(self.nw[f])
+        out = np.zeros((n, n))
+        abi.check(self.lib.fs_fil_get_jacobian(self.h, f, abi.dptr(out)))
+        return out
+
+    def jinv(self, f):
+        n = 6 * int(self.nw[f])
+        out = np.zeros((n, n))
+        abi.check(self.lib.fs_fil_get_jinv(self.h, f, abi.dptr(out)))
+        return out
+
+    def jsolve(self, b, x, it):
+        abi.check(self.lib.fs_fil_jsolve(self.h, _ptr(b), _ptr(x), it))
+
+    def broyden_update(self, E, En, upd, it):
+        abi.check(self.lib.fs_fil_broyden_update(self.h, _ptr(E), _ptr(En),
+                                                 _ptr(upd), it))
+
+    def update(self, u):
+        abi.check(self.lib.fs_fil_update(self.h, _ptr(u)))
+
+    def residual(self, nt, V, W, err):
+        chk = C.c_int()
+        mx = C.c_double()
+        abi.check(self.lib.fs_fil_residual(self.h, nt, _ptr(V), _ptr(W),
+                                           _ptr(err), C.byref(chk),
+                                           C.byref(mx)))
+        return chk.value, mx.value
+
+    def step(self, nt):
+        abi.check(self.lib.fs_fil_step(self.h, nt))
+
+    def mobility(self, nt):
+        abi.check(self.lib.fs_system_mobility(self.h, nt))
+
+    def barrier(self, nt, link):
+        md = C.c_double()
+        abi.check(self.lib.fs_system_barrier(self.h, nt, 1 if link else 0,
+                                             C.byref(md)))
+        return md.value
+
+
+def make_filaments(box, nw, dL=2.2, min_sep=2.2, seed=1, max_tries=100000):
+    """Seeded straight filaments (heads uniform, orientations uniform),
+    overlap-rejected at min_sep. Returns X (np,3), q (np,4)."""
+    lib = abi.load()
+    nw = np.ascontiguousarray(nw, dtype=np.int32)
+    npt = int(nw.sum())
+    X = np.zeros((npt, 3))
+    q = np.zeros((npt, 4))
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    abi.check(lib.fs_make_filaments(abi.dptr(b), len(nw), abi.iptr(nw), dL,
+                                    min_sep, seed, max_tries, abi.dptr(X),
+                                    abi.dptr(q)))
+    return X, q
+
+
+def device_count():
+    lib = abi.load()
+    n = C.c_int()
+    abi.check(lib.fs_device_count(C.byref(n)))
+    return n.value

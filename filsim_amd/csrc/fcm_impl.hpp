@@ -1,0 +1,47 @@
+// fcm_impl.hpp -- host-side state of one FCM solver (struct fs_fcm).
+#pragma once
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace fs {
+
+// Geometry of the gather spread: grid tiles of tx*ty*tz points and the
+// bins (same tiles) that hold the stencil origins able to reach a tile.
+struct SpreadGeom {
+  int nx, ny, nz, ngd;
+  int tx, ty, tz;     // tile edge
+  int nbx, nby, nbz;  // tiles per dimension
+  int hx, hy, hz;     // neighbouring bins per dimension (besides own)
+};
+
+struct FcmImpl {
+  fs_fcm_params p{};
+  int max_np = 0;
+  int precision = FS_FP64;
+  int tsz = 8;  // bytes per real
+  int last_np = 0;
+  int timing = 0;
+  FftPlan plx{}, ply{}, plz{};
+  fs_fcm_tuning tune{};
+  // spread tile geometry (mirrors SpreadGeom in fcm.hip)
+  SpreadGeom geom{};
+  int nbins = 0;
+  dbuf R, C, twx, twy, twz;
+  dbuf tab, origin, coef, bin, slot, ids, bin_count, bin_start;
+  hipEvent_t ev[8] = {};
+
+  int init(const fs_fcm_params &pp, int maxnp, int prec);
+  int set_geometry();
+  void release();
+  int mobility(int np, const double *Y, const double *F, const double *Tq,
+               double *V, double *W, hipStream_t st);
+  int friction(int np, const double *F, const double *Tq, double *V,
+               double *W, hipStream_t st);
+  int stage_times(float *ms7);
+  template <typename T>
+  int run(int np, const double *Y, const double *F, const double *Tq,
+          double *V, double *W, hipStream_t st);
+};
+
+}  // namespace fs

@@ -1,0 +1,207 @@
+// fft.hpp -- mixed-radix (2,3,4,5,8) Stockham FFT staged in LDS.
+//
+// Replaces FFTW 2.1.5 rfftwnd (FCMfunctions.cpp:16-19, 360-406). A workgroup
+// transforms `ncols` sequences of length n held in LDS; element (m, col) lives
+// at buf[m*sm + col*sc]. Each stage of radix R takes n/R butterflies per
+// sequence: butterfly j reads inputs j + r*n/R, twiddles them by
+// W_{ns R}^{r (j mod ns)} and writes outputs (j/ns)*ns*R + (j mod ns) + r*ns
+// (decimation in time, autosorting: the result is in natural order). Stages
+// ping-pong between two LDS buffers; the returned pointer holds the result.
+// Transforms are unnormalised; sign -1 forward, +1 inverse.
+#pragma once
+
+#include "common.hpp"
+
+namespace fs {
+
+constexpr int kMaxStages = 10;
+
+struct FftPlan {
+  int n;
+  int nst;
+  int rad[kMaxStages];
+};
+
+// Radices for n = 2^a 3^b 5^c, largest power-of-two radices first.
+inline bool make_plan(int n, FftPlan *p) {
+  p->n = n;
+  p->nst = 0;
+  int m = n;
+  while (m % 8 == 0 && m > 8 && p->nst < kMaxStages) {
+    p->rad[p->nst++] = 8;
+    m /= 8;
+  }
+  if (m % 8 == 0 && m == 8) {
+    p->rad[p->nst++] = 8;
+    m = 1;
+  }
+  while (m % 4 == 0 && p->nst < kMaxStages) {
+    p->rad[p->nst++] = 4;
+    m /= 4;
+  }
+  while (m % 2 == 0 && p->nst < kMaxStages) {
+    p->rad[p->nst++] = 2;
+    m /= 2;
+  }
+  while (m % 3 == 0 && p->nst < kMaxStages) {
+    p->rad[p->nst++] = 3;
+    m /= 3;
+  }
+  while (m % 5 == 0 && p->nst < kMaxStages) {
+    p->rad[p->nst++] = 5;
+    m /= 5;
+  }
+  return m == 1;
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> mul_mi(cx<T> a) {  // a * (-i)
+  return {a.y, -a.x};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> mul_pi(cx<T> a) {  // a * (+i)
+  return {-a.y, a.x};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> rot(cx<T> a, bool inv) {  // a * (-/+ i)
+  return inv ? mul_pi(a) : mul_mi(a);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft2(cx<T> *v) {
+  cx<T> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(cx<T> *v, bool inv) {
+  cx<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+  cx<T> s13 = cadd(v[1], v[3]), d13 = rot(csub(v[1], v[3]), inv);
+  v[0] = cadd(s02, s13);
+  v[2] = csub(s02, s13);
+  v[1] = cadd(d02, d13);
+  v[3] = csub(d02, d13);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft8(cx<T> *v, bool inv) {
+  const T h = (T)0.70710678118654752440084436210485;
+  cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+  cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4(e, inv);
+  dft4(o, inv);
+  // W8^k, forward: (1-i)/sqrt2, -i, (-1-i)/sqrt2; inverse conjugated
+  cx<T> o1 = inv ? cx<T>{h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)}
+                 : cx<T>{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
+  cx<T> o2 = rot(o[2], inv);
+  cx<T> o3 = inv ? cx<T>{-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)}
+                 : cx<T>{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft3(cx<T> *v, bool inv) {
+  const T s3 = (T)0.86602540378443864676372317075294;
+  cx<T> t1 = cadd(v[1], v[2]);
+  cx<T> t2 = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
+  cx<T> d = csub(v[1], v[2]);
+  cx<T> t3 = rot(cx<T>{s3 * d.x, s3 * d.y}, inv);
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(t2, t3);
+  v[2] = csub(t2, t3);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft5(cx<T> *v, bool inv) {
+  const T c1 = (T)0.30901699437494742410229341718282;
+  const T c2 = (T)-0.80901699437494742410229341718282;
+  const T s1 = (T)0.95105651629515357211643933337938;
+  const T s2 = (T)0.58778525229247312916870595463907;
+  cx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  cx<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  cx<T> a0 = v[0];
+  cx<T> b1 = {a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y};
+  cx<T> b2 = {a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y};
+  cx<T> d1 = {s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y};
+  cx<T> d2 = {s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y};
+  cx<T> r1 = rot(d1, inv), r2 = rot(d2, inv);
+  v[0] = {a0.x + t1.x + t2.x, a0.y + t1.y + t2.y};
+  v[1] = cadd(b1, r1);
+  v[4] = csub(b1, r1);
+  v[2] = cadd(b2, r2);
+  v[3] = csub(b2, r2);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void fft_stage(const cx<T> *__restrict__ a,
+                                          cx<T> *__restrict__ b, int n, int ns,
+                                          int ncols, int sm, int sc,
+                                          const cx<T> *tw, bool inv) {
+  const int nb = n / R;
+  const int total = nb * ncols;
+  const int twstep = n / (ns * R);
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    int j, col;
+    if (sm == 1) {  // sequences contiguous: butterflies fastest
+      j = t % nb;
+      col = t / nb;
+    } else {  // sequences interleaved: columns fastest
+      col = t % ncols;
+      j = t / ncols;
+    }
+    const int k = j % ns;
+    cx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = a[(j + r * nb) * sm + col * sc];
+    if (ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cx<T> w = tw[r * k * twstep];
+        if (inv) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    if (R == 2) dft2(v);
+    if (R == 3) dft3(v, inv);
+    if (R == 4) dft4(v, inv);
+    if (R == 5) dft5(v, inv);
+    if (R == 8) dft8(v, inv);
+    const int d = (j / ns) * ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) b[(d + r * ns) * sm + col * sc] = v[r];
+  }
+}
+
+// Runs all stages; a and b are LDS buffers of the same shape. Returns the
+// buffer holding the result. tw: LDS or global table exp(-2 pi i m / n).
+template <typename T>
+__device__ cx<T> *lds_fft(cx<T> *a, cx<T> *b, const FftPlan &pl, int ncols,
+                          int sm, int sc, const cx<T> *tw, bool inv) {
+  int ns = 1;
+  for (int s = 0; s < pl.nst; ++s) {
+    const int R = pl.rad[s];
+    switch (R) {
+      case 2: fft_stage<T, 2>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
+      case 3: fft_stage<T, 3>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
+      case 4: fft_stage<T, 4>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
+      case 5: fft_stage<T, 5>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
+      default: fft_stage<T, 8>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
+    }
+    __syncthreads();
+    cx<T> *t = a;
+    a = b;
+    b = t;
+    ns *= R;
+  }
+  return a;
+}
+
+}  // namespace fs

@@ -1,0 +1,1233 @@
+// filament.hip -- filament mechanics, BDF2 residual, approximate Jacobian,
+// bad-Broyden solve and the implicit timestep for gfx950 (FP64 throughout).
+//
+// Replaces class Filament (Filament.hpp:21-1123), class Bead (Bead.hpp),
+// UnitQuaternion (UnitQuaternion.hpp), ErrorCheckFCM
+// (MobilityConstraints.cpp:447-770), filamentJsolver
+// (FilamentJacobianSolver.hpp) and the Broyden loop of main.cpp:284-754.
+// State is structure-of-arrays in HBM, indexed by global bead g (per bead)
+// and global link l (per link; links of filament f start at loff[f]).
+// Kernels:
+//   k_guess       initialGuess + unRobotArmify (one workgroup per filament)
+//   k_update      update(u) + unRobotArmify
+//   k_forces      elastic torques, constraint forces/torques, gravity and
+//                 assign_filament_data (Y = fmod(Xs, L), F, T)
+//   k_residual    ErrorCheckFCM per bead + exact infinity-norm reduction
+//   k_jac_build   per-filament approximate Jacobian (rcross, BlockC/D/E/F)
+//   k_gj_inverse  Gauss-Jordan with partial pivoting, one workgroup per
+//                 filament; writes J^-1 column-major with the lambda-row
+//                 rescale of Filament.hpp:863-865
+//   k_jsolve      block-diagonal J^-1 b, then the rank-one history
+//                 x += sum_i C_i (D_i . b) with deterministic reductions
+//   k_step        Filament::step + Bead::step
+// Every kernel here is compiled with FP contraction off, so each product
+// rounds as in the reference's x86-64 build.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "colbar_impl.hpp"
+#include "common.hpp"
+#include "fcm_impl.hpp"
+#include "system_impl.hpp"
+
+#pragma clang fp contract(off)
+
+namespace fs {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr int kMaxNw = 256;
+
+// ---------------------------------------------------------------------
+// quaternion algebra (UnitQuaternion.hpp), q = [scal, v0, v1, v2]
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void dcross(const double *a, const double *b,
+                                       double *c) {
+  double c0 = a[1] * b[2] - a[2] * b[1];
+  double c1 = a[2] * b[0] - a[0] * b[2];
+  double c2 = a[0] * b[1] - a[1] * b[0];
+  c[0] = c0;
+  c[1] = c1;
+  c[2] = c2;
+}
+__device__ __forceinline__ double ddot(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// p * q in place on q (quaproFromLeftInPlace, UnitQuaternion.hpp:137-143)
+__device__ __forceinline__ void q_from_left(double *q, const double *p) {
+  double s = q[0] * p[0] - ddot(q + 1, p + 1);
+  double t[3];
+  dcross(p + 1, q + 1, t);
+  double v0 = q[0] * p[1] + p[0] * q[1] + t[0];
+  double v1 = q[0] * p[2] + p[0] * q[2] + t[1];
+  double v2 = q[0] * p[3] + p[0] * q[3] + t[2];
+  q[0] = s;
+  q[1] = v0;
+  q[2] = v1;
+  q[3] = v2;
+}
+// (p^*) * q in place (quaproFromLeftInPlaceConj, :146-153)
+__device__ __forceinline__ void q_from_left_conj(double *q, const double *p) {
+  double s = q[0] * p[0] + ddot(q + 1, p + 1);
+  double t[3];
+  dcross(q + 1, p + 1, t);
+  double v0 = -q[0] * p[1] + p[0] * q[1] + t[0];
+  double v1 = -q[0] * p[2] + p[0] * q[2] + t[1];
+  double v2 = -q[0] * p[3] + p[0] * q[3] + t[2];
+  q[0] = s;
+  q[1] = v0;
+  q[2] = v1;
+  q[3] = v2;
+}
+// q * p in place (quaproInPlace, :119-125)
+__device__ __forceinline__ void q_times(double *q, const double *p) {
+  double s = q[0] * p[0] - ddot(q + 1, p + 1);
+  double t[3];
+  dcross(q + 1, p + 1, t);
+  double v0 = q[0] * p[1] + p[0] * q[1] + t[0];
+  double v1 = q[0] * p[2] + p[0] * q[2] + t[1];
+  double v2 = q[0] * p[3] + p[0] * q[3] + t[2];
+  q[0] = s;
+  q[1] = v0;
+  q[2] = v1;
+  q[3] = v2;
+}
+// expmInPlace (:227-242): q <- exp(v) q
+__device__ __forceinline__ void q_expm(double *q, const double *v) {
+  double theta = sqrt(ddot(v, v));
+  if (theta > 1.e-12) {
+    double tmp = sin(theta * .5) / theta;
+    double qu[4] = {cos(theta * .5), tmp * v[0], tmp * v[1], tmp * v[2]};
+    q_from_left(q, qu);
+  }
+}
+// assignMidPointOf (:472-489) with quaSquareRootTimInPlace (:411-421)
+__device__ __forceinline__ void q_midpoint(const double *p1, const double *p2,
+                                           double *out) {
+  double t[3];
+  dcross(p2 + 1, p1 + 1, t);
+  double s = p2[0] * p1[0] + ddot(p2 + 1, p1 + 1);
+  double v0 = -p2[0] * p1[1] + p1[0] * p2[1] - t[0];
+  double v1 = -p2[0] * p1[2] + p1[0] * p2[2] - t[1];
+  double v2 = -p2[0] * p1[3] + p1[0] * p2[3] - t[2];
+  if (s < -.9999999999999) {
+    s = 0.;
+    v0 = 0.;
+    v1 = 0.;
+    v2 = 1.;
+  } else {
+    s = sqrt(0.5 * (s + 1.));
+    v0 /= (2. * s);
+    v1 /= (2. * s);
+    v2 /= (2. * s);
+  }
+  out[0] = s;
+  out[1] = v0;
+  out[2] = v1;
+  out[3] = v2;
+  q_times(out, p1);
+}
+// tangent / normal / binormal (:341-378)
+__device__ __forceinline__ void q_tangent(const double *q, double *v) {
+  double q22 = 2 * q[2] * q[2];
+  double q33 = 2 * q[3] * q[3];
+  v[0] = (1 - q22 - q33);
+  v[1] = 2 * (q[1] * q[2] + q[0] * q[3]);
+  v[2] = 2 * (q[1] * q[3] - q[0] * q[2]);
+}
+__device__ __forceinline__ void q_normal(const double *q, double *v) {
+  double q11 = 2 * q[1] * q[1];
+  double q33 = 2 * q[3] * q[3];
+  v[0] = 2 * (q[1] * q[2] - q[0] * q[3]);
+  v[1] = (1 - q11 - q33);
+  v[2] = 2 * (q[2] * q[3] + q[0] * q[1]);
+}
+__device__ __forceinline__ void q_binormal(const double *q, double *v) {
+  double q11 = 2 * q[1] * q[1];
+  double q22 = 2 * q[2] * q[2];
+  v[0] = 2 * (q[1] * q[3] + q[0] * q[2]);
+  v[1] = 2 * (q[2] * q[3] - q[0] * q[1]);
+  v[2] = (1 - q11 - q22);
+}
+// RotateVector (:323-338)
+__device__ __forceinline__ void q_rotate(const double *q, double *v) {
+  double vx = v[0], vy = v[1], vz = v[2];
+  double q11 = 2 * q[1] * q[1];
+  double q22 = 2 * q[2] * q[2];
+  double q33 = 2 * q[3] * q[3];
+  v[0] = (1 - q22 - q33) * vx + 2 * (q[1] * q[2] - q[0] * q[3]) * vy +
+         2 * (q[1] * q[3] + q[0] * q[2]) * vz;
+  v[1] = 2 * (q[1] * q[2] + q[0] * q[3]) * vx + (1 - q11 - q33) * vy +
+         2 * (q[2] * q[3] - q[0] * q[1]) * vz;
+  v[2] = 2 * (q[1] * q[3] - q[0] * q[2]) * vx +
+         2 * (q[2] * q[3] + q[0] * q[1]) * vy + (1 - q11 - q22) * vz;
+}
+// dexpinv (MobilityConstraints.cpp:109-143)
+__device__ __forceinline__ void d_dexpinv(const double *u, const double *w,
+                                          double *out) {
+  double tmp = sqrt(ddot(u, u));
+  double uw[3];
+  dcross(u, w, uw);
+  if (tmp >= 1.e-12) {
+    double tmp2 = tmp / sin(.5 * tmp);
+    tmp2 *= cos(.5 * tmp);
+    double c = .5 * tmp2 - 1.;
+    double un[3] = {u[0] / tmp, u[1] / tmp, u[2] / tmp};
+    double a[3], b[3];
+    dcross(un, w, a);
+    dcross(un, a, b);
+    for (int d = 0; d < 3; ++d) out[d] = (w[d] - .5 * uw[d]) - c * b[d];
+  } else {
+    double c = 1. / 12;
+    double b[3];
+    dcross(u, uw, b);
+    for (int d = 0; d < 3; ++d) out[d] = (w[d] - .5 * uw[d]) + c * b[d];
+  }
+}
+
+// ---------------------------------------------------------------------
+// kinematics
+// ---------------------------------------------------------------------
+// unRobotArmify (Filament.hpp:489-505), run by one lane.
+__device__ void unrobotarmify(const SysDev &S, int b0, int nw) {
+  double Xsum[3] = {S.Xs[3 * b0], S.Xs[3 * b0 + 1], S.Xs[3 * b0 + 2]};
+  double t1[3], t2[3];
+  q_tangent(S.qs + 4 * b0, t1);
+  for (int i = 1; i < nw; ++i) {
+    q_tangent(S.qs + 4 * (b0 + i), t2);
+    for (int d = 0; d < 3; ++d) {
+      Xsum[d] += (t1[d] + t2[d]) * (0.5 * S.dL);
+      S.Xs[3 * (b0 + i) + d] = Xsum[d];
+      t1[d] = t2[d];
+    }
+  }
+}
+
+// initialGuess (Filament.hpp:227-237, Bead.hpp:202-217) + unRobotArmify
+__global__ void k_guess(SysDev S) {
+  const int f = blockIdx.x;
+  const int b0 = S.boff[f], nw = S.nw[f];
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+    int g = b0 + i;
+    if (i == 0)
+      for (int d = 0; d < 3; ++d)
+        S.Xs[3 * g + d] = 2. * S.X[3 * g + d] - S.Xt[3 * g + d];
+    for (int d = 0; d < 3; ++d)
+      S.Us[3 * g + d] = 2. * S.U[3 * g + d] - S.Ut[3 * g + d];
+    double q[4] = {S.q[4 * g], S.q[4 * g + 1], S.q[4 * g + 2], S.q[4 * g + 3]};
+    q_expm(q, S.Us + 3 * g);
+    for (int c = 0; c < 4; ++c) S.qs[4 * g + c] = q[c];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) unrobotarmify(S, b0, nw);
+}
+
+// Filament::update (Filament.hpp:425-452, Bead.hpp:236-256) + unRobotArmify
+__global__ void k_update(SysDev S, const double *__restrict__ u) {
+  const int f = blockIdx.x;
+  const int b0 = S.boff[f], l0 = S.loff[f], nw = S.nw[f];
+  const size_t so = 6 * (size_t)b0;
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+    int g = b0 + i;
+    if (i == 0) {
+      for (int d = 0; d < 3; ++d) {
+        S.Xs[3 * g + d] -= u[so + d];
+        S.Us[3 * g + d] -= u[so + 3 + d];
+      }
+    } else {
+      for (int d = 0; d < 3; ++d) S.Us[3 * g + d] -= u[so + 6 + 3 * (i - 1) + d];
+    }
+    double q[4] = {S.q[4 * g], S.q[4 * g + 1], S.q[4 * g + 2], S.q[4 * g + 3]};
+    q_expm(q, S.Us + 3 * g);
+    for (int c = 0; c < 4; ++c) S.qs[4 * g + c] = q[c];
+  }
+  for (int l = threadIdx.x; l < nw - 1; l += blockDim.x)
+    for (int d = 0; d < 3; ++d)
+      S.lam[3 * (l0 + l) + d] -= u[so + 3 * nw + 3 + 3 * l + d];
+  __syncthreads();
+  if (threadIdx.x == 0) unrobotarmify(S, b0, nw);
+}
+
+// setZeroForcesTorques + applyElasticTorques + applyConstraintForcesTorques
+// + applyExternalForcesTorques (Filament.hpp:271-402) + assign_filament_data
+// (FCMfunctions.cpp:882-918). One workgroup per filament.
+__global__ void k_forces(SysDev S, int nt) {
+  __shared__ double s_v[kMaxNw][3];  // elastic torque of link l
+  __shared__ double s_t[kMaxNw][3];  // tangent of bead i
+  const int f = blockIdx.x;
+  const int b0 = S.boff[f], l0 = S.loff[f], nw = S.nw[f];
+  const fs_fil_props pr = S.props[f];
+  const double dL = S.dL;
+  const double tt = nt * S.dt;
+  const double Lf = (nw - 1) * dL;
+  for (int i = threadIdx.x; i < nw - 1; i += blockDim.x) {
+    double T0 = pr.T0;
+    double s = dL * i + .5 * dL;
+    double facPref;
+    if (S.sperm_decay) {
+      if (s > Lf / 2.0)
+        facPref = T0 * 2.0 * (Lf - s) / Lf;
+      else
+        facPref = T0;
+    } else {
+      facPref = T0 * (Lf - s) / Lf;
+    }
+    const double *qi = S.qs + 4 * (b0 + i), *qj = S.qs + 4 * (b0 + i + 1);
+    double qm[4], q2[4];
+    q_midpoint(qi, qj, qm);
+    for (int c = 0; c < 4; ++c) q2[c] = (qj[c] - qi[c]) * (1. / dL);
+    q_from_left_conj(q2, qm);
+    double ph = pr.wavenumber * s - pr.omega * tt + pr.phase;
+    double st1 = pr.beta * facPref * cos(ph);
+    double st2 = pr.alpha * facPref * sin(ph);
+    double v[3] = {2. * q2[1] - S.st0, 2. * q2[2] - st1, 2. * q2[3] - st2};
+    v[0] *= pr.c;
+    v[1] *= pr.kap;
+    v[2] *= pr.kap;
+    q_rotate(qm, v);
+    s_v[i][0] = v[0];
+    s_v[i][1] = v[1];
+    s_v[i][2] = v[2];
+    if (i == nw - 2) {  // myStrainTwist keeps the last link's values
+      S.st[3 * f + 0] = S.st0;
+      S.st[3 * f + 1] = st1;
+      S.st[3 * f + 2] = st2;
+    }
+  }
+  for (int i = threadIdx.x; i < nw; i += blockDim.x)
+    q_tangent(S.qs + 4 * (b0 + i), s_t[i]);
+  __syncthreads();
+  const double fac = -.5 * dL;
+  const double Lb[3] = {S.Lx, S.Ly, S.Lz};
+  for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+    int g = b0 + k;
+    double tau[3] = {0, 0, 0}, F[3] = {0, 0, 0};
+    // elastic: link k-1 (as j) then link k (as i)
+    if (k > 0)
+      for (int d = 0; d < 3; ++d) tau[d] -= s_v[k - 1][d];
+    if (k < nw - 1)
+      for (int d = 0; d < 3; ++d) tau[d] += s_v[k][d];
+    // constraint: link k-1 (as j) then link k (as i)
+    double c[3];
+    if (k > 0) {
+      const double *lm = S.lam + 3 * (l0 + k - 1);
+      for (int d = 0; d < 3; ++d) F[d] += lm[d];
+      dcross(s_t[k], lm, c);
+      for (int d = 0; d < 3; ++d) tau[d] += fac * c[d];
+    }
+    if (k < nw - 1) {
+      const double *lm = S.lam + 3 * (l0 + k);
+      for (int d = 0; d < 3; ++d) F[d] -= lm[d];
+      dcross(s_t[k], lm, c);
+      for (int d = 0; d < 3; ++d) tau[d] += fac * c[d];
+    }
+    for (int d = 0; d < 3; ++d) {
+      F[d] += S.weight[d] * dL;
+      S.F[3 * g + d] = F[d];
+      S.TAU[3 * g + d] = tau[d];
+      S.Ya[3 * g + d] = fmod(S.Xs[3 * g + d], Lb[d]);
+      S.Fa[3 * g + d] = F[d];
+      S.Ta[3 * g + d] = tau[d];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// residual (ErrorCheckFCM, MobilityConstraints.cpp:447-770)
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_residual(SysDev S, int nt, const double *__restrict__ V,
+               const double *__restrict__ W, double *__restrict__ err,
+               unsigned long long *__restrict__ maxbits) {
+  __shared__ double s_m[256];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  double mx = 0.0;
+  if (g < S.np) {
+    const int f = S.sfil[g], i = S.sseg[g];
+    const int nw = S.nw[f];
+    const size_t k0 = 6 * (size_t)S.boff[f];
+    const double dt = S.dt;
+    const bool euler = nt <= S.impl_euler_steps;
+    const double fourthirds = 4. / 3., onethird = 1. / 3.,
+                 twothirdsDt = dt * 2. / 3.;
+    double dw[3];
+    d_dexpinv(S.Us + 3 * g, W + 3 * g, dw);
+    for (int d = 0; d < 3; ++d) {
+      double e1, e2;
+      if (euler) {
+        e1 = S.Xs[3 * g + d] - S.X[3 * g + d] - dt * V[3 * g + d];
+        e2 = S.Us[3 * g + d] - dt * dw[d];
+      } else {
+        e1 = S.Xs[3 * g + d] - fourthirds * S.X[3 * g + d] +
+             onethird * S.Xt[3 * g + d] - twothirdsDt * V[3 * g + d];
+        e2 = S.Us[3 * g + d] - onethird * S.U[3 * g + d] - twothirdsDt * dw[d];
+      }
+      err[k0 + 3 * i + d] = e1;
+      err[k0 + 3 * nw + 3 * i + d] = e2;
+      mx = fmax(mx, fmax(fabs(e1), fabs(e2)));
+    }
+  }
+  s_m[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      s_m[threadIdx.x] = fmax(s_m[threadIdx.x], s_m[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    atomicMax(maxbits, (unsigned long long)__double_as_longlong(s_m[0]));
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+__global__ void k_fill_i32(int *p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------
+// Jacobian (Filament.hpp:803-1120)
+// ---------------------------------------------------------------------
+struct M3 {
+  double a[3][3];
+};
+__device__ __forceinline__ M3 rcross(const double *x) {
+  M3 A;
+  A.a[0][0] = 0.;
+  A.a[0][1] = x[2];
+  A.a[0][2] = -x[1];
+  A.a[1][0] = -x[2];
+  A.a[1][1] = 0.;
+  A.a[1][2] = x[0];
+  A.a[2][0] = x[1];
+  A.a[2][1] = -x[0];
+  A.a[2][2] = 0.;
+  return A;
+}
+__device__ __forceinline__ M3 mm(const M3 &A, const M3 &B) {
+  M3 C;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      C.a[i][j] = A.a[i][0] * B.a[0][j] + A.a[i][1] * B.a[1][j] +
+                  A.a[i][2] * B.a[2][j];
+  return C;
+}
+__device__ __forceinline__ M3 tr(const M3 &A) {
+  M3 B;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) B.a[i][j] = A.a[j][i];
+  return B;
+}
+__device__ __forceinline__ M3 outer(const double *u, const double *v) {
+  M3 C;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C.a[i][j] = u[i] * v[j];
+  return C;
+}
+__device__ __forceinline__ void put3(double *J, int n, int r, int c,
+                                     const M3 &B) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) J[(size_t)(r + i) * n + c + j] = B.a[i][j];
+}
+__device__ __forceinline__ M3 diag_block(double dt, const M3 &dO, const M3 &LM,
+                                         const M3 &OM, const double *Us) {
+  M3 LdO = mm(LM, dO), LLdO = mm(LM, LdO), LOM = mm(LM, OM);
+  double w[3];
+  for (int i = 0; i < 3; ++i)
+    w[i] = OM.a[i][0] * Us[0] + OM.a[i][1] * Us[1] + OM.a[i][2] * Us[2];
+  M3 R = rcross(w), out;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      out.a[i][j] = -2 * dt *
+                    (dO.a[i][j] - 0.5 * (OM.a[i][j] + LdO.a[i][j]) +
+                     (R.a[i][j] + LOM.a[i][j] + LLdO.a[i][j]) / 12.) /
+                    3.;
+  return out;
+}
+__device__ __forceinline__ M3 off_block(double dt, const M3 &dO, const M3 &LM) {
+  M3 LdO = mm(LM, dO), LLdO = mm(LM, LdO), out;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      out.a[i][j] =
+          -2 * dt * (dO.a[i][j] - 0.5 * LdO.a[i][j] + LLdO.a[i][j] / 12.) / 3.;
+  return out;
+}
+
+struct Frame {
+  double t[3], n[3], b[3];
+};
+__device__ __forceinline__ Frame get_frame(const double *q) {
+  Frame F;
+  q_tangent(q, F.t);
+  q_normal(q, F.n);
+  q_binormal(q, F.b);
+  return F;
+}
+
+// BlockE rows of bead k (diagonal and neighbour blocks).
+__device__ void jac_blockE(const SysDev &S, double *J, int n, int f, int k,
+                           int nw, int b0, int l0) {
+  const fs_fil_props pr = S.props[f];
+  const double dt = S.dt, DL = S.dL, MU = S.mu;
+  const double myC = pr.c, myKAP = pr.kap;
+  const double *ST = S.st + 3 * f;
+  const int R0 = 3 * nw, C0 = 3;
+  const double Tfac = 1 / (8 * kPi * MU * pow(pr.radius, 3.));
+  M3 LM = tr(rcross(S.Us + 3 * (b0 + k)));
+  double tau[3];
+  for (int d = 0; d < 3; ++d) tau[d] = Tfac * S.TAU[3 * (b0 + k) + d];
+  M3 OM = rcross(tau);
+  Frame f0 = get_frame(S.qs + 4 * (b0 + k));
+  M3 rt = rcross(f0.t), rn = rcross(f0.n), rb = rcross(f0.b);
+  M3 EP, dO, Blk;
+  if (k == 0) {
+    Frame fp1 = get_frame(S.qs + 4 * (b0 + 1));
+    M3 rl = rcross(S.lam + 3 * (l0 + 0));
+    M3 CP = mm(rl, rt);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) CP.a[i][j] = -0.5 * DL * CP.a[i][j];
+    double beta = -ST[2] + 0.5 * (ddot(fp1.n, f0.b) - ddot(f0.n, fp1.b)) / DL;
+    double c1[3], c2[3], cp[3], ts[3];
+    dcross(f0.b, fp1.n, c1);
+    dcross(f0.n, fp1.b, c2);
+    for (int d = 0; d < 3; ++d) {
+      cp[d] = c1[d] - c2[d];
+      ts[d] = 0.5 * (f0.t[d] + fp1.t[d]);
+    }
+    M3 tmp1 = outer(ts, cp);
+    M3 rtp1 = rcross(fp1.t);
+    M3 tmp2 = mm(rtp1, rt);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        EP.a[i][j] = 0.5 * myC * (tmp1.a[i][j] / DL + beta * rt.a[i][j]);
+        EP.a[i][j] += 0.5 * myKAP *
+                      (tmp2.a[i][j] / DL -
+                       0.5 * (ST[0] * rn.a[i][j] + ST[1] * rb.a[i][j]));
+        dO.a[i][j] = Tfac * (EP.a[i][j] + CP.a[i][j]);
+      }
+    Blk = diag_block(dt, dO, LM, OM, S.Us + 3 * (b0 + k));
+    put3(J, n, R0, C0, Blk);
+    M3 rnp1 = rcross(fp1.n), rbp1 = rcross(fp1.b);
+    tmp2 = mm(tr(rt), rtp1);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        EP.a[i][j] = 0.5 * myC * (beta * rtp1.a[i][j] - tmp1.a[i][j] / DL);
+        EP.a[i][j] += 0.5 * myKAP *
+                      (tmp2.a[i][j] / DL -
+                       0.5 * (ST[0] * rnp1.a[i][j] + ST[1] * rbp1.a[i][j]));
+        dO.a[i][j] = Tfac * EP.a[i][j];
+      }
+    Blk = off_block(dt, dO, LM);
+    put3(J, n, R0, C0 + 3, Blk);
+  } else if (k == nw - 1) {
+    Frame fm1 = get_frame(S.qs + 4 * (b0 + k - 1));
+    M3 rl = rcross(S.lam + 3 * (l0 + nw - 2));
+    M3 CP = mm(rl, rt);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) CP.a[i][j] = -0.5 * DL * CP.a[i][j];
+    double beta = -ST[2] + 0.5 * (ddot(f0.n, fm1.b) - ddot(fm1.n, f0.b)) / DL;
+    double c1[3], c2[3], cp[3], ts[3];
+    dcross(fm1.b, f0.n, c1);
+    dcross(fm1.n, f0.b, c2);
+    for (int d = 0; d < 3; ++d) {
+      cp[d] = c1[d] - c2[d];
+      ts[d] = 0.5 * (f0.t[d] + fm1.t[d]);
+    }
+    M3 tmp1 = outer(ts, cp);
+    M3 rtm1 = rcross(fm1.t);
+    M3 tmp2 = mm(tr(rtm1), rt);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        EP.a[i][j] = -0.5 * myC * (beta * rt.a[i][j] - tmp1.a[i][j] / DL);
+        EP.a[i][j] -= 0.5 * myKAP *
+                      (tmp2.a[i][j] / DL -
+                       0.5 * (ST[0] * rn.a[i][j] + ST[1] * rb.a[i][j]));
+        dO.a[i][j] = Tfac * (EP.a[i][j] + CP.a[i][j]);
+      }
+    Blk = diag_block(dt, dO, LM, OM, S.Us + 3 * (b0 + k));
+    put3(J, n, R0 + 3 * k, C0 + 3 * k, Blk);
+    M3 rnm1 = rcross(fm1.n), rbm1 = rcross(fm1.b);
+    tmp2 = mm(rt, rtm1);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        EP.a[i][j] = -0.5 * myC * (beta * rtm1.a[i][j] - tmp1.a[i][j] / DL);
+        EP.a[i][j] -= 0.5 * myKAP *
+                      (tmp2.a[i][j] / DL -
+                       0.5 * (ST[0] * rnm1.a[i][j] + ST[1] * rbm1.a[i][j]));
+        dO.a[i][j] = Tfac * EP.a[i][j];
+      }
+    Blk = off_block(dt, dO, LM);
+    put3(J, n, R0 + 3 * k, C0 + 3 * k - 3, Blk);
+  } else {
+    Frame fm1 = get_frame(S.qs + 4 * (b0 + k - 1));
+    Frame fp1 = get_frame(S.qs + 4 * (b0 + k + 1));
+    double lv[3];
+    for (int d = 0; d < 3; ++d)
+      lv[d] = S.lam[3 * (l0 + k - 1) + d] + S.lam[3 * (l0 + k) + d];
+    M3 rl = rcross(lv);
+    M3 CP = mm(rl, rt);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) CP.a[i][j] = -0.5 * DL * CP.a[i][j];
+    double beta = -ST[2] + 0.5 * (ddot(fp1.n, f0.b) - ddot(f0.n, fp1.b)) / DL;
+    double c1[3], c2[3], cp[3], ts[3], tsm[3];
+    dcross(f0.b, fp1.n, c1);
+    dcross(f0.n, fp1.b, c2);
+    for (int d = 0; d < 3; ++d) {
+      cp[d] = c1[d] - c2[d];
+      ts[d] = 0.5 * (f0.t[d] + fp1.t[d]);
+      tsm[d] = 0.5 * (f0.t[d] + fm1.t[d]);
+    }
+    M3 tmp1 = outer(ts, cp), tmp3 = outer(tsm, cp);
+    M3 rtp1 = rcross(fp1.t), rtm1 = rcross(fm1.t);
+    M3 tmp2 = mm(rtp1, rt);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        EP.a[i][j] = 0.5 * myC * (tmp1.a[i][j] / DL + beta * rt.a[i][j]);
+        EP.a[i][j] += 0.5 * myKAP *
+                      (tmp2.a[i][j] / DL -
+                       0.5 * (ST[0] * rn.a[i][j] + ST[1] * rb.a[i][j]));
+      }
+    // backward side: beta recomputed, cross part reused (Filament.hpp:1065)
+    beta = -ST[2] + 0.5 * (ddot(f0.n, fm1.b) - ddot(fm1.n, f0.b)) / DL;
+    tmp2 = mm(tr(rtm1), rt);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        EP.a[i][j] -= 0.5 * myC * (beta * rt.a[i][j] - tmp3.a[i][j] / DL);
+        EP.a[i][j] -= 0.5 * myKAP *
+                      (tmp2.a[i][j] / DL -
+                       0.5 * (ST[0] * rn.a[i][j] + ST[1] * rb.a[i][j]));
+        dO.a[i][j] = Tfac * (CP.a[i][j] + EP.a[i][j]);
+      }
+    Blk = diag_block(dt, dO, LM, OM, S.Us + 3 * (b0 + k));
+    put3(J, n, R0 + 3 * k, C0 + 3 * k, Blk);
+    M3 rnm1 = rcross(fm1.n), rbm1 = rcross(fm1.b);
+    tmp2 = mm(rt, rtm1);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        EP.a[i][j] = -0.5 * myC * (beta * rtm1.a[i][j] - tmp3.a[i][j] / DL);
+        EP.a[i][j] -= 0.5 * myKAP *
+                      (tmp2.a[i][j] / DL -
+                       0.5 * (ST[0] * rnm1.a[i][j] + ST[1] * rbm1.a[i][j]));
+        dO.a[i][j] = Tfac * EP.a[i][j];
+      }
+    Blk = off_block(dt, dO, LM);
+    put3(J, n, R0 + 3 * k, C0 + 3 * k - 3, Blk);
+    M3 rnp1 = rcross(fp1.n), rbp1 = rcross(fp1.b);
+    tmp2 = mm(tr(rt), rtp1);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        EP.a[i][j] = 0.5 * myC * (beta * rtp1.a[i][j] - tmp1.a[i][j] / DL);
+        EP.a[i][j] += 0.5 * myKAP *
+                      (tmp2.a[i][j] / DL -
+                       0.5 * (ST[0] * rnp1.a[i][j] + ST[1] * rbp1.a[i][j]));
+        dO.a[i][j] = Tfac * EP.a[i][j];
+      }
+    Blk = off_block(dt, dO, LM);
+    put3(J, n, R0 + 3 * k, C0 + 3 * k + 3, Blk);
+  }
+  for (int d = 0; d < 3; ++d)
+    J[(size_t)(R0 + 3 * k + d) * n + C0 + 3 * k + d] += 1.;
+}
+
+__global__ void k_fill_f64(double *p, size_t n, double v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// buildApproximateJacobian (Filament.hpp:803-867), one workgroup per
+// filament, one lane per bead / link.
+__global__ void k_jac_build(SysDev S, int nt, double *__restrict__ Jall) {
+  const int f = blockIdx.x;
+  const int b0 = S.boff[f], l0 = S.loff[f], nw = S.nw[f];
+  const int n = 6 * nw;
+  double *J = Jall + S.joff[f];
+  const fs_fil_props pr = S.props[f];
+  const double dt = S.dt, DL = S.dL, MU = S.mu;
+  const double rad = pr.radius;
+  if (threadIdx.x == 0) {
+    int jj = 3 * nw + 3;
+    double temp = dt / (9 * kPi * MU * rad);
+    for (int d = 0; d < 3; ++d) {
+      J[(size_t)d * n + d] = 1;
+      J[(size_t)d * n + jj + d] = temp / dt;
+    }
+  }
+  const double Jprefac = 10.;
+  // BlockC(0.5 DL): row block r (bead r+1) = [f R(t0), 2f R(t1) .. 2f R(tr),
+  // f R(t_{r+1}), 0 ..]: the reference's cumulative copy (Filament.hpp:
+  // 885-909) sums each shared term twice, exactly.
+  const double fac = 0.5 * DL;
+  for (int r = threadIdx.x; r < nw - 1; r += blockDim.x) {
+    for (int c = 0; c <= r + 1; ++c) {
+      double t[3];
+      q_tangent(S.qs + 4 * (b0 + c), t);
+      M3 Rt = rcross(t);
+      M3 B;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double v = fac * Rt.a[i][j];
+          B.a[i][j] = (c >= 1 && c <= r) ? v + v : v;
+        }
+      put3(J, n, 3 + 3 * r, 3 + 3 * c, B);
+    }
+    // Jprefac * BlockD(nt) / dt (Filament.hpp:913-946)
+    int m = r;
+    double facn = -2. * dt / (18. * kPi * MU * rad);
+    if (nt <= S.impl_euler_steps) facn *= (3. / 2.);
+    int i = 3 * m;
+    for (int d = 0; d < 3; ++d)
+      J[(size_t)(3 + i + d) * n + 3 * nw + 3 + i + d] = Jprefac * facn / dt;
+    if (m < nw - 2) {
+      facn = 2. * dt / (18. * kPi * MU * rad);
+      if (nt <= S.impl_euler_steps) facn *= (3. / 2.);
+      for (int d = 0; d < 3; ++d)
+        J[(size_t)(3 + i + d) * n + 3 * nw + 3 + i + 3 + d] = Jprefac * facn / dt;
+    }
+    // BlockF / dt (Filament.hpp:1098-1120): link r couples beads r, r+1
+    for (int side = 0; side < 2; ++side) {
+      int bead = r + side;
+      double fc = (side == 0) ? dt * DL / (24 * kPi * MU * pow(rad, 3.))
+                              : dt * DL / (24. * kPi * MU * pow(rad, 3));
+      double tv[3];
+      q_tangent(S.qs + 4 * (b0 + bead), tv);
+      M3 tmat = tr(rcross(tv));
+      M3 umat = rcross(S.Us + 3 * (b0 + bead));
+      M3 um = mm(umat, tmat);
+      M3 uu = mm(umat, um);
+      M3 B;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          B.a[a][b] = fc * (tmat.a[a][b] - 0.5 * um.a[a][b] + uu.a[a][b] / 12.) / dt;
+      put3(J, n, 3 * nw + 3 * r + 3 * side, 3 * nw + 3 + 3 * r, B);
+    }
+  }
+  for (int k = threadIdx.x; k < nw; k += blockDim.x)
+    jac_blockE(S, J, n, f, k, nw, b0, l0);
+}
+
+// In-place Gauss-Jordan with partial pivoting (pivot rows are not moved:
+// the step that pivots on row p_k is unscrambled at the end as
+// A^-1[k][p_m] = M[p_k][m]). Output written column-major (Jinv^T) with the
+// lambda rows scaled by 1/dt.
+__global__ void __launch_bounds__(1024)
+    k_gj_inverse(SysDev S, double *__restrict__ Jall,
+                 double *__restrict__ JTall, int *__restrict__ singular) {
+  __shared__ double colbuf[6 * kMaxNw / 4 * 4];
+  __shared__ double rowbuf[6 * kMaxNw / 4 * 4];
+  __shared__ int used[6 * kMaxNw / 4 * 4], pivrow[6 * kMaxNw / 4 * 4],
+      rowstep[6 * kMaxNw / 4 * 4];
+  __shared__ int s_p;
+  const int f = blockIdx.x;
+  const int nw = S.nw[f];
+  const int n = 6 * nw;
+  double *M = Jall + S.joff[f];
+  double *JT = JTall + S.joff[f];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += blockDim.x) used[i] = 0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    for (int i = tid; i < n; i += blockDim.x) colbuf[i] = M[(size_t)i * n + k];
+    __syncthreads();
+    if (tid < 64) {
+      double best = -1.0;
+      int bi = n;
+      for (int i = tid; i < n; i += 64) {
+        double v = used[i] ? -1.0 : fabs(colbuf[i]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        double ob = __shfl_xor(best, off, 64);
+        int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (tid == 0) {
+        if (!(best > 0.0)) atomicExch(singular, 1);
+        s_p = bi;
+        used[bi] = 1;
+        pivrow[k] = bi;
+        rowstep[bi] = k;
+      }
+    }
+    __syncthreads();
+    const int p = s_p;
+    for (int j = tid; j < n; j += blockDim.x) rowbuf[j] = M[(size_t)p * n + j];
+    __syncthreads();
+    const double inv = 1.0 / rowbuf[k];
+    for (int e = tid; e < n * n; e += blockDim.x) {
+      int i = e / n, j = e - i * n;
+      if (i == p) {
+        M[e] = (j == k) ? inv : rowbuf[j] * inv;
+      } else {
+        double fct = colbuf[i];
+        if (j == k)
+          M[e] = -fct * inv;
+        else if (fct != 0.0)
+          M[e] = M[e] - fct * (rowbuf[j] * inv);
+      }
+    }
+    __syncthreads();
+  }
+  const double idt = 1.0 / S.dt;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    int i = e / n, j = e - i * n;
+    int r = rowstep[i], c = pivrow[j];
+    double v = M[e];
+    if (r >= 3 * nw + 3) v /= S.dt;
+    JT[(size_t)c * n + r] = v;
+  }
+  (void)idt;
+}
+
+// ---------------------------------------------------------------------
+// Broyden solve (FilamentJacobianSolver.hpp:49-73)
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_jsolve_block(SysDev S, const double *__restrict__ JTall,
+                   const double *__restrict__ b, double *__restrict__ x) {
+  __shared__ double sb[6 * kMaxNw];
+  const int f = blockIdx.x;
+  const int nw = S.nw[f], n = 6 * nw;
+  const size_t so = 6 * (size_t)S.boff[f];
+  const double *JT = JTall + S.joff[f];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) sb[i] = b[so + i];
+  __syncthreads();
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    double s = 0;
+    for (int c = 0; c < n; ++c) s += JT[(size_t)c * n + r] * sb[c];
+    x[so + r] = s;
+  }
+}
+
+// partial[blk * ncol + i] = sum over rows of chunk blk of A_i[r] * b[r]
+// (A column-major, column stride lda). Fixed partition: deterministic.
+__global__ void __launch_bounds__(256)
+    k_dots_partial(const double *__restrict__ A, size_t lda, int ncol,
+                   const double *__restrict__ b, size_t nrow, int chunk,
+                   double *__restrict__ partial) {
+  __shared__ double red[256];
+  const int blk = blockIdx.x, i = blockIdx.y;
+  const size_t lo = (size_t)blk * chunk;
+  const size_t hi = min(nrow, lo + chunk);
+  const double *Ai = A + lda * i;
+  double s = 0;
+  for (size_t r = lo + threadIdx.x; r < hi; r += 256) s += Ai[r] * b[r];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(size_t)blk * ncol + i] = red[0];
+}
+
+__global__ void k_dots_final(const double *__restrict__ partial, int nblk,
+                             int ncol, double *__restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncol) return;
+  double s = 0;
+  for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * ncol + i];
+  out[i] = s;
+}
+
+__global__ void k_hist_axpy(const double *__restrict__ C, size_t lda, int ncol,
+                            const double *__restrict__ coef, size_t nrow,
+                            double *__restrict__ x) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow) return;
+  double v = x[r];
+  for (int i = 0; i < ncol; ++i) v += C[lda * i + r] * coef[i];
+  x[r] = v;
+}
+
+__global__ void k_sub(const double *__restrict__ a, const double *__restrict__ b,
+                      double *__restrict__ c, size_t n) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) c[r] = a[r] - b[r];
+}
+
+// bad-Broyden columns (main.cpp:640-663)
+__global__ void k_broyden_cols(const double *__restrict__ tmp,
+                               const double *__restrict__ upd,
+                               const double *__restrict__ dE,
+                               const double *__restrict__ nrm2, size_t n,
+                               double *__restrict__ Ccol,
+                               double *__restrict__ Dcol) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double nrm = sqrt(*nrm2);
+  double t = tmp[r] * (-1. / nrm);
+  t -= upd[r] / nrm;
+  Ccol[r] = t;
+  Dcol[r] = dE[r] / nrm;
+}
+
+// Filament::step (Filament.hpp:455-478) + Bead::step (Bead.hpp:266-276)
+__global__ void k_step(SysDev S, int nt) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (size_t)S.np) {
+    size_t g = t;
+    for (int d = 0; d < 3; ++d) {
+      S.Xt[3 * g + d] = S.X[3 * g + d];
+      S.X[3 * g + d] = S.Xs[3 * g + d];
+      S.Ut[3 * g + d] = S.U[3 * g + d];
+      S.U[3 * g + d] = S.Us[3 * g + d];
+    }
+    double q[4];
+    for (int c = 0; c < 4; ++c) {
+      S.qt[4 * g + c] = S.q[4 * g + c];
+      q[c] = S.q[4 * g + c];
+    }
+    q_expm(q, S.U + 3 * g);
+    for (int c = 0; c < 4; ++c) S.q[4 * g + c] = q[c];
+  }
+  if (t < 3 * (size_t)S.nlam) {
+    size_t l = t;
+    if (nt > 12) {
+      double v = 3 * S.lam[l] - 3 * S.lam1[l] + S.lam2[l];
+      S.lam2[l] = S.lam1[l];
+      S.lam1[l] = S.lam[l];
+      S.lam[l] = v;
+    } else if (nt == 12) {
+      double v = 2. * S.lam[l] - S.lam1[l];
+      S.lam2[l] = S.lam1[l];
+      S.lam1[l] = S.lam[l];
+      S.lam[l] = v;
+    } else {
+      S.lam1[l] = S.lam[l];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------
+int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
+                     const fs_fil_props *props) {
+  p = pp;
+  FS_CHECK(nfil >= 1, FS_ERR_ARG, "nfil must be >= 1");
+  FS_CHECK(p.dt > 0 && p.dL > 0 && p.tol > 0, FS_ERR_ARG,
+           "dt, dL, tol must be positive");
+  FS_CHECK(p.broyden_maxiter >= 1, FS_ERR_ARG, "broyden_maxiter must be >= 1");
+  nf = nfil;
+  hnw.assign(nwv, nwv + nfil);
+  hboff.resize(nfil + 1);
+  hloff.resize(nfil + 1);
+  hjoff.resize(nfil + 1);
+  hboff[0] = hloff[0] = 0;
+  hjoff[0] = 0;
+  for (int f = 0; f < nfil; ++f) {
+    FS_CHECK(nwv[f] >= 2 && nwv[f] <= kMaxNw, FS_ERR_ARG,
+             "filament %d: %d segments (need 2..%d)", f, nwv[f], kMaxNw);
+    hboff[f + 1] = hboff[f] + nwv[f];
+    hloff[f + 1] = hloff[f] + nwv[f] - 1;
+    hjoff[f + 1] = hjoff[f] + (long)(6 * nwv[f]) * (6 * nwv[f]);
+  }
+  np = hboff[nfil];
+  nlam = hloff[nfil];
+  std::vector<int> sf(np), ss(np);
+  for (int f = 0; f < nfil; ++f)
+    for (int i = 0; i < nwv[f]; ++i) {
+      sf[hboff[f] + i] = f;
+      ss[hboff[f] + i] = i;
+    }
+  const size_t n3 = 3 * (size_t)np, n4 = 4 * (size_t)np, l3 = 3 * (size_t)nlam;
+  const size_t n6 = 6 * (size_t)np;
+  // one arena for the per-bead/per-link doubles
+  size_t nd = 6 * n3 + 3 * n4 + 2 * n3 + 3 * l3 + 3 * (size_t)nf + 5 * n3 +
+              6 * n6;
+  FS_TRY(arena.alloc(sizeof(double) * nd));
+  double *a = arena.as<double>();
+  auto take = [&](size_t k) {
+    double *r = a;
+    a += k;
+    return r;
+  };
+  d.X = take(n3);
+  d.Xs = take(n3);
+  d.Xt = take(n3);
+  d.U = take(n3);
+  d.Us = take(n3);
+  d.Ut = take(n3);
+  d.q = take(n4);
+  d.qs = take(n4);
+  d.qt = take(n4);
+  d.F = take(n3);
+  d.TAU = take(n3);
+  d.lam = take(l3);
+  d.lam1 = take(l3);
+  d.lam2 = take(l3);
+  d.st = take(3 * (size_t)nf);
+  d.Ya = take(n3);
+  d.Fa = take(n3);
+  d.Ta = take(n3);
+  d.V = take(n3);
+  d.W = take(n3);
+  E = take(n6);
+  NewE = take(n6);
+  Upd = take(n6);
+  tmp = take(n6);
+  dE = take(n6);
+  scratch6 = take(n6);
+  FS_HIP(hipMemset(arena.p, 0, sizeof(double) * nd));
+  // identity quaternions
+  std::vector<double> qi(n4, 0.0);
+  for (int g = 0; g < np; ++g) qi[4 * g] = 1.0;
+  FS_HIP(hipMemcpy(d.q, qi.data(), sizeof(double) * n4, hipMemcpyHostToDevice));
+  FS_HIP(hipMemcpy(d.qs, qi.data(), sizeof(double) * n4, hipMemcpyHostToDevice));
+  FS_HIP(hipMemcpy(d.qt, qi.data(), sizeof(double) * n4, hipMemcpyHostToDevice));
+  std::vector<double> st0(3 * (size_t)nf, 0.0);
+  for (int f = 0; f < nf; ++f) st0[3 * f] = p.strain_twist0;
+  FS_HIP(hipMemcpy(d.st, st0.data(), sizeof(double) * 3 * nf, hipMemcpyHostToDevice));
+  FS_TRY(ints.alloc(sizeof(int) * (3 * (size_t)(nf + 1) + 2 * (size_t)np)));
+  int *ip = ints.as<int>();
+  d.nw = ip;
+  d.boff = ip + (nf + 1);
+  d.loff = ip + 2 * (nf + 1);
+  d.sfil = ip + 3 * (nf + 1);
+  d.sseg = d.sfil + np;
+  std::vector<int> hnw1(hnw);
+  hnw1.push_back(0);
+  FS_HIP(hipMemcpy(d.nw, hnw1.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice));
+  FS_HIP(hipMemcpy(d.boff, hboff.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice));
+  FS_HIP(hipMemcpy(d.loff, hloff.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice));
+  FS_HIP(hipMemcpy(d.sfil, sf.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+  FS_HIP(hipMemcpy(d.sseg, ss.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+  FS_TRY(joffb.alloc(sizeof(long) * (nf + 1)));
+  FS_HIP(hipMemcpy(joffb.p, hjoff.data(), sizeof(long) * (nf + 1), hipMemcpyHostToDevice));
+  d.joff = joffb.as<long>();
+  FS_TRY(propsb.alloc(sizeof(fs_fil_props) * nf));
+  FS_HIP(hipMemcpy(propsb.p, props, sizeof(fs_fil_props) * nf, hipMemcpyHostToDevice));
+  d.props = propsb.as<fs_fil_props>();
+  FS_TRY(Jb.alloc(sizeof(double) * (size_t)hjoff[nf]));
+  FS_TRY(JTb.alloc(sizeof(double) * (size_t)hjoff[nf]));
+  FS_TRY(Mb.alloc(sizeof(double) * (size_t)hjoff[nf]));
+  FS_TRY(Cm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
+  FS_TRY(Dm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
+  dot_chunk = 16384;
+  dot_blocks = ceil_div((long)n6, dot_chunk);
+  FS_TRY(partial.alloc(sizeof(double) * (size_t)dot_blocks * p.broyden_maxiter));
+  FS_TRY(coefb.alloc(sizeof(double) * (p.broyden_maxiter + 1)));
+  FS_TRY(flags.alloc(64));
+  FS_HIP(hipHostMalloc((void **)&hflags, 64, hipHostMallocDefault));
+  d.np = np;
+  d.nlam = nlam;
+  d.nf = nf;
+  d.dL = p.dL;
+  d.dt = p.dt;
+  d.mu = p.fcm.mu;
+  d.Lx = p.fcm.nx * p.fcm.dx;
+  d.Ly = p.fcm.ny * p.fcm.dx;
+  d.Lz = p.fcm.nz * p.fcm.dx;
+  d.impl_euler_steps = p.impl_euler_steps;
+  d.sperm_decay = p.sperm_decay;
+  d.st0 = p.strain_twist0;
+  for (int k = 0; k < 3; ++k) d.weight[k] = p.weight_per_length[k];
+  // FCM solver and collision barrier on the same box
+  FS_TRY(fcm.init(p.fcm, np, p.precision));
+  fs_colbar_params cp;
+  cp.box[0] = d.Lx;
+  cp.box[1] = d.Ly;
+  cp.box[2] = d.Lz;
+  cp.a = p.fcm.a;
+  cp.fs_over_2a = p.barrier_fs_over_2a;
+  cp.cap = p.barrier_cap;
+  cp.initial_friction_steps = p.initial_friction_steps;
+  if (p.barrier_on) FS_TRY(cb.init(cp, np, sf.data(), ss.data()));
+  FS_HIP(hipStreamCreate(&st));
+  return FS_OK;
+}
+
+void SystemImpl::release() {
+  if (st) (void)hipStreamSynchronize(st);
+  arena.release();
+  ints.release();
+  joffb.release();
+  propsb.release();
+  Jb.release();
+  Mb.release();
+  JTb.release();
+  Cm.release();
+  Dm.release();
+  partial.release();
+  coefb.release();
+  flags.release();
+  if (hflags) (void)hipHostFree(hflags);
+  hflags = nullptr;
+  fcm.release();
+  cb.release();
+  if (st) (void)hipStreamDestroy(st);
+  st = nullptr;
+}
+
+int SystemImpl::initial_guess() {
+  hipLaunchKernelGGL(k_guess, dim3(nf), dim3(64), 0, st, d);
+  FS_LAUNCHED();
+  return FS_OK;
+}
+
+int SystemImpl::forces(int nt) {
+  hipLaunchKernelGGL(k_forces, dim3(nf), dim3(64), 0, st, d, nt);
+  FS_LAUNCHED();
+  return FS_OK;
+}
+
+int SystemImpl::update(const double *u) {
+  hipLaunchKernelGGL(k_update, dim3(nf), dim3(64), 0, st, d, u);
+  FS_LAUNCHED();
+  return FS_OK;
+}
+
+int SystemImpl::build_jacobian(int nt) {
+  size_t nJ = (size_t)hjoff[nf];
+  hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nJ + 255) / 256)),
+                     dim3(256), 0, st, Jb.as<double>(), nJ, 0.0);
+  hipLaunchKernelGGL(k_jac_build, dim3(nf), dim3(64), 0, st, d, nt,
+                     Jb.as<double>());
+  FS_LAUNCHED();
+  int *sing = flags.as<int>() + 4;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, sing, 1, 0);
+  // J is kept for inspection; the inverse works on a scratch copy
+  FS_HIP(hipMemcpyAsync(Mb.p, Jb.p, sizeof(double) * nJ,
+                        hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_gj_inverse, dim3(nf), dim3(1024), 0, st, d,
+                     Mb.as<double>(), Jinv_cm(), sing);
+  FS_LAUNCHED();
+  // history reset (FilamentJacobianSolver.hpp:40-41)
+  size_t nh = 6 * (size_t)np * p.broyden_maxiter;
+  hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nh + 255) / 256)),
+                     dim3(256), 0, st, Cm.as<double>(), nh, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nh + 255) / 256)),
+                     dim3(256), 0, st, Dm.as<double>(), nh, 0.0);
+  FS_LAUNCHED();
+  FS_HIP(hipMemcpyAsync(hflags + 4, sing, sizeof(int), hipMemcpyDeviceToHost, st));
+  FS_HIP(hipStreamSynchronize(st));
+  FS_CHECK(((int *)hflags)[4] == 0, FS_ERR_ARG,
+           "approximate Jacobian is singular");
+  return FS_OK;
+}
+
+int SystemImpl::jsolve(const double *b, double *x, int iter) {
+  FS_CHECK(iter >= 0 && iter <= p.broyden_maxiter, FS_ERR_ARG, "bad iter %d",
+           iter);
+  FS_CHECK(b != x, FS_ERR_ARG, "jsolve needs distinct b and x");
+  hipLaunchKernelGGL(k_jsolve_block, dim3(nf), dim3(256), 0, st, d, Jinv_cm(),
+                     b, x);
+  FS_LAUNCHED();
+  if (iter > 0) {
+    size_t n6 = 6 * (size_t)np;
+    hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, iter), dim3(256), 0, st,
+                       Dm.as<double>(), n6, iter, b, n6, dot_chunk,
+                       partial.as<double>());
+    hipLaunchKernelGGL(k_dots_final, dim3(ceil_div(iter, 64)), dim3(64), 0, st,
+                       partial.as<double>(), dot_blocks, iter,
+                       coefb.as<double>());
+    hipLaunchKernelGGL(k_hist_axpy, dim3(ceil_div((long)n6, 256)), dim3(256), 0,
+                       st, Cm.as<double>(), n6, iter, coefb.as<double>(), n6, x);
+    FS_LAUNCHED();
+  }
+  return FS_OK;
+}
+
+int SystemImpl::broyden_update(const double *Eo, const double *En,
+                               const double *upd, int iter) {
+  FS_CHECK(iter >= 0 && iter < p.broyden_maxiter, FS_ERR_ARG, "bad iter %d",
+           iter);
+  size_t n6 = 6 * (size_t)np;
+  hipLaunchKernelGGL(k_sub, dim3(ceil_div((long)n6, 256)), dim3(256), 0, st, En,
+                     Eo, dE, n6);
+  // ||dE||^2 through the same deterministic dot reduction
+  hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, 1), dim3(256), 0, st, dE,
+                     n6, 1, dE, n6, dot_chunk, partial.as<double>());
+  double *nrm2 = coefb.as<double>() + p.broyden_maxiter;
+  hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st,
+                     partial.as<double>(), dot_blocks, 1, nrm2);
+  FS_LAUNCHED();
+  FS_TRY(jsolve(dE, tmp, iter));
+  hipLaunchKernelGGL(k_broyden_cols, dim3(ceil_div((long)n6, 256)), dim3(256), 0,
+                     st, tmp, upd, dE, nrm2, n6, Cm.as<double>() + n6 * iter,
+                     Dm.as<double>() + n6 * iter);
+  FS_LAUNCHED();
+  return FS_OK;
+}
+
+int SystemImpl::residual(int nt, const double *V, const double *W, double *err,
+                         int *check, double *maxabs) {
+  unsigned long long *mb = flags.as<unsigned long long>();
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, mb, 0ull);
+  hipLaunchKernelGGL(k_residual, dim3(ceil_div(np, 256)), dim3(256), 0, st, d,
+                     nt, V, W, err, mb);
+  FS_LAUNCHED();
+  FS_HIP(hipMemcpyAsync(hflags, mb, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, st));
+  FS_HIP(hipStreamSynchronize(st));
+  double m;
+  memcpy(&m, hflags, sizeof m);
+  if (maxabs) *maxabs = m;
+  if (check) *check = (m > p.tol) ? 1 : 0;
+  return FS_OK;
+}
+
+int SystemImpl::step(int nt) {
+  size_t n = std::max((size_t)np, 3 * (size_t)nlam);
+  hipLaunchKernelGGL(k_step, dim3(ceil_div((long)n, 256)), dim3(256), 0, st, d,
+                     nt);
+  FS_LAUNCHED();
+  return FS_OK;
+}
+
+int SystemImpl::mobility(int nt) {
+  if (nt > p.initial_friction_steps)
+    return fcm.mobility(np, d.Ya, d.Fa, d.Ta, d.V, d.W, st);
+  return fcm.friction(np, d.Fa, d.Ta, d.V, d.W, st);
+}
+
+int SystemImpl::barrier(int nt, int link, double *min_dist2) {
+  if (!p.barrier_on) {
+    if (min_dist2) *min_dist2 = 100000.0;
+    return FS_OK;
+  }
+  if (link) FS_TRY(cb.link(d.Xs, st));
+  return cb.apply(d.Ya, nt, d.Fa, min_dist2, st);
+}
+
+// main.cpp:284-754
+int SystemImpl::timestep(int nt, int *iters) {
+  const size_t n6 = 6 * (size_t)np;
+  FS_TRY(initial_guess());
+  FS_TRY(forces(nt));
+  FS_TRY(build_jacobian(nt));
+  FS_TRY(barrier(nt, 1, nullptr));
+  FS_TRY(mobility(nt));
+  int check = 0;
+  FS_TRY(residual(nt, d.V, d.W, E, &check, &last_res0));
+  int iter = 0;
+  while (check && iter < p.broyden_maxiter) {
+    FS_TRY(jsolve(E, Upd, iter));
+    FS_TRY(update(Upd));
+    FS_TRY(forces(nt));
+    FS_TRY(barrier(nt, 0, nullptr));
+    FS_TRY(mobility(nt));
+    FS_TRY(residual(nt, d.V, d.W, NewE, &check, &last_res));
+    FS_TRY(broyden_update(E, NewE, Upd, iter));
+    FS_HIP(hipMemcpyAsync(E, NewE, sizeof(double) * n6,
+                          hipMemcpyDeviceToDevice, st));
+    ++iter;
+  }
+  FS_TRY(step(nt));
+  FS_HIP(hipStreamSynchronize(st));
+  if (iters) *iters = iter;
+  return FS_OK;
+}
+
+}  // namespace fs

@@ -1,0 +1,336 @@
+// system_api.cpp -- C-ABI of the filament system (fs_system_*, fs_fil_*)
+// and the deterministic initial-configuration generator.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+#include "system_impl.hpp"
+#include "warmstart.hpp"
+
+using fs::SystemImpl;
+
+struct fs_system {
+  SystemImpl impl;
+  fs::WarmStart ws;
+};
+
+namespace {
+
+int copy_field(double *dst, const double *src, size_t n, hipMemcpyKind k) {
+  if (!dst || !src || n == 0) return FS_OK;
+  FS_HIP(hipMemcpy(dst, src, sizeof(double) * n, k));
+  return FS_OK;
+}
+
+}  // namespace
+
+extern "C" int fs_system_create(const fs_sim_params *p, int nfil, const int *nw,
+                                const fs_fil_props *props, fs_system **out) {
+  FS_CHECK(p && nw && props && out, FS_ERR_ARG, "null argument");
+  *out = nullptr;
+  fs_system *h = new fs_system();
+  int rc = h->impl.init(*p, nfil, nw, props);
+  if (rc != FS_OK) {
+    h->impl.release();
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return FS_OK;
+}
+
+extern "C" int fs_system_destroy(fs_system *h) {
+  if (!h) return FS_OK;
+  h->ws.release();
+  h->impl.release();
+  delete h;
+  return FS_OK;
+}
+
+extern "C" int fs_system_sizes(fs_system *h, int *nfil, int *np, int *nlam) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  if (nfil) *nfil = h->impl.nf;
+  if (np) *np = h->impl.np;
+  if (nlam) *nlam = h->impl.nlam;
+  return FS_OK;
+}
+
+extern "C" int fs_system_set_state(fs_system *h, const fs_state_ptrs *s) {
+  FS_CHECK(h && s, FS_ERR_ARG, "null argument");
+  FS_HIP(hipStreamSynchronize(h->impl.st));
+  auto &d = h->impl.d;
+  size_t n3 = 3 * (size_t)h->impl.np, n4 = 4 * (size_t)h->impl.np,
+         l3 = 3 * (size_t)h->impl.nlam;
+  auto K = hipMemcpyHostToDevice;
+  FS_TRY(copy_field(d.X, s->X, n3, K));
+  FS_TRY(copy_field(d.Xs, s->Xs, n3, K));
+  FS_TRY(copy_field(d.Xt, s->Xt, n3, K));
+  FS_TRY(copy_field(d.U, s->U, n3, K));
+  FS_TRY(copy_field(d.Us, s->Us, n3, K));
+  FS_TRY(copy_field(d.Ut, s->Ut, n3, K));
+  FS_TRY(copy_field(d.q, s->q, n4, K));
+  FS_TRY(copy_field(d.qs, s->qs, n4, K));
+  FS_TRY(copy_field(d.qt, s->qt, n4, K));
+  FS_TRY(copy_field(d.lam, s->lam, l3, K));
+  FS_TRY(copy_field(d.lam1, s->lam1, l3, K));
+  FS_TRY(copy_field(d.lam2, s->lam2, l3, K));
+  FS_TRY(copy_field(d.F, s->F, n3, K));
+  FS_TRY(copy_field(d.TAU, s->TAU, n3, K));
+  return FS_OK;
+}
+
+extern "C" int fs_system_get_state(fs_system *h, fs_state_ptrs *s) {
+  FS_CHECK(h && s, FS_ERR_ARG, "null argument");
+  FS_HIP(hipStreamSynchronize(h->impl.st));
+  auto &d = h->impl.d;
+  size_t n3 = 3 * (size_t)h->impl.np, n4 = 4 * (size_t)h->impl.np,
+         l3 = 3 * (size_t)h->impl.nlam;
+  auto K = hipMemcpyDeviceToHost;
+  FS_TRY(copy_field(s->X, d.X, n3, K));
+  FS_TRY(copy_field(s->Xs, d.Xs, n3, K));
+  FS_TRY(copy_field(s->Xt, d.Xt, n3, K));
+  FS_TRY(copy_field(s->U, d.U, n3, K));
+  FS_TRY(copy_field(s->Us, d.Us, n3, K));
+  FS_TRY(copy_field(s->Ut, d.Ut, n3, K));
+  FS_TRY(copy_field(s->q, d.q, n4, K));
+  FS_TRY(copy_field(s->qs, d.qs, n4, K));
+  FS_TRY(copy_field(s->qt, d.qt, n4, K));
+  FS_TRY(copy_field(s->lam, d.lam, l3, K));
+  FS_TRY(copy_field(s->lam1, d.lam1, l3, K));
+  FS_TRY(copy_field(s->lam2, d.lam2, l3, K));
+  FS_TRY(copy_field(s->F, d.F, n3, K));
+  FS_TRY(copy_field(s->TAU, d.TAU, n3, K));
+  return FS_OK;
+}
+
+extern "C" int fs_system_fcm_arrays(fs_system *h, double **Y, double **F,
+                                    double **T, double **V, double **W) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  if (Y) *Y = h->impl.d.Ya;
+  if (F) *F = h->impl.d.Fa;
+  if (T) *T = h->impl.d.Ta;
+  if (V) *V = h->impl.d.V;
+  if (W) *W = h->impl.d.W;
+  return FS_OK;
+}
+
+extern "C" int fs_system_timestep(fs_system *h, int nt, int *iters) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  if (h->ws.attached()) return h->ws.timestep(h->impl, nt, iters);
+  return h->impl.timestep(nt, iters);
+}
+
+extern "C" int fs_fil_initial_guess(fs_system *h) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.initial_guess();
+}
+
+extern "C" int fs_fil_forces(fs_system *h, int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.forces(nt);
+}
+
+extern "C" int fs_fil_build_jacobian(fs_system *h, int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.build_jacobian(nt);
+}
+
+static int get_matrix(fs_system *h, int f, double *out, bool inverse) {
+  FS_CHECK(h && out, FS_ERR_ARG, "null argument");
+  FS_CHECK(f >= 0 && f < h->impl.nf, FS_ERR_ARG, "bad filament %d", f);
+  FS_HIP(hipStreamSynchronize(h->impl.st));
+  int n = 6 * h->impl.hnw[f];
+  size_t nn = (size_t)n * n;
+  std::vector<double> m(nn);
+  const double *src = (inverse ? h->impl.JTb.as<double>() : h->impl.Jb.as<double>()) +
+                      h->impl.hjoff[f];
+  FS_HIP(hipMemcpy(m.data(), src, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  if (inverse) {  // stored column-major
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c) out[(size_t)r * n + c] = m[(size_t)c * n + r];
+  } else {
+    std::copy(m.begin(), m.end(), out);
+  }
+  return FS_OK;
+}
+
+extern "C" int fs_fil_get_jinv(fs_system *h, int f, double *out) {
+  return get_matrix(h, f, out, true);
+}
+
+extern "C" int fs_fil_get_jacobian(fs_system *h, int f, double *out) {
+  return get_matrix(h, f, out, false);
+}
+
+extern "C" int fs_fil_jsolve(fs_system *h, const double *b, double *x,
+                             int iter) {
+  FS_CHECK(h && b && x, FS_ERR_ARG, "null argument");
+  return h->impl.jsolve(b, x, iter);
+}
+
+extern "C" int fs_fil_broyden_update(fs_system *h, const double *E,
+                                     const double *Enew, const double *update,
+                                     int iter) {
+  FS_CHECK(h && E && Enew && update, FS_ERR_ARG, "null argument");
+  return h->impl.broyden_update(E, Enew, update, iter);
+}
+
+extern "C" int fs_fil_update(fs_system *h, const double *update) {
+  FS_CHECK(h && update, FS_ERR_ARG, "null argument");
+  return h->impl.update(update);
+}
+
+extern "C" int fs_fil_residual(fs_system *h, int nt, const double *V,
+                               const double *W, double *err, int *check,
+                               double *maxabs) {
+  FS_CHECK(h && V && W && err, FS_ERR_ARG, "null argument");
+  return h->impl.residual(nt, V, W, err, check, maxabs);
+}
+
+extern "C" int fs_fil_step(fs_system *h, int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.step(nt);
+}
+
+extern "C" int fs_system_mobility(fs_system *h, int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.mobility(nt);
+}
+
+extern "C" int fs_system_barrier(fs_system *h, int nt, int link,
+                                 double *min_dist2) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.barrier(nt, link, min_dist2);
+}
+
+// ---------------------------------------------------------------------
+// Initial configuration: straight filaments, heads uniform in the box,
+// orientation uniform on the sphere (random unit quaternion), rejected
+// while any two segments of different filaments are closer than min_sep
+// (periodic). Deterministic: splitmix64 stream from the seed.
+// ---------------------------------------------------------------------
+namespace {
+
+struct Rng {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+  double normal() {  // Box-Muller
+    double u1 = uniform(), u2 = uniform();
+    if (u1 < 1e-300) u1 = 1e-300;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+};
+
+struct Grid {
+  double L[3];
+  int M[3];
+  double cw[3];
+  std::unordered_map<long, std::vector<int>> cells;
+  long key(int a, int b, int c) const {
+    return ((long)a * M[1] + b) * M[2] + c;
+  }
+  void coords(const double *x, int *c) const {
+    for (int d = 0; d < 3; ++d) {
+      double w = x[d] - L[d] * std::floor(x[d] / L[d]);
+      int ci = (int)(w / cw[d]);
+      c[d] = std::min(std::max(ci, 0), M[d] - 1);
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" int fs_make_filaments(const double box[3], int nfil, const int *nw,
+                                 double dL, double min_sep,
+                                 unsigned long long seed, int max_tries,
+                                 double *X, double *q) {
+  FS_CHECK(box && nw && X && q && nfil >= 1 && dL > 0 && min_sep >= 0,
+           FS_ERR_ARG, "bad argument");
+  Grid G;
+  for (int d = 0; d < 3; ++d) {
+    G.L[d] = box[d];
+    G.M[d] = std::max(1, (int)(box[d] / std::max(min_sep, 1e-12)));
+    G.M[d] = std::min(G.M[d], 4096);
+    G.cw[d] = box[d] / G.M[d];
+  }
+  Rng rng{seed};
+  std::vector<double> pts;  // accepted segment positions
+  int g0 = 0;
+  const double ms2 = min_sep * min_sep;
+  for (int f = 0; f < nfil; ++f) {
+    int n = nw[f];
+    bool ok = false;
+    std::vector<double> cand(3 * n);
+    double qq[4];
+    for (int tr = 0; tr < max_tries && !ok; ++tr) {
+      double head[3];
+      for (int d = 0; d < 3; ++d) head[d] = rng.uniform() * box[d];
+      double nn = 0;
+      for (int c = 0; c < 4; ++c) {
+        qq[c] = rng.normal();
+        nn += qq[c] * qq[c];
+      }
+      nn = std::sqrt(nn);
+      for (int c = 0; c < 4; ++c) qq[c] /= nn;
+      // tangent = R(q) e_x (UnitQuaternion.hpp:341-352)
+      double t[3] = {1 - 2 * qq[2] * qq[2] - 2 * qq[3] * qq[3],
+                     2 * (qq[1] * qq[2] + qq[0] * qq[3]),
+                     2 * (qq[1] * qq[3] - qq[0] * qq[2])};
+      for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) cand[3 * i + d] = head[d] + (i * dL) * t[d];
+      ok = true;
+      for (int i = 0; i < n && ok; ++i) {
+        int c[3];
+        G.coords(&cand[3 * i], c);
+        for (int a = -1; a <= 1 && ok; ++a)
+          for (int b = -1; b <= 1 && ok; ++b)
+            for (int e = -1; e <= 1 && ok; ++e) {
+              int ca = ((c[0] + a) % G.M[0] + G.M[0]) % G.M[0];
+              int cb = ((c[1] + b) % G.M[1] + G.M[1]) % G.M[1];
+              int cc = ((c[2] + e) % G.M[2] + G.M[2]) % G.M[2];
+              auto it = G.cells.find(G.key(ca, cb, cc));
+              if (it == G.cells.end()) continue;
+              for (int j : it->second) {
+                double r2 = 0;
+                for (int d = 0; d < 3; ++d) {
+                  double x = cand[3 * i + d] - pts[3 * j + d];
+                  x -= G.L[d] * std::floor(x / G.L[d] + 0.5);
+                  r2 += x * x;
+                }
+                if (r2 < ms2) {
+                  ok = false;
+                  break;
+                }
+              }
+            }
+      }
+    }
+    FS_CHECK(ok, FS_ERR_ARG,
+             "could not place filament %d without overlap in %d tries", f,
+             max_tries);
+    for (int i = 0; i < n; ++i) {
+      int g = g0 + i;
+      for (int d = 0; d < 3; ++d) {
+        X[3 * g + d] = cand[3 * i + d];
+        pts.push_back(cand[3 * i + d]);
+      }
+      for (int c = 0; c < 4; ++c) q[4 * g + c] = qq[c];
+      int cc[3];
+      G.coords(&cand[3 * i], cc);
+      G.cells[G.key(cc[0], cc[1], cc[2])].push_back(g);
+    }
+    g0 += n;
+  }
+  return FS_OK;
+}

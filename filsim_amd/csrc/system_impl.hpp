@@ -1,0 +1,63 @@
+// system_impl.hpp -- device state of a filament system (fs_system).
+#pragma once
+
+#include <vector>
+
+#include "colbar_impl.hpp"
+#include "common.hpp"
+#include "fcm_impl.hpp"
+
+namespace fs {
+
+// Plain struct of device pointers and scalars passed by value to kernels.
+struct SysDev {
+  double *X, *Xs, *Xt, *U, *Us, *Ut, *q, *qs, *qt, *F, *TAU;
+  double *lam, *lam1, *lam2;
+  double *st;              // myStrainTwist per filament (3)
+  double *Ya, *Fa, *Ta;    // FCM inputs (assign_filament_data)
+  double *V, *W;           // FCM outputs
+  int *nw, *boff, *loff, *sfil, *sseg;
+  long *joff;
+  fs_fil_props *props;
+  int np, nlam, nf;
+  double dL, dt, mu, Lx, Ly, Lz, st0;
+  double weight[3];
+  int impl_euler_steps, sperm_decay;
+};
+
+struct SystemImpl {
+  fs_sim_params p{};
+  int nf = 0, np = 0, nlam = 0;
+  std::vector<int> hnw, hboff, hloff;
+  std::vector<long> hjoff;
+  SysDev d{};
+  dbuf arena, ints, joffb, propsb, Jb, Mb, JTb, Cm, Dm, partial, coefb, flags;
+  double *E = nullptr, *NewE = nullptr, *Upd = nullptr, *tmp = nullptr,
+         *dE = nullptr, *scratch6 = nullptr;
+  unsigned long long *hflags = nullptr;
+  int dot_chunk = 0, dot_blocks = 0;
+  double last_res0 = 0, last_res = 0;
+  FcmImpl fcm;
+  ColbarImpl cb;
+  hipStream_t st = nullptr;
+
+  double *Jinv_cm() { return JTb.as<double>(); }
+  int init(const fs_sim_params &pp, int nfil, const int *nw,
+           const fs_fil_props *props);
+  void release();
+  int initial_guess();
+  int forces(int nt);
+  int update(const double *u);
+  int build_jacobian(int nt);
+  int jsolve(const double *b, double *x, int iter);
+  int broyden_update(const double *Eo, const double *En, const double *upd,
+                     int iter);
+  int residual(int nt, const double *V, const double *W, double *err,
+               int *check, double *maxabs);
+  int step(int nt);
+  int mobility(int nt);
+  int barrier(int nt, int link, double *min_dist2);
+  int timestep(int nt, int *iters);
+};
+
+}  // namespace fs

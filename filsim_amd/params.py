@@ -1,0 +1,108 @@
+"""Parameter sets for the FCM and the filament timestep.
+
+The defaults restate multi_filament_header.hpp:47-60 (sigma = a/sqrt(pi),
+sigma_dip = a/(36 pi)^(1/6), dx = sigma_dip/1.5) and the workloads the
+benchmarks and tests run on (grid spacing a/1.86 with 12 support points).
+"""
+import math
+
+from . import abi
+
+PI = 3.14159265358979323846264338327950288
+
+
+def fcm_params(n, a=1.0, mu=1.0, dx=None, ngd=12):
+    """FcmParams for an n^3 (or (nx,ny,nz)) periodic grid."""
+    if isinstance(n, int):
+        n = (n, n, n)
+    sigma_dip = a / (36.0 * PI) ** (1.0 / 6.0)
+    if dx is None:
+        dx = a / 1.86
+    p = abi.FcmParams()
+    p.nx, p.ny, p.nz = n
+    p.ngd = ngd
+    p.a = a
+    p.mu = mu
+    p.dx = dx
+    p.sigma = a / math.sqrt(PI)
+    p.sigma_dip = sigma_dip
+    return p
+
+
+def reference_fcm_params(n, a=1.0, mu=1.0):
+    """The reference's own resolution: dx = sigma_dip/1.5, ngd = 20."""
+    sigma_dip = a / (36.0 * PI) ** (1.0 / 6.0)
+    return fcm_params(n, a=a, mu=mu, dx=sigma_dip / 1.5, ngd=20)
+
+
+def box(p):
+    return (p.nx * p.dx, p.ny * p.dx, p.nz * p.dx)
+
+
+def colbar_params(fp, fs_over_2a=15 * PI, cap=2.0, initial_friction_steps=0):
+    c = abi.ColbarParams()
+    b = box(fp)
+    for d in range(3):
+        c.box[d] = b[d]
+    c.a = fp.a
+    c.fs_over_2a = fs_over_2a
+    c.cap = cap
+    c.initial_friction_steps = initial_friction_steps
+    return c
+
+
+def sim_params(fp, dL=2.2, dt=0.01, tol=1e-4, maxiter=100, precision=0,
+               impl_euler_steps=1, initial_friction_steps=0,
+               weight=(0.0, 0.0, 0.0), barrier_on=1, fs_over_2a=15 * PI,
+               cap=2.0, sperm_decay=0, strain_twist0=0.0):
+    s = abi.SimParams()
+    s.fcm = fp
+    s.precision = precision
+    s.dL = dL
+    s.dt = dt
+    s.tol = tol
+    s.broyden_maxiter = maxiter
+    s.impl_euler_steps = impl_euler_steps
+    s.initial_friction_steps = initial_friction_steps
+    for d in range(3):
+        s.weight_per_length[d] = weight[d]
+    s.barrier_on = barrier_on
+    s.barrier_fs_over_2a = fs_over_2a
+    s.barrier_cap = cap
+    s.sperm_decay = sperm_decay
+    s.strain_twist0 = strain_twist0
+    return s
+
+
+def fil_props(nfil, kap=1.0, c=None, T0=0.0, omega=0.0, wavenumber=0.0,
+              phase=0.0, alpha=0.0, beta=1.0, radius=1.0):
+    """Array of FilProps; scalar or per-filament sequence arguments."""
+    arr = (abi.FilProps * nfil)()
+
+    def pick(v, f):
+        return v[f] if hasattr(v, "__len__") else v
+
+    for f in range(nfil):
+        pr = arr[f]
+        pr.kap = pick(kap, f)
+        pr.c = pick(c if c is not None else kap, f)
+        pr.T0 = pick(T0, f)
+        pr.omega = pick(omega, f)
+        pr.wavenumber = pick(wavenumber, f)
+        pr.phase = pick(phase, f)
+        pr.alpha = pick(alpha, f)
+        pr.beta = pick(beta, f)
+        pr.radius = pick(radius, f)
+    return arr
+
+
+def swimming_props(nfil, nw, dL=2.2, amplitude_times_L=0.5, period=1.0,
+                   kap=1.0, phases=None):
+    """Travelling-wave swimmers: preferred curvature amplitude 0.5/L,
+    wavenumber 2 pi/L, period 1 time unit (L = Nw*dL, the reference's
+    L_filament_long)."""
+    L = nw * dL
+    ph = phases if phases is not None else 0.0
+    return fil_props(nfil, kap=kap, T0=amplitude_times_L / L,
+                     omega=2 * PI / period, wavenumber=2 * PI / L, phase=ph,
+                     alpha=0.0, beta=1.0)

@@ -1,0 +1,233 @@
+/*
+ * filsim.h -- C-ABI of the MI355X (gfx950) FilSim hot path.
+ *
+ * One shared library (libfilsim_amd.so) exports everything below. Every
+ * entry point returns an int status: FS_OK (0) on success, a negative FS_ERR_*
+ * code otherwise; fs_last_error() gives the message of the last failure on the
+ * calling thread. Handles are not thread-safe: one handle, one host thread.
+ *
+ * Pointer conventions
+ *   - "device" arguments are HBM pointers (hipMalloc / torch.cuda tensors).
+ *   - "host" arguments are ordinary host memory.
+ *   - Per-segment arrays are Np x 3 row-major doubles, the layout of the
+ *     reference's double** Y/F/T/V/W blocks (c_array_functions.cpp:12-21,
+ *     FCMfunctions.hpp:13-17): &Y[0][0] may be passed directly.
+ *   - stream arguments are hipStream_t passed as void* (NULL = null stream).
+ *
+ * Reference call surface each group replaces is cited per function.
+ */
+#ifndef FILSIM_H
+#define FILSIM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS_OK 0
+#define FS_ERR_ARG (-1)    /* invalid argument or size                  */
+#define FS_ERR_HIP (-2)    /* a HIP runtime call failed                 */
+#define FS_ERR_SIZE (-3)   /* input larger than the handle was built for */
+#define FS_ERR_STATE (-4)  /* call out of order (e.g. apply before link) */
+
+#define FS_FP64 0 /* spread/FFT/projection/interpolation in FP64           */
+#define FS_FP32 1 /* the same four stages in FP32 with FP32 accumulation   */
+
+/* ------------------------------------------------------------------ */
+/* Library                                                             */
+/* ------------------------------------------------------------------ */
+const char *fs_last_error(void);
+int fs_version(void);
+/* Number of visible HIP devices (0 when none); never faults. */
+int fs_device_count(int *count);
+
+/* ------------------------------------------------------------------ */
+/* FCM mobility (replaces struct Fcm, FCMfunctions.hpp:7-59)            */
+/* ------------------------------------------------------------------ */
+typedef struct fs_fcm_params {
+  int nx, ny, nz;   /* NPTS_X/Y/Z, periodic grid (config.hpp:73-75)         */
+  int ngd;          /* Gaussian support points per dimension (config.hpp:76)*/
+  double a;         /* segment radius                                       */
+  double mu;        /* viscosity                                            */
+  double dx;        /* grid spacing (multi_filament_header.hpp:48)          */
+  double sigma;     /* force envelope width  (multi_filament_header.hpp:57) */
+  double sigma_dip; /* dipole envelope width (multi_filament_header.hpp:47) */
+} fs_fcm_params;
+
+typedef struct fs_fcm fs_fcm;
+
+/* Fcm::initialise (FCMfunctions.cpp:10-168): grids, wavenumbers, tables.
+ * Sizes must factor as 2^a 3^b 5^c with nx,ny,nz <= 1024 (even). */
+int fs_fcm_create(const fs_fcm_params *p, int max_np, int precision,
+                  fs_fcm **out);
+/* Fcm::free_memory (FCMfunctions.cpp:468-519). */
+int fs_fcm_destroy(fs_fcm *h);
+/* Fcm::mobility_solve (FCMfunctions.cpp:175-464): V,W from Y,F,T (device). */
+int fs_fcm_mobility(fs_fcm *h, int np, const double *Y, const double *F,
+                    const double *T, double *V, double *W, void *stream);
+/* Same with host arrays (copies in and out; synchronous). */
+int fs_fcm_mobility_host(fs_fcm *h, int np, const double *Y, const double *F,
+                         const double *T, double *V, double *W);
+/* Fcm::friction_mobility_solve (FCMfunctions.cpp:865-880) (device). */
+int fs_fcm_friction(fs_fcm *h, int np, const double *F, const double *T,
+                    double *V, double *W, void *stream);
+/* Per-stage device time of the last solve in ms, in the order of the
+ * reference profiler buckets (profilers.hpp:38-51): make-zero, gaussian
+ * setup, force distribution, FFT forward, flow solve, FFT backward,
+ * particle velocities+rotations. Stages fused into one kernel report their
+ * combined time in the first bucket and 0 in the others. Requires
+ * fs_fcm_set_timing(h,1) before the solve. */
+int fs_fcm_set_timing(fs_fcm *h, int on);
+int fs_fcm_stage_times(fs_fcm *h, float *ms7);
+/* Tunables chosen by the autotuner (tile/batch shapes); -1 keeps current. */
+typedef struct fs_fcm_tuning {
+  int fft_batch;    /* columns per workgroup in the strided FFT passes */
+  int spread_tile;  /* tile edge (grid points) of the gather spread   */
+  int interp_wave;  /* 1: one wavefront per segment in interpolation  */
+} fs_fcm_tuning;
+int fs_fcm_get_tuning(fs_fcm *h, fs_fcm_tuning *t);
+int fs_fcm_set_tuning(fs_fcm *h, const fs_fcm_tuning *t);
+
+/* ------------------------------------------------------------------ */
+/* Collision barrier (replaces struct LinkedList,                      */
+/* CollisionBarrierFilament.hpp:19-57)                                 */
+/* ------------------------------------------------------------------ */
+typedef struct fs_colbar_params {
+  double box[3];        /* LfcmBox_x/y/z = NPTS*dx                         */
+  double a;             /* segment radius (Rnm = 2a)                       */
+  double fs_over_2a;    /* Barrier_FS_over_2a                              */
+  double cap;           /* Barrier_distance_cap                            */
+  int initial_friction_steps; /* adjacent pairs are repelled while nt < it */
+} fs_colbar_params;
+
+typedef struct fs_colbar fs_colbar;
+
+/* LinkedList::initialise + set_global_segment_number
+ * (CollisionBarrierFilament.cpp:274-367). seg_fil/seg_idx: host, np each. */
+int fs_colbar_create(const fs_colbar_params *p, int np, const int *seg_fil,
+                     const int *seg_idx, fs_colbar **out);
+int fs_colbar_destroy(fs_colbar *h);
+/* M_X, M_Y, M_Z cell counts (CollisionBarrierFilament.cpp:277-280). */
+int fs_colbar_dims(fs_colbar *h, int *m3);
+/* LinkedList::link (CollisionBarrierFilament.cpp:370-433) from Xs (device,
+ * np x 3, unwrapped positions). */
+int fs_colbar_link(fs_colbar *h, const double *Xs, void *stream);
+/* LinkedList::apply_collision_barrier (CollisionBarrierFilament.cpp:24-271):
+ * F (device, np x 3) += barrier forces computed from Y (device) with the
+ * list built by the last link. min_dist2 (host, may be NULL) receives the
+ * minimum squared distance over all visited pairs. */
+int fs_colbar_apply(fs_colbar *h, const double *Y, int nt, double *F,
+                    double *min_dist2, void *stream);
+/* The reference's head[ncell] and list[np] arrays (host out). */
+int fs_colbar_get_lists(fs_colbar *h, int *head, int *list);
+/* Unordered pairs (i<j) inside the cutoff r^2 < 1.21(2a)^2 found through the
+ * cell list, sorted by (i,j); pairs: host, 2*max_pairs ints. */
+int fs_colbar_pairs(fs_colbar *h, const double *Y, int nt, int max_pairs,
+                    int *pairs, int *npairs);
+
+/* ------------------------------------------------------------------ */
+/* Filament system + implicit timestep (replaces class Filament,       */
+/* filamentJsolver, ErrorCheckFCM and the Broyden loop of main.cpp)    */
+/* ------------------------------------------------------------------ */
+typedef struct fs_sim_params {
+  fs_fcm_params fcm;
+  int precision;            /* FS_FP64 or FS_FP32 for the FCM stages       */
+  double dL;                /* segment spacing DL                          */
+  double dt;                /* time step                                    */
+  double tol;               /* Broyden tolerance TOL (infinity norm)       */
+  int broyden_maxiter;      /* broyden_maxiter                              */
+  int impl_euler_steps;     /* ImplEulerSteps                               */
+  int initial_friction_steps; /* initialFrictionSteps                       */
+  double weight_per_length[3]; /* gravity force per unit length            */
+  int barrier_on;
+  double barrier_fs_over_2a;
+  double barrier_cap;
+  int sperm_decay;          /* SpermCurvatureDecay                          */
+  double strain_twist0;     /* STRAINTWISTZ: static twist strain            */
+} fs_sim_params;
+
+/* Per-filament properties (Filament.hpp:27-44). */
+typedef struct fs_fil_props {
+  double kap;        /* myKAP: bending modulus                               */
+  double c;          /* myC: twist modulus                                   */
+  double T0;         /* preferred-curvature amplitude (curvature/L_long)     */
+  double omega;      /* AngFrequency                                         */
+  double wavenumber; /* Wavenumber                                           */
+  double phase;      /* Phase                                                */
+  double alpha;      /* Alpha (helix, sin part)                              */
+  double beta;       /* Beta (planar, cos part)                              */
+  double radius;     /* bead radius                                          */
+} fs_fil_props;
+
+typedef struct fs_system fs_system;
+
+/* Filament constructors + Jsolver(Np) + LinkedList/Fcm initialise
+ * (main.cpp:230-261). nw: host, segments per filament (each >= 2). */
+int fs_system_create(const fs_sim_params *p, int nfil, const int *nw,
+                     const fs_fil_props *props, fs_system **out);
+int fs_system_destroy(fs_system *h);
+int fs_system_sizes(fs_system *h, int *nfil, int *np, int *nlam);
+/* State in and out (host). Per bead: X,Xs,Xt,U,Us,Ut (np x 3), q,qs,qt
+ * (np x 4, scalar first); per link: lam,lam1,lam2 (nlam x 3, links of one
+ * filament contiguous). Any pointer may be NULL to skip that field. */
+typedef struct fs_state_ptrs {
+  double *X, *Xs, *Xt, *U, *Us, *Ut, *q, *qs, *qt, *lam, *lam1, *lam2;
+  double *F, *TAU;
+} fs_state_ptrs;
+int fs_system_set_state(fs_system *h, const fs_state_ptrs *s);
+int fs_system_get_state(fs_system *h, fs_state_ptrs *s);
+/* Device pointers of the FCM arrays the system drives (np x 3 each). */
+int fs_system_fcm_arrays(fs_system *h, double **Y, double **F, double **T,
+                         double **V, double **W);
+
+/* One whole implicit timestep nt (main.cpp:284-754) on the device: initial
+ * guess, forces, Jacobian, link, barrier, mobility, Broyden loop, step.
+ * *iters receives the Broyden iteration count. */
+int fs_system_timestep(fs_system *h, int nt, int *iters);
+
+/* The same step decomposed, one call per reference call site. */
+/* Filament::initialGuess + unRobotArmify (main.cpp:334-336). */
+int fs_fil_initial_guess(fs_system *h);
+/* setZeroForcesTorques + applyElasticTorques + applyConstraintForcesTorques
+ * + applyExternalForcesTorques + Fcm::assign_filament_data
+ * (main.cpp:337-361, Filament.hpp:271-402, FCMfunctions.cpp:882-918). */
+int fs_fil_forces(fs_system *h, int nt);
+/* filamentJsolver::buildJacobian (FilamentJacobianSolver.hpp:36-47):
+ * per-filament approximate Jacobian and its inverse; resets the history. */
+int fs_fil_build_jacobian(fs_system *h, int nt);
+/* Copy filament f's inverse Jacobian (6Nw x 6Nw row-major) to host. */
+int fs_fil_get_jinv(fs_system *h, int f, double *out);
+/* Copy filament f's Jacobian before inversion (host out). */
+int fs_fil_get_jacobian(fs_system *h, int f, double *out);
+/* filamentJsolver::solve (FilamentJacobianSolver.hpp:49-73): x = H b with
+ * the first iter history columns. b, x: device, 6Np. */
+int fs_fil_jsolve(fs_system *h, const double *b, double *x, int iter);
+/* Bad-Broyden column update (main.cpp:640-663): stores column iter from
+ * dE = Enew - E and the last update (device, 6Np each). */
+int fs_fil_broyden_update(fs_system *h, const double *E, const double *Enew,
+                          const double *update, int iter);
+/* Filament::update + unRobotArmify (main.cpp:528-531); update: device 6Np. */
+int fs_fil_update(fs_system *h, const double *update);
+/* ErrorCheckFCM (MobilityConstraints.cpp:447-770): err (device, 6Np) from
+ * V, W (device); *check = 1 if any |err| > tol; *maxabs = max |err|. */
+int fs_fil_residual(fs_system *h, int nt, const double *V, const double *W,
+                    double *err, int *check, double *maxabs);
+/* Filament::step (Filament.hpp:455-478, Bead.hpp:266-276). */
+int fs_fil_step(fs_system *h, int nt);
+/* Mobility on the system's Y/F/T into its V/W (the reference's
+ * fcm.mobility_solve or friction_mobility_solve, chosen by nt). */
+int fs_system_mobility(fs_system *h, int nt);
+/* Collision barrier on the system's Y/F (link = 1 rebuilds the list). */
+int fs_system_barrier(fs_system *h, int nt, int link, double *min_dist2);
+
+/* Deterministic initial configuration: nfil straight filaments of nw[f]
+ * segments, heads uniform in the box, orientation uniform on the sphere,
+ * rejected while any two segments of different filaments are closer than
+ * min_sep (periodic). Writes X (np x 3) and q (np x 4). seed: 64-bit. */
+int fs_make_filaments(const double box[3], int nfil, const int *nw,
+                      double dL, double min_sep, unsigned long long seed,
+                      int max_tries, double *X, double *q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FILSIM_H */

@@ -1,0 +1,104 @@
+"""FCM mobility on the GPU against the FP64 oracle (same seeded inputs)."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from filsim_amd import abi, api, params
+from helpers import random_segments, rel, to_dev
+
+pytestmark = pytest.mark.gpu
+
+# (grid, ngd, dx) -- radix 2, 3 and 5 sizes, non-cubic boxes, reference dx
+CASES = [
+    ((32, 32, 32), 12, None),
+    ((48, 32, 40), 12, None),
+    ((64, 64, 64), 12, None),
+    ((30, 36, 50), 12, None),
+    ((40, 40, 40), 20, "ref"),
+]
+
+
+def _fp(grid, ngd, dx):
+    if dx == "ref":
+        return params.reference_fcm_params(grid)
+    return params.fcm_params(grid, ngd=ngd)
+
+
+@pytest.mark.parametrize("grid,ngd,dx", CASES)
+def test_mobility_fp64_matches_oracle(cuda, grid, ngd, dx):
+    fp = _fp(grid, ngd, dx)
+    Y, F, T = random_segments(fp, 200, seed=sum(grid))
+    Vo, Wo = O.mobility(fp, Y, F, T)
+    with api.Fcm(fp, 200) as fcm:
+        V, W = fcm.mobility_host(Y, F, T)
+    print("grid", grid, "relV", rel(V, Vo), "relW", rel(W, Wo))
+    assert rel(V, Vo) <= 1e-12
+    assert rel(W, Wo) <= 1e-12
+
+
+def test_mobility_fp32_error_bound(cuda):
+    fp = params.fcm_params(64)
+    Y, F, T = random_segments(fp, 400, seed=5)
+    Vo, Wo = O.mobility(fp, Y, F, T)
+    with api.Fcm(fp, 400, precision=abi.FS_FP32) as fcm:
+        V, W = fcm.mobility_host(Y, F, T)
+    ev, ew = rel(V, Vo), rel(W, Wo)
+    print("fp32 relV %.3e relW %.3e" % (ev, ew))
+    assert ev < 1e-5 and ew < 1e-5
+
+
+def test_negative_positions_truncation(cuda):
+    """Y from fmod can be negative: the stencil centre truncates toward 0
+    (FCMfunctions.cpp:565-567)."""
+    fp = params.fcm_params(32)
+    Y, F, T = random_segments(fp, 64, seed=11)
+    L = np.array(params.box(fp))
+    Y = Y - L * (np.arange(64)[:, None] % 2)  # half the segments in (-L, 0)
+    Vo, Wo = O.mobility(fp, Y, F, T)
+    with api.Fcm(fp, 64) as fcm:
+        V, W = fcm.mobility_host(Y, F, T)
+    assert rel(V, Vo) <= 1e-12 and rel(W, Wo) <= 1e-12
+
+
+def test_device_pointers_and_determinism(cuda):
+    torch = cuda
+    fp = params.fcm_params(48)
+    Y, F, T = random_segments(fp, 300, seed=3)
+    dY, dF, dT = (to_dev(torch, a) for a in (Y, F, T))
+    V1 = torch.zeros_like(dY)
+    W1 = torch.zeros_like(dY)
+    V2 = torch.zeros_like(dY)
+    W2 = torch.zeros_like(dY)
+    with api.Fcm(fp, 300) as fcm:
+        fcm.mobility(dY, dF, dT, V1, W1)
+        fcm.mobility(dY, dF, dT, V2, W2)
+        torch.cuda.synchronize()
+    assert torch.equal(V1, V2) and torch.equal(W1, W2)
+    Vo, Wo = O.mobility(fp, Y, F, T)
+    assert rel(V1.cpu().numpy(), Vo) <= 1e-12
+
+
+@pytest.mark.parametrize("batch,tile", [(1, 4), (2, 8), (8, 16), (16, 8)])
+def test_tunings_agree(cuda, batch, tile):
+    fp = params.fcm_params(64)
+    Y, F, T = random_segments(fp, 500, seed=2)
+    Vo, Wo = O.mobility(fp, Y, F, T)
+    with api.Fcm(fp, 500) as fcm:
+        fcm.set_tuning(fft_batch=batch, spread_tile=tile)
+        V, W = fcm.mobility_host(Y, F, T)
+    assert rel(V, Vo) <= 1e-12 and rel(W, Wo) <= 1e-12
+
+
+def test_friction(cuda):
+    torch = cuda
+    fp = params.fcm_params(32)
+    _, F, T = random_segments(fp, 50, seed=9)
+    dF, dT = to_dev(torch, F), to_dev(torch, T)
+    V = torch.zeros_like(dF)
+    W = torch.zeros_like(dF)
+    with api.Fcm(fp, 50) as fcm:
+        fcm.friction(dF, dT, V, W)
+        torch.cuda.synchronize()
+    Vo, Wo = O.friction(fp, F, T)
+    assert np.array_equal(V.cpu().numpy(), Vo)
+    assert np.array_equal(W.cpu().numpy(), Wo)

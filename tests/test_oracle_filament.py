@@ -1,0 +1,204 @@
+"""Pins the oracle filament mechanics, residual, Jacobian and Broyden."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from filsim_amd import params
+from sysfix import config, rich_state
+
+PI = 3.14159265358979323846
+
+
+def rotmat(q):
+    s, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - s * z), 2 * (x * z + s * y)],
+        [2 * (x * y + s * z), 1 - 2 * (x * x + z * z), 2 * (y * z - s * x)],
+        [2 * (x * z - s * y), 2 * (y * z + s * x), 1 - 2 * (x * x + y * y)]])
+
+
+def qmul(p, q):
+    return np.concatenate([[p[0] * q[0] - p[1:] @ q[1:]],
+                           p[0] * q[1:] + q[0] * p[1:] + np.cross(p[1:], q[1:])])
+
+
+def unit_q(rng):
+    q = rng.standard_normal(4)
+    return q / np.linalg.norm(q)
+
+
+def test_quaternion_identities():
+    rng = np.random.default_rng(0)
+    L = O.lib()
+    for _ in range(20):
+        q = unit_q(rng)
+        v = rng.standard_normal(3) * 0.7
+        # expm: q <- exp(v) q = (cos |v|/2, sin(|v|/2) v/|v|) * q
+        qe = q.copy()
+        L.or_quat_expm(O.dp(qe), O.dp(v))
+        th = np.linalg.norm(v)
+        ex = np.concatenate([[np.cos(th / 2)], np.sin(th / 2) * v / th])
+        assert np.allclose(qe, qmul(ex, q), atol=1e-15)
+        assert abs(np.linalg.norm(qe) - 1) < 1e-14
+        # rotation by exp(v) = Rodrigues rotation about v by |v|
+        k = v / th
+        K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+        Rv = np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K
+        assert np.allclose(rotmat(qe), Rv @ rotmat(q), atol=1e-14)
+        # tangent = R(q) e_x; RotateVector = R(q) v
+        t = np.zeros(3)
+        L.or_quat_tangent(O.dp(q), O.dp(t))
+        assert np.allclose(t, rotmat(q)[:, 0], atol=1e-15)
+        w = rng.standard_normal(3)
+        w2 = w.copy()
+        L.or_quat_rotate(O.dp(q), O.dp(w2))
+        assert np.allclose(w2, rotmat(q) @ w, atol=1e-14)
+        # midpoint: m^2 relates p1 to p2 (m p1^* m = p2 up to sign)
+        p1 = unit_q(rng)
+        p2 = qmul(ex, p1)
+        m = np.zeros(4)
+        L.or_quat_midpoint(O.dp(p1), O.dp(p2), O.dp(m))
+        rel_half = qmul(m, np.concatenate([[p1[0]], -p1[1:]]))
+        assert np.allclose(rotmat(qmul(rel_half, rel_half)) @ rotmat(p1),
+                           rotmat(p2), atol=1e-13)
+
+
+def test_dexpinv_is_inverse_of_exp_derivative():
+    """u' = dexpinv(u, w) means exp(u + h u') q = exp(h w) exp(u) q + O(h^2)
+    (both branches of MobilityConstraints.cpp:109-143)."""
+    rng = np.random.default_rng(1)
+    L = O.lib()
+    for scale in (0.9, 1e-13):
+        u = rng.standard_normal(3) * scale
+        w = rng.standard_normal(3)
+        ud = np.zeros(3)
+        L.or_dexpinv(O.dp(u), O.dp(w), O.dp(ud))
+        q0 = unit_q(rng)
+        h = 1e-6
+
+        def ex(v, q):
+            qq = q.copy()
+            L.or_quat_expm(O.dp(qq), O.dp(np.ascontiguousarray(v)))
+            return qq
+        a = ex(u + h * ud, q0)
+        b = ex(h * w, ex(u, q0))
+        if scale < 1e-6:  # exp of a sub-threshold vector is the identity
+            b = ex(h * w, q0)
+        assert np.abs(rotmat(a) - rotmat(b)).max() < 50 * h * h + 1e-12
+
+
+def test_straight_filament_no_torque_constraint_sum():
+    sp, nws, props, st = config(grid=64, nfil=4, nw=10, seed=2,
+                                weight=(0, 0, -1.0))
+    s = O.Sim(sp, nws, props)
+    st = dict(st)
+    st["lam"] = np.random.default_rng(0).standard_normal((4 * 9, 3))
+    s.set_state(**st)
+    s.forces(1)
+    out = s.get_state()
+    # straight, untwisted, no preferred curvature: elastic torque vanishes;
+    # the constraint torques are -dL/2 t x lam on parallel tangents
+    F = out["F"].reshape(4, 10, 3)
+    # constraint forces telescope: sum over a filament = total gravity
+    assert np.allclose(F.sum(1), np.array([0, 0, -1.0 * 2.2 * 10]), atol=1e-12)
+    s2 = O.Sim(sp, nws, props)
+    s2.set_state(**dict(st, lam=np.zeros((36, 3))))
+    s2.forces(1)
+    assert np.abs(s2.get_state()["TAU"]).max() < 1e-14
+
+
+def test_residual_zero_on_exact_translation():
+    """BDF2 integrates uniform sedimentation under the friction mobility
+    exactly (MobilityConstraints.cpp:616-760), so the extrapolated guess
+    already solves the step: zero Broyden iterations."""
+    sp, nws, props, st = config(grid=64, nfil=3, nw=8, seed=4, ifs=1000,
+                                weight=(0.3, -0.2, -1.0), barrier=0)
+    V = np.array([0.3, -0.2, -1.0]) * sp.dL / (6 * PI)
+    X = st["X"]
+    st = dict(st, Xt=X - V * sp.dt, Xs=X)
+    s = O.Sim(sp, nws, props)
+    s.set_state(**st)
+    assert s.timestep(5) == 0
+    out = s.get_state()
+    assert np.abs(out["X"] - (X + V * sp.dt)).max() < 1e-13
+
+
+def _fd_jacobian(sp, nws, props, st, nt, h=1e-6):
+    s = O.Sim(sp, nws, props)
+    n = 6 * sum(nws)
+
+    def resid(u):
+        s.set_state(**st)
+        s.initial_guess()  # qs, Xs from the state (Us, U history)
+        s.set_state(Us=st["Us"], qs=st["qs"], Xs=st["Xs"])
+        s.update(u)
+        s.forces(nt)
+        s.mobility(nt)
+        _, _, _, V, W = s.fcm_arrays()
+        e, _, _ = s.residual(nt, V.copy(), W.copy())
+        return e
+    cols = []
+    for c in range(n):
+        u = np.zeros(n)
+        u[c] = h
+        cols.append(-(resid(u) - resid(-u)) / (2 * h))
+    return np.array(cols).T
+
+
+def test_jacobian_exact_blocks_against_finite_differences():
+    """Blocks of buildApproximateJacobian that are exact derivatives of the
+    residual under the friction mobility at Us = 0 (Filament.hpp:811-853):
+    first-bead rows, BlockC (positions vs Lie elements), BlockD (positions vs
+    tensions, scaled by the 10x pre-factor) and BlockF (Lie vs tensions).
+    Tension columns are in units of lambda*dt, as the reference's J."""
+    sp, nws, props, st0 = config(grid=64, nfil=1, nw=5, seed=6, ifs=1000,
+                                 barrier=0)
+    st = rich_state(st0, nws, seed=3)
+    st["Us"] = np.zeros_like(st["Us"])
+    st["qs"] = st["q"]
+    nt = 5
+    nw = nws[0]
+    s = O.Sim(sp, nws, props)
+    s.set_state(**st)
+    s.forces(nt)
+    s.build_jacobian(nt)
+    J = s.jacobian(0)
+    FD = _fd_jacobian(sp, nws, props, st, nt)
+    lam = slice(3 * nw + 3, 6 * nw)
+    FD[:, lam] /= sp.dt
+    U = slice(3, 3 * nw + 3)
+    tol = 1e-6 * np.abs(J).max()
+    assert np.abs(J[0:3] - FD[0:3]).max() < tol
+    assert np.abs(J[3:3 * nw, U] - FD[3:3 * nw, U]).max() < tol
+    assert np.abs(J[3:3 * nw, lam] - 10 * FD[3:3 * nw, lam]).max() < 10 * tol
+    assert np.abs(J[3 * nw:, lam] - FD[3 * nw:, lam]).max() < tol
+
+
+def test_inverse_and_broyden_secant():
+    sp, nws, props, st0 = config(grid=64, nfil=3, nw=6, seed=7, ifs=1000)
+    st = rich_state(st0, nws, seed=2)
+    s = O.Sim(sp, nws, props)
+    s.set_state(**st)
+    s.initial_guess()
+    s.forces(14)
+    s.build_jacobian(14)
+    for f in range(3):
+        J, Ji = s.jacobian(f), s.jinv(f)
+        D = np.ones(len(J))
+        D[3 * nws[f] + 3:] = sp.dt
+        assert np.abs(np.diag(D) @ Ji @ J - np.eye(len(J))).max() < 1e-10
+    n = 6 * sum(nws)
+    rng = np.random.default_rng(0)
+    E, En, upd = (rng.standard_normal(n) for _ in range(3))
+    s.broyden_update(E, En, upd, 0)
+    assert np.allclose(s.jsolve(En - E, 1), -upd, atol=1e-10)
+
+
+def test_timestep_converges_to_tol():
+    sp, nws, props, st = config(grid=48, nfil=4, nw=8, seed=9, tol=1e-8,
+                                swim=True)
+    s = O.Sim(sp, nws, props)
+    s.set_state(**st)
+    its = [s.timestep(nt) for nt in (1, 2)]
+    assert all(0 < i < sp.broyden_maxiter for i in its)
+    s.forces(2)
