@@ -23,7 +23,7 @@ class FcmParams(C.Structure):
 
 class FcmTuning(C.Structure):
     _fields_ = [("fft_batch", C.c_int), ("spread_tile", C.c_int),
-                ("interp_wave", C.c_int)]
+                ("interp_wave", C.c_int), ("fft_xlines", C.c_int)]
 
 
 class ColbarParams(C.Structure):
@@ -84,6 +84,8 @@ PROTOS = {
     "fs_colbar_link": [P, P, P],
     "fs_colbar_apply": [P, P, I, P, _DP, P],
     "fs_colbar_get_lists": [P, IP, IP],
+    "fs_colbar_link_host": [P, _DP],
+    "fs_colbar_apply_host": [P, _DP, I, _DP, _DP],
     "fs_colbar_pairs": [P, P, I, I, IP, IP],
     "fs_system_create": [C.POINTER(SimParams), I, IP, C.POINTER(FilProps),
                          C.POINTER(P)],

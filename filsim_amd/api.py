@@ -91,8 +91,9 @@ class Fcm:
         abi.check(self.lib.fs_fcm_get_tuning(self.h, C.byref(t)))
         return t
 
-    def set_tuning(self, fft_batch=-1, spread_tile=-1, interp_wave=-1):
-        t = abi.FcmTuning(fft_batch, spread_tile, interp_wave)
+    def set_tuning(self, fft_batch=-1, spread_tile=-1, interp_wave=-1,
+                   fft_xlines=-1):
+        t = abi.FcmTuning(fft_batch, spread_tile, interp_wave, fft_xlines)
         abi.check(self.lib.fs_fcm_set_tuning(self.h, C.byref(t)))
 
 

@@ -47,6 +47,12 @@ def build(verbose=False):
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, OUT)
+    # the C++ driver that mirrors main.cpp through include/filsim_shim.hpp
+    drv = os.path.join(HERE, "filsim_driver")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", drv,
+                    os.path.join(HERE, "host", "driver.cpp"),
+                    "-L" + HERE, "-lfilsim_amd", "-Wl,-rpath,$ORIGIN"],
+                   check=True)
     return OUT
 
 

@@ -25,6 +25,7 @@
 
 #include "colbar_impl.hpp"
 #include "common.hpp"
+#include "scan.hpp"
 
 namespace fs {
 
@@ -60,30 +61,6 @@ __global__ void k_cb_bin(int np, const double *__restrict__ Xs, CbGeom g,
   int c = cb_cell(Xs + 3 * s, g);
   cell[s] = c;
   slot[s] = atomicAdd(&cnt[c], 1);
-}
-
-__global__ void k_cb_scan(const int *__restrict__ cnt, int n,
-                          int *__restrict__ start) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int lo = min(n, t * per), hi = min(n, lo + per);
-  int s = 0;
-  for (int i = lo; i < hi; ++i) s += cnt[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = (t >= off) ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = (t == 0) ? 0 : part[t - 1];
-  for (int i = lo; i < hi; ++i) {
-    start[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) start[n] = part[1023];
 }
 
 __global__ void k_cb_scatter(int np, const int *__restrict__ cell,
@@ -245,6 +222,7 @@ int ColbarImpl::init(const fs_colbar_params &pp, int n, const int *seg_fil,
   FS_TRY(ids.alloc(sizeof(int) * std::max(n, 1)));
   FS_TRY(cnt.alloc(sizeof(int) * ncell));
   FS_TRY(start.alloc(sizeof(int) * (ncell + 1)));
+  FS_TRY(scan_tmp.alloc(sizeof(int) * (ncell / kScanBlock + 4)));
   FS_TRY(sfil.alloc(sizeof(int) * std::max(n, 1)));
   FS_TRY(sseg.alloc(sizeof(int) * std::max(n, 1)));
   FS_TRY(minb.alloc(sizeof(unsigned long long) + sizeof(int)));
@@ -262,6 +240,7 @@ void ColbarImpl::release() {
   ids.release();
   cnt.release();
   start.release();
+  scan_tmp.release();
   sfil.release();
   sseg.release();
   minb.release();
@@ -274,8 +253,8 @@ int ColbarImpl::link(const double *Xs, hipStream_t st) {
   if (np > 0)
     hipLaunchKernelGGL(k_cb_bin, dim3(ceil_div(np, 256)), dim3(256), 0, st, np,
                        Xs, g, cell.as<int>(), cnt.as<int>(), slot.as<int>());
-  hipLaunchKernelGGL(k_cb_scan, dim3(1), dim3(1024), 0, st, cnt.as<int>(),
-                     ncell, start.as<int>());
+  FS_TRY(device_scan(cnt.as<int>(), ncell, start.as<int>(),
+                     scan_tmp.as<int>(), st));
   if (np > 0)
     hipLaunchKernelGGL(k_cb_scatter, dim3(ceil_div(np, 256)), dim3(256), 0, st,
                        np, cell.as<int>(), slot.as<int>(), start.as<int>(),
@@ -439,4 +418,49 @@ extern "C" int fs_colbar_pairs(fs_colbar *h, const double *Y, int nt,
                                int max_pairs, int *pairs, int *npairs) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
   return h->impl.pairs(Y, nt, max_pairs, pairs, npairs);
+}
+
+extern "C" int fs_colbar_link_host(fs_colbar *h, const double *Xs) {
+  FS_CHECK(h && (h->impl.np == 0 || Xs), FS_ERR_ARG, "null argument");
+  int np = h->impl.np;
+  if (np == 0) return h->impl.link(nullptr, nullptr);
+  fs::dbuf d;
+  FS_TRY(d.alloc(sizeof(double) * 3 * (size_t)np));
+  int rc = FS_OK;
+  if (hipMemcpy(d.p, Xs, sizeof(double) * 3 * np, hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    fs::set_error("host to device copy failed");
+    rc = FS_ERR_HIP;
+  }
+  if (rc == FS_OK) rc = h->impl.link(d.as<double>(), nullptr);
+  if (rc == FS_OK && hipDeviceSynchronize() != hipSuccess) {
+    fs::set_error("link failed");
+    rc = FS_ERR_HIP;
+  }
+  d.release();
+  return rc;
+}
+
+extern "C" int fs_colbar_apply_host(fs_colbar *h, const double *Y, int nt,
+                                    double *F, double *min_dist2) {
+  FS_CHECK(h && (h->impl.np == 0 || (Y && F)), FS_ERR_ARG, "null argument");
+  int np = h->impl.np;
+  fs::dbuf d;
+  FS_TRY(d.alloc(sizeof(double) * 6 * (size_t)std::max(np, 1)));
+  double *dY = d.as<double>(), *dF = dY + 3 * np;
+  size_t b = sizeof(double) * 3 * np;
+  int rc = FS_OK;
+  if (np > 0 && (hipMemcpy(dY, Y, b, hipMemcpyHostToDevice) != hipSuccess ||
+                 hipMemcpy(dF, F, b, hipMemcpyHostToDevice) != hipSuccess)) {
+    fs::set_error("host to device copy failed");
+    rc = FS_ERR_HIP;
+  }
+  if (rc == FS_OK) rc = h->impl.apply(dY, nt, dF, min_dist2, nullptr);
+  if (rc == FS_OK && np > 0 &&
+      hipMemcpy(F, dF, b, hipMemcpyDeviceToHost) != hipSuccess) {
+    fs::set_error("device to host copy failed");
+    rc = FS_ERR_HIP;
+  }
+  d.release();
+  return rc;
 }

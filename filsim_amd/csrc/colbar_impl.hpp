@@ -21,7 +21,7 @@ struct ColbarImpl {
   int ncell = 0;
   CbGeom g{};
   bool linked = false;
-  dbuf cell, slot, ids, cnt, start, sfil, sseg, minb;
+  dbuf cell, slot, ids, cnt, start, scan_tmp, sfil, sseg, minb;
 
   int init(const fs_colbar_params &pp, int n, const int *seg_fil,
            const int *seg_idx);

@@ -31,6 +31,7 @@
 #include "common.hpp"
 #include "fcm_impl.hpp"
 #include "fft.hpp"
+#include "scan.hpp"
 
 namespace fs {
 
@@ -97,31 +98,6 @@ __global__ void k_fill_int(int *p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
-// Exclusive scan of n counts into start[0..n] by one workgroup of 1024.
-__global__ void k_scan(const int *__restrict__ cnt, int n,
-                       int *__restrict__ start) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int lo = min(n, t * per), hi = min(n, lo + per);
-  int s = 0;
-  for (int i = lo; i < hi; ++i) s += cnt[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = (t >= off) ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = (t == 0) ? 0 : part[t - 1];
-  for (int i = lo; i < hi; ++i) {
-    start[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) start[n] = part[1023];
-}
-
 __global__ void k_scatter(int np, const int *__restrict__ bin,
                           const int *__restrict__ slot,
                           const int *__restrict__ start, int *__restrict__ ids) {
@@ -148,9 +124,15 @@ __global__ void k_sort_bins(int nbins, const int *__restrict__ start,
 }
 
 // ---------------------------------------------------------------------
-// spread (gather): one workgroup of 256 per tile of tx*ty*tz <= 1024 points
+// spread (gather): one workgroup per grid tile of tx * ty * TZ points, one
+// thread per (x, y) column owning TZ consecutive z points. Candidates (the
+// segments whose stencil origin lies in the bins that can reach the tile)
+// are staged in LDS in a fixed order (bins, then ascending segment id) with
+// their z overlap precomputed; each thread tests x and y once per
+// candidate and accumulates the z run in registers. Every grid point is
+// written once: no atomics, no zeroing pass, deterministic.
 // ---------------------------------------------------------------------
-template <typename T>
+template <typename T, int TZ>
 __global__ void __launch_bounds__(256)
     k_spread(SpreadGeom G, const T *__restrict__ tab,
              const int *__restrict__ origin, const T *__restrict__ coef,
@@ -160,7 +142,8 @@ __global__ void __launch_bounds__(256)
   __shared__ int s_bfirst[kMaxBinsPerTile];
   __shared__ int s_ntot;
   __shared__ T s_tab[kCandChunk][9 * kMaxNgd];
-  __shared__ int s_org[kCandChunk][3];
+  __shared__ int s_org[kCandChunk][2];
+  __shared__ int s_z[kCandChunk][3];  // zlo, zhi, stencil z index of z0
   __shared__ T s_coef[kCandChunk][6];
 
   const int ntiles = G.nbx * G.nby * G.nbz;
@@ -172,7 +155,6 @@ __global__ void __launch_bounds__(256)
   const int bz = bid / (G.nbx * G.nby);
   const int ngd = G.ngd;
 
-  // bins that can reach this tile: origins in [tile - (ngd-1), tile_end)
   const int cx = min(G.hx + 1, G.nbx), cy = min(G.hy + 1, G.nby),
             cz = min(G.hz + 1, G.nbz);
   const int nbins = cx * cy * cz;
@@ -194,131 +176,152 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   const int ntot = s_ntot;
-
-  const int npts = G.tx * G.ty * G.tz;
-  constexpr int PPT = 4;
-  T acc[PPT][3];
-  int px[PPT], py[PPT], pz[PPT];
+  const int lx = threadIdx.x / G.ty, ly = threadIdx.x % G.ty;
+  const bool active = lx < G.tx;
+  const int px = bx * G.tx + lx, py = by * G.ty + ly, pz0 = bz * TZ;
+  T acc[TZ][3];
 #pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    acc[k][0] = acc[k][1] = acc[k][2] = (T)0;
-    int p = threadIdx.x + k * 256;
-    int lz = p % G.tz, ly = (p / G.tz) % G.ty, lx = p / (G.tz * G.ty);
-    px[k] = bx * G.tx + lx;
-    py[k] = by * G.ty + ly;
-    pz[k] = bz * G.tz + lz;
-  }
+  for (int k = 0; k < TZ; ++k) acc[k][0] = acc[k][1] = acc[k][2] = (T)0;
 
   for (int c0 = 0; c0 < ntot; c0 += kCandChunk) {
     const int nc = min(kCandChunk, ntot - c0);
-    // stage candidate tables
-    for (int e = threadIdx.x; e < nc * 9 * ngd; e += 256) {
-      int c = e / (9 * ngd), r = e % (9 * ngd);
+    for (int e = threadIdx.x; e < nc * 9 * ngd; e += blockDim.x) {
+      int c = e / (9 * ngd), r = e - c * (9 * ngd);
       int gc = c0 + c;
       int t = 0;
       while (gc >= s_bstart[t + 1]) ++t;
       int s = ids[s_bfirst[t] + gc - s_bstart[t]];
       s_tab[c][r] = tab[(size_t)s * 9 * ngd + r];
-      if (r < 3) s_org[c][r] = origin[3 * s + r];
+      if (r < 2) s_org[c][r] = origin[3 * s + r];
       if (r < 6) s_coef[c][r] = coef[6 * s + r];
+      if (r == 0) {
+        // z overlap of the stencil [oz, oz+ngd) with [pz0, pz0+TZ) (mod nz;
+        // nz >= TZ + ngd so at most one interval)
+        int oz = origin[3 * s + 2];
+        int o = pz0 - oz;
+        if (o < 0) o += G.nz;
+        int zlo = 0, zhi = 0, izb = 0;
+        if (o < ngd) {
+          zlo = 0;
+          zhi = min(TZ, ngd - o);
+          izb = o;
+        } else if (o + TZ > G.nz) {
+          zlo = G.nz - o;
+          zhi = min(TZ, G.nz - o + ngd);
+          izb = o - G.nz;
+        }
+        s_z[c][0] = zlo;
+        s_z[c][1] = zhi;
+        s_z[c][2] = izb;
+      }
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-      if (threadIdx.x + k * 256 >= npts) break;
+    if (active) {
       for (int c = 0; c < nc; ++c) {
-        int ox = px[k] - s_org[c][0];
-        if (ox < 0) ox += G.nx;
-        int oy = py[k] - s_org[c][1];
-        if (oy < 0) oy += G.ny;
-        int oz = pz[k] - s_org[c][2];
-        if (oz < 0) oz += G.nz;
-        if (ox >= ngd || oy >= ngd || oz >= ngd) continue;
+        int ix = px - s_org[c][0];
+        if (ix < 0) ix += G.nx;
+        int iy = py - s_org[c][1];
+        if (iy < 0) iy += G.ny;
+        if (ix >= ngd || iy >= ngd) continue;
+        const int zlo = s_z[c][0], zhi = s_z[c][1], izb = s_z[c][2];
         const T *tb = s_tab[c];
-        T temp = tb[0 * ngd + ox] * tb[1 * ngd + oy] * tb[2 * ngd + oz];
-        T temp2 = tb[3 * ngd + ox] * tb[4 * ngd + oy] * tb[5 * ngd + oz];
-        T xx = tb[6 * ngd + ox], yy = tb[7 * ngd + oy], zz = tb[8 * ngd + oz];
+        const T gxy = tb[0 * ngd + ix] * tb[1 * ngd + iy];
+        const T dxy = tb[3 * ngd + ix] * tb[4 * ngd + iy];
+        const T xx = tb[6 * ngd + ix], yy = tb[7 * ngd + iy];
         const T *cf = s_coef[c];
-        acc[k][0] += cf[0] * temp + (cf[5] * yy - cf[4] * zz) * temp2;
-        acc[k][1] += cf[1] * temp + (cf[3] * zz - cf[5] * xx) * temp2;
-        acc[k][2] += cf[2] * temp + (cf[4] * xx - cf[3] * yy) * temp2;
+        // constant parts of the dipole terms (g = T/2 pattern, GA_setup)
+        const T ax = cf[5] * yy, ay = -cf[5] * xx, az = cf[4] * xx - cf[3] * yy;
+#pragma unroll
+        for (int k = 0; k < TZ; ++k) {
+          if (k >= zlo && k < zhi) {
+            const int iz = izb + k;
+            const T temp = gxy * tb[2 * ngd + iz];
+            const T temp2 = dxy * tb[5 * ngd + iz];
+            const T zz = tb[8 * ngd + iz];
+            acc[k][0] += cf[0] * temp + (ax - cf[4] * zz) * temp2;
+            acc[k][1] += cf[1] * temp + (cf[3] * zz + ay) * temp2;
+            acc[k][2] += cf[2] * temp + az * temp2;
+          }
+        }
       }
     }
     __syncthreads();
   }
-  const size_t N = (size_t)G.nx * G.ny * G.nz;
+  if (active) {
+    const size_t N = (size_t)G.nx * G.ny * G.nz;
+    const size_t o = ((size_t)px * G.ny + py) * G.nz + pz0;
 #pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    if (threadIdx.x + k * 256 >= npts) break;
-    size_t o = ((size_t)px[k] * G.ny + py[k]) * G.nz + pz[k];
-    R[o] = acc[k][0];
-    R[N + o] = acc[k][1];
-    R[2 * N + o] = acc[k][2];
+    for (int k = 0; k < TZ; ++k) {
+      R[o + k] = acc[k][0];
+      R[N + o + k] = acc[k][1];
+      R[2 * N + o + k] = acc[k][2];
+    }
   }
 }
 
 // ---------------------------------------------------------------------
-// z passes: P row pairs per workgroup, LDS layout [pair][n]
+// FFT passes. Half spectrum layout C[c][kz][y][x] (x fastest): the z pass
+// writes it transposed so the x pass reads contiguous lines and the y pass
+// reads runs of consecutive x.
 // ---------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256)
+
+// z forward: XB = 2P consecutive x rows at one y, as P complex sequences
+// (row 2p in the real part, row 2p+1 in the imaginary part).
+template <typename T, int E>
+__global__ void __launch_bounds__(1024)
     k_fft_z_r2c(const T *__restrict__ R, cx<T> *__restrict__ C, FftPlan pl,
-                const cx<T> *__restrict__ twg, long nrows, int P) {
+                const cx<T> *__restrict__ twg, int nx, int ny, int P) {
   extern __shared__ unsigned char smem_raw[];
-  const int n = pl.n, nh = n / 2 + 1;
+  const int n = pl.n, nh = n / 2 + 1, XB = 2 * P;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *a = tw + n;
-  cx<T> *b = a + (size_t)P * n;
   const int comp = blockIdx.y;
-  const long npairs = (nrows + 1) / 2;
-  const long pair0 = (long)blockIdx.x * P;
-  const T *Rc = R + (size_t)comp * nrows * n;
-  cx<T> *Cc = C + (size_t)comp * nrows * nh;
+  const int nxb = nx / XB;
+  const int x0 = (blockIdx.x % nxb) * XB, y = blockIdx.x / nxb;
+  const T *Rc = R + (size_t)comp * nx * ny * n;
+  cx<T> *Cc = C + (size_t)comp * nh * ny * nx;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
-  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
-    int m = e % n, p = e / n;
-    long pr = pair0 + p;
-    long ra = 2 * pr, rb = 2 * pr + 1;
-    T va = (pr < npairs) ? Rc[(size_t)ra * n + m] : (T)0;
-    T vb = (pr < npairs && rb < nrows) ? Rc[(size_t)rb * n + m] : (T)0;
-    a[p * n + m] = {va, vb};
+  for (int e = threadIdx.x; e < XB * n; e += blockDim.x) {
+    int i = e / n, z = e - i * n;
+    T v = Rc[((size_t)(x0 + i) * ny + y) * n + z];
+    if (i & 1)
+      a[(i >> 1) * n + z].y = v;
+    else
+      a[(i >> 1) * n + z].x = v;
   }
   __syncthreads();
-  cx<T> *res = lds_fft(a, b, pl, P, 1, n, tw, false);
-  for (int e = threadIdx.x; e < P * nh; e += blockDim.x) {
-    int k = e % nh, p = e / nh;
-    long pr = pair0 + p;
-    if (pr >= npairs) continue;
-    long ra = 2 * pr, rb = 2 * pr + 1;
-    cx<T> z = res[p * n + k];
-    cx<T> w = res[p * n + (k == 0 ? 0 : n - k)];
-    const T half = (T)0.5;
-    Cc[(size_t)ra * nh + k] = {half * (z.x + w.x), half * (z.y - w.y)};
-    if (rb < nrows) Cc[(size_t)rb * nh + k] = {half * (z.y + w.y), half * (w.x - z.x)};
+  lds_fft<T, E>(a, pl, P, 1, n, tw, false);
+  const T half = (T)0.5;
+  for (int e = threadIdx.x; e < nh * XB; e += blockDim.x) {
+    int kz = e / XB, i = e - kz * XB;
+    int p = i >> 1;
+    cx<T> z = a[p * n + kz];
+    cx<T> w = a[p * n + (kz == 0 ? 0 : n - kz)];
+    cx<T> o = (i & 1) ? cx<T>{half * (z.y + w.y), half * (w.x - z.x)}
+                      : cx<T>{half * (z.x + w.x), half * (z.y - w.y)};
+    Cc[((size_t)kz * ny + y) * nx + x0 + i] = o;
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256)
+// z inverse: the mirror of k_fft_z_r2c.
+template <typename T, int E>
+__global__ void __launch_bounds__(1024)
     k_fft_z_c2r(const cx<T> *__restrict__ C, T *__restrict__ R, FftPlan pl,
-                const cx<T> *__restrict__ twg, long nrows, int P) {
+                const cx<T> *__restrict__ twg, int nx, int ny, int P) {
   extern __shared__ unsigned char smem_raw[];
-  const int n = pl.n, nh = n / 2 + 1;
+  const int n = pl.n, nh = n / 2 + 1, XB = 2 * P;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *a = tw + n;
-  cx<T> *b = a + (size_t)P * n;
   const int comp = blockIdx.y;
-  const long npairs = (nrows + 1) / 2;
-  const long pair0 = (long)blockIdx.x * P;
-  T *Rc = R + (size_t)comp * nrows * n;
-  const cx<T> *Cc = C + (size_t)comp * nrows * nh;
+  const int nxb = nx / XB;
+  const int x0 = (blockIdx.x % nxb) * XB, y = blockIdx.x / nxb;
+  T *Rc = R + (size_t)comp * nx * ny * n;
+  const cx<T> *Cc = C + (size_t)comp * nh * ny * nx;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
-  for (int e = threadIdx.x; e < P * nh; e += blockDim.x) {
-    int k = e % nh, p = e / nh;
-    long pr = pair0 + p;
-    long ra = 2 * pr, rb = 2 * pr + 1;
-    cx<T> A = (pr < npairs) ? Cc[(size_t)ra * nh + k] : cx<T>{0, 0};
-    cx<T> B = (pr < npairs && rb < nrows) ? Cc[(size_t)rb * nh + k] : cx<T>{0, 0};
+  for (int e = threadIdx.x; e < nh * P; e += blockDim.x) {
+    int k = e / P, p = e - k * P;
+    const cx<T> *src = Cc + ((size_t)k * ny + y) * nx + x0 + 2 * p;
+    cx<T> A = src[0], B = src[1];
     if (k == 0 || 2 * k == n) {  // DC / Nyquist of a real line are real
       A.y = 0;
       B.y = 0;
@@ -327,108 +330,82 @@ __global__ void __launch_bounds__(256)
     if (k > 0 && 2 * k < n) a[p * n + n - k] = {A.x + B.y, B.x - A.y};
   }
   __syncthreads();
-  cx<T> *res = lds_fft(a, b, pl, P, 1, n, tw, true);
-  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
-    int m = e % n, p = e / n;
-    long pr = pair0 + p;
-    if (pr >= npairs) continue;
-    long ra = 2 * pr, rb = 2 * pr + 1;
-    cx<T> z = res[p * n + m];
-    Rc[(size_t)ra * n + m] = z.x;
-    if (rb < nrows) Rc[(size_t)rb * n + m] = z.y;
+  lds_fft<T, E>(a, pl, P, 1, n, tw, true);
+  for (int e = threadIdx.x; e < XB * n; e += blockDim.x) {
+    int i = e / n, z = e - i * n;
+    cx<T> v = a[(i >> 1) * n + z];
+    Rc[((size_t)(x0 + i) * ny + y) * n + z] = (i & 1) ? v.y : v.x;
   }
 }
 
-// ---------------------------------------------------------------------
-// y pass: one workgroup per (comp, x, kz chunk of B columns)
-// ---------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256)
+// y pass: one workgroup per (comp, kz, run of B consecutive x).
+template <typename T, int E>
+__global__ void __launch_bounds__(1024)
     k_fft_y(cx<T> *__restrict__ C, FftPlan pl, const cx<T> *__restrict__ twg,
             int nx, int nh, int B, int inverse) {
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n;  // ny
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *a = tw + n;
-  cx<T> *b = a + (size_t)B * n;
-  const int nchunk = (nh + B - 1) / B;
-  const int chunk = blockIdx.x % nchunk;
-  const int x = blockIdx.x / nchunk;
+  const int nxb = nx / B;
+  const int x0 = (blockIdx.x % nxb) * B, kz = blockIdx.x / nxb;
   const int comp = blockIdx.y;
-  const int kz0 = chunk * B;
-  cx<T> *base = C + ((size_t)comp * nx + x) * n * nh;
+  cx<T> *base = C + ((size_t)comp * nh + kz) * n * nx + x0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
   for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    int col = e % B, m = e / B;
-    int kz = kz0 + col;
-    a[m * B + col] = (kz < nh) ? base[(size_t)m * nh + kz] : cx<T>{0, 0};
+    int m = e / B, col = e - m * B;
+    a[e] = base[(size_t)m * nx + col];
   }
   __syncthreads();
-  cx<T> *res = lds_fft(a, b, pl, B, B, 1, tw, inverse != 0);
+  lds_fft<T, E>(a, pl, B, B, 1, tw, inverse != 0);
   for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    int col = e % B, m = e / B;
-    int kz = kz0 + col;
-    if (kz < nh) base[(size_t)m * nh + kz] = res[m * B + col];
+    int m = e / B, col = e - m * B;
+    base[(size_t)m * nx + col] = a[e];
   }
 }
 
-// ---------------------------------------------------------------------
-// x pass + projection: one workgroup per (y, kz chunk), all 3 components
-// ---------------------------------------------------------------------
 struct ProjConst {
   int nx, ny, nz, nh;
   double kfx, kfy, kfz;  // 2 pi / L per axis
   double scale;          // 1 / (N mu)
 };
 
-template <typename T>
-__global__ void __launch_bounds__(256)
+// x pass + Stokes projection + inverse x pass: one workgroup per (kz, Q
+// consecutive y), the three components' contiguous x lines in LDS.
+template <typename T, int E>
+__global__ void __launch_bounds__(1024)
     k_fft_x_project(cx<T> *__restrict__ C, FftPlan pl,
-                    const cx<T> *__restrict__ twg, ProjConst pc, int B) {
+                    const cx<T> *__restrict__ twg, ProjConst pc, int Q) {
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n;  // nx
   const int nh = pc.nh;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
-  cx<T> *buf[4];
-  buf[0] = tw + n;
-  for (int i = 1; i < 4; ++i) buf[i] = buf[i - 1] + (size_t)B * n;
-  const int nchunk = (nh + B - 1) / B;
-  const int chunk = blockIdx.x % nchunk;
-  const int y = blockIdx.x / nchunk;
-  const int kz0 = chunk * B;
-  const size_t comp_stride = (size_t)pc.nx * pc.ny * nh;
-  const size_t xs = (size_t)pc.ny * nh;
-  cx<T> *base = C + (size_t)y * nh;
+  cx<T> *a = tw + n;  // [3][Q][n]
+  const int nyb = pc.ny / Q;
+  const int y0 = (blockIdx.x % nyb) * Q, kz = blockIdx.x / nyb;
+  const size_t comp_stride = (size_t)nh * pc.ny * n;
+  cx<T> *base = C + ((size_t)kz * pc.ny + y0) * n;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
-  for (int c = 0; c < 3; ++c)
-    for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-      int col = e % B, m = e / B;
-      int kz = kz0 + col;
-      buf[c][m * B + col] =
-          (kz < nh) ? base[c * comp_stride + m * xs + kz] : cx<T>{0, 0};
-    }
-  __syncthreads();
-  cx<T> *cur[3];
-  cx<T> *spare = buf[3];
-  for (int c = 0; c < 3; ++c) {
-    cx<T> *r = lds_fft(buf[c], spare, pl, B, B, 1, tw, false);
-    if (r == spare) {
-      spare = buf[c];
-    }
-    cur[c] = r;
+  const int nseq = 3 * Q;
+  for (int e = threadIdx.x; e < nseq * n; e += blockDim.x) {
+    int sq = e / n, x = e - sq * n;
+    int c = sq / Q, ql = sq - c * Q;
+    a[e] = base[c * comp_stride + (size_t)ql * n + x];
   }
-  // Stokes projection (flow_solve, FCMfunctions.cpp:648-736)
-  const T q2 = (T)(((y <= pc.ny / 2) ? y : y - pc.ny) * pc.kfy);
-  for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    int col = e % B, m = e / B;
-    int kz = kz0 + col;
-    T q1 = (T)(((m <= pc.nx / 2) ? m : m - pc.nx) * pc.kfx);
-    T q3 = (T)(kz * pc.kfz);
+  __syncthreads();
+  lds_fft<T, E>(a, pl, nseq, 1, n, tw, false);
+  // flow_solve (FCMfunctions.cpp:648-736)
+  const T q3 = (T)(kz * pc.kfz);
+  const T sc = (T)pc.scale;
+  for (int e = threadIdx.x; e < Q * n; e += blockDim.x) {
+    int ql = e / n, x = e - ql * n;
+    int y = y0 + ql;
+    T q1 = (T)(((x <= pc.nx / 2) ? x : x - pc.nx) * pc.kfx);
+    T q2 = (T)(((y <= pc.ny / 2) ? y : y - pc.ny) * pc.kfy);
     T qq = q2 * q2 + q1 * q1 + (T)1e-18;
     T norm = (T)1 / (qq + q3 * q3);
-    T sc = (T)pc.scale;
-    cx<T> f1 = cur[0][m * B + col], f2 = cur[1][m * B + col],
-          f3 = cur[2][m * B + col];
+    cx<T> f1 = a[(0 * Q + ql) * n + x], f2 = a[(1 * Q + ql) * n + x],
+          f3 = a[(2 * Q + ql) * n + x];
     cx<T> u1, u2, u3;
     T kf = (q1 * f1.x + q2 * f2.x + q3 * f3.x) * norm;
     u1.x = norm * (f1.x - q1 * kf) * sc;
@@ -438,27 +415,25 @@ __global__ void __launch_bounds__(256)
     u1.y = norm * (f1.y - q1 * kf) * sc;
     u2.y = norm * (f2.y - q2 * kf) * sc;
     u3.y = norm * (f3.y - q3 * kf) * sc;
-    if (m == 0 && y == 0 && kz == 0) u1 = u2 = u3 = cx<T>{0, 0};
-    cur[0][m * B + col] = u1;
-    cur[1][m * B + col] = u2;
-    cur[2][m * B + col] = u3;
+    if (x == 0 && y == 0 && kz == 0) u1 = u2 = u3 = cx<T>{0, 0};
+    a[(0 * Q + ql) * n + x] = u1;
+    a[(1 * Q + ql) * n + x] = u2;
+    a[(2 * Q + ql) * n + x] = u3;
   }
   __syncthreads();
-  for (int c = 0; c < 3; ++c) {
-    cx<T> *r = lds_fft(cur[c], spare, pl, B, B, 1, tw, true);
-    if (r == spare) spare = cur[c];
-    cur[c] = r;
+  lds_fft<T, E>(a, pl, nseq, 1, n, tw, true);
+  for (int e = threadIdx.x; e < nseq * n; e += blockDim.x) {
+    int sq = e / n, x = e - sq * n;
+    int c = sq / Q, ql = sq - c * Q;
+    base[c * comp_stride + (size_t)ql * n + x] = a[e];
   }
-  for (int c = 0; c < 3; ++c)
-    for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-      int col = e % B, m = e / B;
-      int kz = kz0 + col;
-      if (kz < nh) base[c * comp_stride + m * xs + kz] = cur[c][m * B + col];
-    }
 }
 
 // ---------------------------------------------------------------------
-// interpolation: one 64-lane wavefront per segment (bin order)
+// interpolation: one 64-lane wavefront per segment (segments in bin order
+// so neighbours share L2 lines). Lane l owns the (x, y) stencil columns
+// l, l + 64, ... and loads each column's z run at once (NGD independent
+// loads per component in flight), then the wavefront reduces.
 // ---------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -467,47 +442,63 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-template <typename T>
+template <typename T, int NGD>
 __global__ void __launch_bounds__(256)
     k_interp(int np, SpreadGeom G, double dx3, const T *__restrict__ tab,
              const int *__restrict__ origin, const int *__restrict__ ids,
              const T *__restrict__ R, double *__restrict__ V,
              double *__restrict__ W) {
-  __shared__ T s_tab[4][9 * kMaxNgd];
+  __shared__ T s_tab[4][9 * NGD];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int w = blockIdx.x * 4 + wave;
   const bool valid = w < np;
   const int s = valid ? ids[w] : 0;
-  const int ngd = G.ngd;
   if (valid)
-    for (int r = lane; r < 9 * ngd; r += 64)
-      s_tab[wave][r] = tab[(size_t)s * 9 * ngd + r];
+    for (int r = lane; r < 9 * NGD; r += 64)
+      s_tab[wave][r] = tab[(size_t)s * 9 * NGD + r];
   const int ox = valid ? origin[3 * s] : 0, oy = valid ? origin[3 * s + 1] : 0,
             oz = valid ? origin[3 * s + 2] : 0;
   __syncthreads();
   const T *tb = s_tab[wave];
   const size_t N = (size_t)G.nx * G.ny * G.nz;
+  // wrapped z indices of the stencil, shared by every column
+  int kz[NGD];
+#pragma unroll
+  for (int k = 0; k < NGD; ++k) {
+    int z = oz + k;
+    kz[k] = (z >= G.nz) ? z - G.nz : z;
+  }
   T v0 = 0, v1 = 0, v2 = 0, w0 = 0, w1 = 0, w2 = 0;
-  const int npts = valid ? ngd * ngd * ngd : 0;
-  for (int p = lane; p < npts; p += 64) {
-    int k = p % ngd, j = (p / ngd) % ngd, i = p / (ngd * ngd);
+  const int ncol = valid ? NGD * NGD : 0;
+  for (int q = lane; q < ncol; q += 64) {
+    const int i = q / NGD, j = q - (q / NGD) * NGD;
     int ii = ox + i;
     if (ii >= G.nx) ii -= G.nx;
     int jj = oy + j;
     if (jj >= G.ny) jj -= G.ny;
-    int kk = oz + k;
-    if (kk >= G.nz) kk -= G.nz;
-    size_t o = ((size_t)ii * G.ny + jj) * G.nz + kk;
-    T ux = R[o], uy = R[N + o], uz = R[2 * N + o];
-    T temp = tb[0 * ngd + i] * tb[1 * ngd + j] * tb[2 * ngd + k];
-    T temp2 = tb[3 * ngd + i] * tb[4 * ngd + j] * tb[5 * ngd + k];
-    T xx = tb[6 * ngd + i], yy = tb[7 * ngd + j], zz = tb[8 * ngd + k];
-    v0 += ux * temp;
-    v1 += uy * temp;
-    v2 += uz * temp;
-    w0 += (uz * yy - uy * zz) * temp2;
-    w1 += (ux * zz - uz * xx) * temp2;
-    w2 += (uy * xx - ux * yy) * temp2;
+    const size_t rb = ((size_t)ii * G.ny + jj) * G.nz;
+    T ux[NGD], uy[NGD], uz[NGD];
+#pragma unroll
+    for (int k = 0; k < NGD; ++k) {
+      ux[k] = R[rb + kz[k]];
+      uy[k] = R[N + rb + kz[k]];
+      uz[k] = R[2 * N + rb + kz[k]];
+    }
+    const T gxy = tb[0 * NGD + i] * tb[1 * NGD + j];
+    const T dxy = tb[3 * NGD + i] * tb[4 * NGD + j];
+    const T xx = tb[6 * NGD + i], yy = tb[7 * NGD + j];
+#pragma unroll
+    for (int k = 0; k < NGD; ++k) {
+      const T temp = gxy * tb[2 * NGD + k];
+      const T temp2 = dxy * tb[5 * NGD + k];
+      const T zz = tb[8 * NGD + k];
+      v0 += ux[k] * temp;
+      v1 += uy[k] * temp;
+      v2 += uz[k] * temp;
+      w0 += (uz[k] * yy - uy[k] * zz) * temp2;
+      w1 += (ux[k] * zz - uz[k] * xx) * temp2;
+      w2 += (uy[k] * xx - ux[k] * yy) * temp2;
+    }
   }
   v0 = wave_sum(v0);
   v1 = wave_sum(v1);
@@ -538,6 +529,76 @@ __global__ void k_friction(int n3, const double *__restrict__ F,
 // ---------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------
+// Calls f(std::integral_constant<int, E>) for the register tile E of a plan.
+template <typename F>
+static bool with_E(int E, F f) {
+  switch (E) {
+    case 6: f(std::integral_constant<int, 6>{}); return true;
+    case 8: f(std::integral_constant<int, 8>{}); return true;
+    case 10: f(std::integral_constant<int, 10>{}); return true;
+    case 12: f(std::integral_constant<int, 12>{}); return true;
+    case 20: f(std::integral_constant<int, 20>{}); return true;
+    case 24: f(std::integral_constant<int, 24>{}); return true;
+    case 30: f(std::integral_constant<int, 30>{}); return true;
+  }
+  return false;
+}
+
+static int largest_divisor_le(int n, int t);
+
+template <typename F>
+static bool with_TZ(int tz, F f) {
+  switch (tz) {
+    case 4: f(std::integral_constant<int, 4>{}); return true;
+    case 5: f(std::integral_constant<int, 5>{}); return true;
+    case 6: f(std::integral_constant<int, 6>{}); return true;
+    case 8: f(std::integral_constant<int, 8>{}); return true;
+    case 10: f(std::integral_constant<int, 10>{}); return true;
+    case 12: f(std::integral_constant<int, 12>{}); return true;
+    case 16: f(std::integral_constant<int, 16>{}); return true;
+  }
+  return false;
+}
+
+template <typename F>
+static bool with_NGD(int ngd, F f) {
+  switch (ngd) {
+    case 6: f(std::integral_constant<int, 6>{}); return true;
+    case 8: f(std::integral_constant<int, 8>{}); return true;
+    case 10: f(std::integral_constant<int, 10>{}); return true;
+    case 12: f(std::integral_constant<int, 12>{}); return true;
+    case 16: f(std::integral_constant<int, 16>{}); return true;
+    case 20: f(std::integral_constant<int, 20>{}); return true;
+    case 24: f(std::integral_constant<int, 24>{}); return true;
+  }
+  return false;
+}
+
+template <typename T>
+static int set_fft_lds_attr(const FftPlan &plx, const FftPlan &ply,
+                            const FftPlan &plz) {
+  const int lds_max = 160 * 1024;
+  bool ok = true;
+  ok &= with_E(plz.E, [&](auto e) {
+    constexpr int EE = decltype(e)::value;
+    (void)hipFuncSetAttribute((const void *)k_fft_z_r2c<T, EE>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    (void)hipFuncSetAttribute((const void *)k_fft_z_c2r<T, EE>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  });
+  ok &= with_E(ply.E, [&](auto e) {
+    constexpr int EE = decltype(e)::value;
+    (void)hipFuncSetAttribute((const void *)k_fft_y<T, EE>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  });
+  ok &= with_E(plx.E, [&](auto e) {
+    constexpr int EE = decltype(e)::value;
+    (void)hipFuncSetAttribute((const void *)k_fft_x_project<T, EE>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  });
+  FS_CHECK(ok, FS_ERR_ARG, "no FFT register tile for these sizes");
+  return FS_OK;
+}
 static bool good_size(int n) {
   if (n < 2 || n > 1024 || (n & 1)) return false;
   FftPlan p;
@@ -575,10 +636,12 @@ int FcmImpl::init(const fs_fcm_params &pp, int maxnp, int prec) {
   FS_CHECK(good_size(p.nx) && good_size(p.ny) && good_size(p.nz), FS_ERR_ARG,
            "grid %dx%dx%d: each size must be even, <= 1024 and 2^a 3^b 5^c",
            p.nx, p.ny, p.nz);
-  FS_CHECK(p.ngd >= 2 && p.ngd <= kMaxNgd && p.ngd <= p.nx &&
-               p.ngd <= p.ny && p.ngd <= p.nz,
-           FS_ERR_ARG, "ngd=%d must be in [2,%d] and <= every grid size",
-           p.ngd, kMaxNgd);
+  FS_CHECK((p.ngd == 6 || p.ngd == 8 || p.ngd == 10 || p.ngd == 12 ||
+            p.ngd == 16 || p.ngd == 20 || p.ngd == 24) &&
+               p.ngd + 4 <= p.nx && p.ngd + 4 <= p.ny && p.ngd + 4 <= p.nz,
+           FS_ERR_ARG,
+           "ngd=%d must be one of 6,8,10,12,16,20,24 and ngd+4 <= every grid "
+           "size", p.ngd);
   FS_CHECK(p.dx > 0 && p.sigma > 0 && p.sigma_dip > 0 && p.mu > 0 && p.a > 0,
            FS_ERR_ARG, "dx, sigma, sigma_dip, mu, a must be positive");
   FS_CHECK(maxnp >= 1, FS_ERR_ARG, "max_np must be >= 1");
@@ -605,24 +668,16 @@ int FcmImpl::init(const fs_fcm_params &pp, int maxnp, int prec) {
   FS_TRY(bin.alloc((size_t)maxnp * sizeof(int)));
   FS_TRY(slot.alloc((size_t)maxnp * sizeof(int)));
   FS_TRY(ids.alloc((size_t)maxnp * sizeof(int)));
-  tune.fft_batch = (prec == FS_FP64) ? 4 : 8;
-  tune.spread_tile = 8;
+  tune.fft_batch = (prec == FS_FP64) ? 8 : 16;
+  tune.spread_tile = 16;
   tune.interp_wave = 1;
+  tune.fft_xlines = 2;
   FS_TRY(set_geometry());
   for (int i = 0; i < 8; ++i) FS_HIP(hipEventCreate(&ev[i]));
-  // FFT kernels stage up to 160 KB of LDS per workgroup
-  const int lds_max = 160 * 1024;
-  if (prec == FS_FP64) {
-    FS_HIP(hipFuncSetAttribute((const void *)k_fft_z_r2c<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-    FS_HIP(hipFuncSetAttribute((const void *)k_fft_z_c2r<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-    FS_HIP(hipFuncSetAttribute((const void *)k_fft_y<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-    FS_HIP(hipFuncSetAttribute((const void *)k_fft_x_project<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-  } else {
-    FS_HIP(hipFuncSetAttribute((const void *)k_fft_z_r2c<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-    FS_HIP(hipFuncSetAttribute((const void *)k_fft_z_c2r<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-    FS_HIP(hipFuncSetAttribute((const void *)k_fft_y<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-    FS_HIP(hipFuncSetAttribute((const void *)k_fft_x_project<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-  }
+  if (prec == FS_FP64)
+    FS_TRY(set_fft_lds_attr<double>(plx, ply, plz));
+  else
+    FS_TRY(set_fft_lds_attr<float>(plx, ply, plz));
   return FS_OK;
 }
 
@@ -636,46 +691,64 @@ int FcmImpl::set_geometry() {
   int t = tune.spread_tile;
   FS_CHECK(t == 4 || t == 8 || t == 16, FS_ERR_ARG,
            "spread_tile must be 4, 8 or 16");
-  // target tile (x, y, z); z is the contiguous axis
-  int tgt[3] = {8, 8, 8};
-  if (t == 4) tgt[0] = tgt[1] = 4, tgt[2] = 32;
-  if (t == 16) tgt[2] = 16;
-  const int n[3] = {p.nx, p.ny, p.nz};
-  int td[3], bins = 0;
-  for (int attempt = 0; attempt < 6; ++attempt) {
-    bins = 1;
-    for (int d = 0; d < 3; ++d) {
-      td[d] = largest_divisor_le(n[d], tgt[d]);
-      int nb = n[d] / td[d];
-      int h = (p.ngd - 1 + td[d] - 1) / td[d];
-      bins *= std::min(h + 1, nb);
+  // z run per thread: the largest instantiated length dividing nz with
+  // nz >= run + ngd (one overlap interval per stencil)
+  static const int runs[] = {16, 12, 10, 8, 6, 5, 4};
+  int tz = 0;
+  for (int r : runs)
+    if (r <= (precision == FS_FP64 ? 8 : 16) && p.nz % r == 0 &&
+        p.nz >= r + p.ngd) {
+      tz = r;
+      break;
     }
-    if (bins <= kMaxBinsPerTile && td[0] * td[1] * td[2] <= 1024) break;
-    // grow the smallest tile edge and retry
-    int dmin = 0;
-    for (int d = 1; d < 3; ++d)
-      if (td[d] < td[dmin]) dmin = d;
-    tgt[dmin] *= 2;
+  FS_CHECK(tz > 0, FS_ERR_ARG, "no spread z run divides nz=%d", p.nz);
+  int tx = largest_divisor_le(p.nx, t), ty = largest_divisor_le(p.ny, t);
+  int bins = 0;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    int hx = (p.ngd - 1 + tx - 1) / tx, hy = (p.ngd - 1 + ty - 1) / ty,
+        hz = (p.ngd - 1 + tz - 1) / tz;
+    bins = std::min(hx + 1, p.nx / tx) * std::min(hy + 1, p.ny / ty) *
+           std::min(hz + 1, p.nz / tz);
+    if (bins <= kMaxBinsPerTile) break;
+    int nx2 = largest_divisor_le(p.nx, 2 * tx), ny2 = largest_divisor_le(p.ny, 2 * ty);
+    if (nx2 * ny2 > 256) break;
+    tx = nx2;
+    ty = ny2;
   }
-  FS_CHECK(bins <= kMaxBinsPerTile && td[0] * td[1] * td[2] <= 1024,
-           FS_ERR_ARG, "no spread tile fits grid %dx%dx%d with ngd=%d", p.nx,
-           p.ny, p.nz, p.ngd);
+  FS_CHECK(bins <= kMaxBinsPerTile && tx * ty <= 256, FS_ERR_ARG,
+           "no spread tile fits grid %dx%dx%d with ngd=%d", p.nx, p.ny, p.nz,
+           p.ngd);
   geom.nx = p.nx;
   geom.ny = p.ny;
   geom.nz = p.nz;
   geom.ngd = p.ngd;
-  geom.tx = td[0];
-  geom.ty = td[1];
-  geom.tz = td[2];
-  geom.nbx = p.nx / td[0];
-  geom.nby = p.ny / td[1];
-  geom.nbz = p.nz / td[2];
-  geom.hx = (p.ngd - 1 + td[0] - 1) / td[0];
-  geom.hy = (p.ngd - 1 + td[1] - 1) / td[1];
-  geom.hz = (p.ngd - 1 + td[2] - 1) / td[2];
+  geom.tx = tx;
+  geom.ty = ty;
+  geom.tz = tz;
+  geom.nbx = p.nx / tx;
+  geom.nby = p.ny / ty;
+  geom.nbz = p.nz / tz;
+  geom.hx = (p.ngd - 1 + tx - 1) / tx;
+  geom.hy = (p.ngd - 1 + ty - 1) / ty;
+  geom.hz = (p.ngd - 1 + tz - 1) / tz;
   nbins = geom.nbx * geom.nby * geom.nbz;
   FS_TRY(bin_count.alloc((size_t)nbins * sizeof(int)));
   FS_TRY(bin_start.alloc((size_t)(nbins + 1) * sizeof(int)));
+  FS_TRY(scan_tmp.alloc((size_t)(nbins / kScanBlock + 4) * sizeof(int)));
+  // FFT pass geometry: z rows per workgroup (even divisor of nx), y-pass run
+  // of consecutive x, x-pass lines per workgroup; workgroups stay <= 1024
+  zP = 0;
+  for (int xb = std::min(16, p.nx); xb >= 2; --xb)
+    if (xb % 2 == 0 && p.nx % xb == 0 && p.nz / plz.E * (xb / 2) <= 1024) {
+      zP = xb / 2;
+      break;
+    }
+  yB = largest_divisor_le(p.nx, tune.fft_batch);
+  while (yB > 1 && p.ny / ply.E * yB > 1024) yB = largest_divisor_le(p.nx, yB - 1);
+  xQ = largest_divisor_le(p.ny, tune.fft_xlines);
+  while (xQ > 1 && p.nx / plx.E * 3 * xQ > 1024) xQ = largest_divisor_le(p.ny, xQ - 1);
+  FS_CHECK(zP >= 1 && p.ny / ply.E * yB <= 1024 && p.nx / plx.E * 3 * xQ <= 1024,
+           FS_ERR_ARG, "no FFT workgroup shape fits %dx%dx%d", p.nx, p.ny, p.nz);
   return FS_OK;
 }
 
@@ -693,6 +766,7 @@ void FcmImpl::release() {
   ids.release();
   bin_count.release();
   bin_start.release();
+  scan_tmp.release();
   for (int i = 0; i < 8; ++i)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
 }
@@ -726,8 +800,8 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                      np, Y, F, Tq, g, geom.tx, geom.ty, geom.tz, geom.nbx,
                      geom.nby, tab.as<T>(), origin.as<int>(), coef.as<T>(),
                      bin.as<int>(), bin_count.as<int>(), slot.as<int>());
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, bin_count.as<int>(),
-                     nbins, bin_start.as<int>());
+  FS_TRY(device_scan(bin_count.as<int>(), nbins, bin_start.as<int>(),
+                     scan_tmp.as<int>(), st));
   hipLaunchKernelGGL(k_scatter, dim3(ceil_div(np, 256)), dim3(256), 0, st, np,
                      bin.as<int>(), slot.as<int>(), bin_start.as<int>(),
                      ids.as<int>());
@@ -735,26 +809,31 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                      nbins, bin_start.as<int>(), ids.as<int>());
   mark(1);
   const int ntiles = nbins;
-  hipLaunchKernelGGL(k_spread<T>, dim3(ntiles), dim3(256), 0, st, geom,
-                     tab.as<T>(), origin.as<int>(), coef.as<T>(),
-                     bin_start.as<int>(), ids.as<int>(), R.as<T>());
+  bool okz = with_TZ(geom.tz, [&](auto z) {
+    constexpr int TZ = decltype(z)::value;
+    hipLaunchKernelGGL((k_spread<T, TZ>), dim3(ntiles), dim3(geom.tx * geom.ty),
+                       0, st, geom, tab.as<T>(), origin.as<int>(), coef.as<T>(),
+                       bin_start.as<int>(), ids.as<int>(), R.as<T>());
+  });
+  FS_CHECK(okz, FS_ERR_ARG, "spread dispatch failed");
   FS_LAUNCHED();
   mark(2);
-  const long nrows = (long)p.nx * p.ny;
   const int nh = p.nz / 2 + 1;
-  // z: pairs per block so that LDS stays <= 64 KB
-  int P = std::max(1, std::min(16, (int)(65536 / (2 * sizeof(cx<T>) * p.nz))));
-  size_t shz = sizeof(cx<T>) * (p.nz + 2 * (size_t)P * p.nz);
-  long npairs = (nrows + 1) / 2;
-  hipLaunchKernelGGL(k_fft_z_r2c<T>, dim3(ceil_div(npairs, P), 3), dim3(256),
-                     shz, st, R.as<T>(), C.as<cx<T>>(), plz, twz.as<cx<T>>(),
-                     nrows, P);
-  FS_LAUNCHED();
-  const int B = tune.fft_batch;
-  const int nchunk = (nh + B - 1) / B;
-  size_t shy = sizeof(cx<T>) * (p.ny + 2 * (size_t)B * p.ny);
-  hipLaunchKernelGGL(k_fft_y<T>, dim3(p.nx * nchunk, 3), dim3(256), shy, st,
-                     C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, B, 0);
+  const size_t csz = sizeof(cx<T>);
+  bool ok = with_E(plz.E, [&](auto e) {
+    constexpr int EE = decltype(e)::value;
+    hipLaunchKernelGGL((k_fft_z_r2c<T, EE>), dim3(p.ny * (p.nx / (2 * zP)), 3),
+                       dim3(p.nz / EE * zP), csz * (p.nz + (size_t)zP * p.nz), st,
+                       R.as<T>(), C.as<cx<T>>(), plz, twz.as<cx<T>>(), p.nx,
+                       p.ny, zP);
+  });
+  ok &= with_E(ply.E, [&](auto e) {
+    constexpr int EE = decltype(e)::value;
+    hipLaunchKernelGGL((k_fft_y<T, EE>), dim3(nh * (p.nx / yB), 3),
+                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st,
+                       C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 0);
+  });
+  FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
   mark(3);
   ProjConst pc;
@@ -766,22 +845,40 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   pc.kfy = 2.0 * kPi / (p.ny * p.dx);
   pc.kfz = 2.0 * kPi / (p.nz * p.dx);
   pc.scale = 1.0 / ((double)p.nx * p.ny * p.nz * p.mu);
-  size_t shx = sizeof(cx<T>) * (p.nx + 4 * (size_t)B * p.nx);
-  hipLaunchKernelGGL(k_fft_x_project<T>, dim3(p.ny * nchunk), dim3(256), shx,
-                     st, C.as<cx<T>>(), plx, twx.as<cx<T>>(), pc, B);
+  ok = with_E(plx.E, [&](auto e) {
+    constexpr int EE = decltype(e)::value;
+    hipLaunchKernelGGL((k_fft_x_project<T, EE>), dim3(nh * (p.ny / xQ)),
+                       dim3(p.nx / EE * 3 * xQ),
+                       csz * (p.nx + 3 * (size_t)xQ * p.nx), st, C.as<cx<T>>(),
+                       plx, twx.as<cx<T>>(), pc, xQ);
+  });
+  FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
   mark(4);
-  hipLaunchKernelGGL(k_fft_y<T>, dim3(p.nx * nchunk, 3), dim3(256), shy, st,
-                     C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, B, 1);
-  hipLaunchKernelGGL(k_fft_z_c2r<T>, dim3(ceil_div(npairs, P), 3), dim3(256),
-                     shz, st, C.as<cx<T>>(), R.as<T>(), plz, twz.as<cx<T>>(),
-                     nrows, P);
+  ok = with_E(ply.E, [&](auto e) {
+    constexpr int EE = decltype(e)::value;
+    hipLaunchKernelGGL((k_fft_y<T, EE>), dim3(nh * (p.nx / yB), 3),
+                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st,
+                       C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 1);
+  });
+  ok &= with_E(plz.E, [&](auto e) {
+    constexpr int EE = decltype(e)::value;
+    hipLaunchKernelGGL((k_fft_z_c2r<T, EE>), dim3(p.ny * (p.nx / (2 * zP)), 3),
+                       dim3(p.nz / EE * zP), csz * (p.nz + (size_t)zP * p.nz), st,
+                       C.as<cx<T>>(), R.as<T>(), plz, twz.as<cx<T>>(), p.nx,
+                       p.ny, zP);
+  });
+  FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
   mark(5);
   double dx3 = p.dx * p.dx * p.dx;
-  hipLaunchKernelGGL(k_interp<T>, dim3(ceil_div(np, 4)), dim3(256), 0, st, np,
-                     geom, dx3, tab.as<T>(), origin.as<int>(), ids.as<int>(),
-                     R.as<T>(), V, W);
+  bool okn = with_NGD(p.ngd, [&](auto g) {
+    constexpr int NG = decltype(g)::value;
+    hipLaunchKernelGGL((k_interp<T, NG>), dim3(ceil_div(np, 4)), dim3(256), 0,
+                       st, np, geom, dx3, tab.as<T>(), origin.as<int>(),
+                       ids.as<int>(), R.as<T>(), V, W);
+  });
+  FS_CHECK(okn, FS_ERR_ARG, "interp dispatch failed");
   FS_LAUNCHED();
   mark(6);
   return FS_OK;
@@ -927,13 +1024,15 @@ extern "C" int fs_fcm_set_tuning(fs_fcm *h, const fs_fcm_tuning *t) {
   if (t->fft_batch > 0) nt.fft_batch = t->fft_batch;
   if (t->spread_tile > 0) nt.spread_tile = t->spread_tile;
   if (t->interp_wave >= 0) nt.interp_wave = t->interp_wave;
-  FS_CHECK(nt.fft_batch == 1 || nt.fft_batch == 2 || nt.fft_batch == 4 ||
-               nt.fft_batch == 8 || nt.fft_batch == 16,
-           FS_ERR_ARG, "fft_batch must be 1,2,4,8 or 16");
-  // the fused x pass holds 4*B*nx complex in LDS: cap at 160 KB
-  size_t lds = (size_t)(h->impl.tsz * 2) * (h->impl.p.nx + 4 * (size_t)nt.fft_batch * h->impl.p.nx);
-  FS_CHECK(lds <= 160 * 1024, FS_ERR_ARG, "fft_batch %d needs %zu B of LDS",
-           nt.fft_batch, lds);
+  if (t->fft_xlines > 0) nt.fft_xlines = t->fft_xlines;
+  FS_CHECK(nt.fft_batch >= 1 && nt.fft_batch <= 64 && nt.fft_xlines >= 1 &&
+               nt.fft_xlines <= 16,
+           FS_ERR_ARG, "fft_batch must be in [1,64], fft_xlines in [1,16]");
+  // LDS: y pass holds (B+1)*ny complex, x pass (3Q+1)*nx
+  size_t cs = (size_t)h->impl.tsz * 2;
+  size_t lds = std::max(cs * (h->impl.p.ny + (size_t)nt.fft_batch * h->impl.p.ny),
+                        cs * (h->impl.p.nx + 3 * (size_t)nt.fft_xlines * h->impl.p.nx));
+  FS_CHECK(lds <= 160 * 1024, FS_ERR_ARG, "tuning needs %zu B of LDS", lds);
   h->impl.tune = nt;
   int rc = h->impl.set_geometry();
   if (rc != FS_OK) {

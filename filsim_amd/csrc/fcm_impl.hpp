@@ -27,8 +27,9 @@ struct FcmImpl {
   // spread tile geometry (mirrors SpreadGeom in fcm.hip)
   SpreadGeom geom{};
   int nbins = 0;
+  int zP = 1, yB = 1, xQ = 1;  // FFT pass workgroup shapes
   dbuf R, C, twx, twy, twz;
-  dbuf tab, origin, coef, bin, slot, ids, bin_count, bin_start;
+  dbuf tab, origin, coef, bin, slot, ids, bin_count, bin_start, scan_tmp;
   hipEvent_t ev[8] = {};
 
   int init(const fs_fcm_params &pp, int maxnp, int prec);

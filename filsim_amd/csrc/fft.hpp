@@ -2,56 +2,58 @@
 //
 // Replaces FFTW 2.1.5 rfftwnd (FCMfunctions.cpp:16-19, 360-406). A workgroup
 // transforms `ncols` sequences of length n held in LDS; element (m, col) lives
-// at buf[m*sm + col*sc]. Each stage of radix R takes n/R butterflies per
+// at buf[m*sm + col*sc]. A stage of radix R has n/R butterflies per
 // sequence: butterfly j reads inputs j + r*n/R, twiddles them by
 // W_{ns R}^{r (j mod ns)} and writes outputs (j/ns)*ns*R + (j mod ns) + r*ns
-// (decimation in time, autosorting: the result is in natural order). Stages
-// ping-pong between two LDS buffers; the returned pointer holds the result.
-// Transforms are unnormalised; sign -1 forward, +1 inverse.
+// (decimation in time, autosorting: the result is in natural order).
+// Stages run in place: every thread holds E = lcm(radices) complex values
+// (E/R butterflies) in registers across the barrier that separates the
+// reads from the writes, so one LDS buffer holds the data and the workgroup
+// has exactly n*ncols/E threads. Transforms are unnormalised; sign -1
+// forward, +1 inverse.
 #pragma once
 
 #include "common.hpp"
 
 namespace fs {
 
-constexpr int kMaxStages = 10;
+constexpr int kMaxStages = 12;
 
 struct FftPlan {
   int n;
   int nst;
+  int E;  // elements per thread (multiple of every radix)
   int rad[kMaxStages];
 };
 
-// Radices for n = 2^a 3^b 5^c, largest power-of-two radices first.
+inline int gcd_i(int a, int b) { return b == 0 ? a : gcd_i(b, a % b); }
+inline int lcm_i(int a, int b) { return a / gcd_i(a, b) * b; }
+
+// Radices for n = 2^a 3^b 5^c: largest power-of-two radices first, reduced
+// until lcm(radices) <= 30 (the largest register tile instantiated).
 inline bool make_plan(int n, FftPlan *p) {
-  p->n = n;
-  p->nst = 0;
-  int m = n;
-  while (m % 8 == 0 && m > 8 && p->nst < kMaxStages) {
-    p->rad[p->nst++] = 8;
-    m /= 8;
+  int a = 0, b = 0, c = 0, m = n;
+  while (m % 2 == 0) m /= 2, ++a;
+  while (m % 3 == 0) m /= 3, ++b;
+  while (m % 5 == 0) m /= 5, ++c;
+  if (m != 1 || n < 2) return false;
+  for (int maxr2 = 8; maxr2 >= 2; maxr2 /= 2) {
+    p->n = n;
+    p->nst = 0;
+    int r2 = a;
+    while (r2 > 0 && p->nst < kMaxStages) {
+      int r = (r2 >= 3 && maxr2 >= 8) ? 8 : (r2 >= 2 && maxr2 >= 4) ? 4 : 2;
+      p->rad[p->nst++] = r;
+      r2 -= (r == 8 ? 3 : r == 4 ? 2 : 1);
+    }
+    for (int i = 0; i < b && p->nst < kMaxStages; ++i) p->rad[p->nst++] = 3;
+    for (int i = 0; i < c && p->nst < kMaxStages; ++i) p->rad[p->nst++] = 5;
+    int L = 1;
+    for (int s = 0; s < p->nst; ++s) L = lcm_i(L, p->rad[s]);
+    p->E = (L < 6) ? 8 : L;  // n >= 8 with only 2s always has an 8
+    if (p->E <= 30) return true;
   }
-  if (m % 8 == 0 && m == 8) {
-    p->rad[p->nst++] = 8;
-    m = 1;
-  }
-  while (m % 4 == 0 && p->nst < kMaxStages) {
-    p->rad[p->nst++] = 4;
-    m /= 4;
-  }
-  while (m % 2 == 0 && p->nst < kMaxStages) {
-    p->rad[p->nst++] = 2;
-    m /= 2;
-  }
-  while (m % 3 == 0 && p->nst < kMaxStages) {
-    p->rad[p->nst++] = 3;
-    m /= 3;
-  }
-  while (m % 5 == 0 && p->nst < kMaxStages) {
-    p->rad[p->nst++] = 5;
-    m /= 5;
-  }
-  return m == 1;
+  return false;
 }
 
 template <typename T>
@@ -141,14 +143,27 @@ __device__ __forceinline__ void dft5(cx<T> *v, bool inv) {
 }
 
 template <typename T, int R>
-__device__ __forceinline__ void fft_stage(const cx<T> *__restrict__ a,
-                                          cx<T> *__restrict__ b, int n, int ns,
-                                          int ncols, int sm, int sc,
-                                          const cx<T> *tw, bool inv) {
+__device__ __forceinline__ void dftR(cx<T> *v, bool inv) {
+  if (R == 2) dft2(v);
+  if (R == 3) dft3(v, inv);
+  if (R == 4) dft4(v, inv);
+  if (R == 5) dft5(v, inv);
+  if (R == 8) dft8(v, inv);
+}
+
+// One in-place stage; blockDim.x * (E/R) == (n/R) * ncols exactly.
+template <typename T, int R, int E>
+__device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int ns, int ncols,
+                                          int sm, int sc, const cx<T> *tw,
+                                          bool inv) {
+  constexpr int NB = E / R;
   const int nb = n / R;
-  const int total = nb * ncols;
   const int twstep = n / (ns * R);
-  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+  cx<T> v[E];
+  int jj[NB], cc[NB];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    const int t = threadIdx.x + m * blockDim.x;
     int j, col;
     if (sm == 1) {  // sequences contiguous: butterflies fastest
       j = t % nb;
@@ -157,51 +172,51 @@ __device__ __forceinline__ void fft_stage(const cx<T> *__restrict__ a,
       col = t % ncols;
       j = t / ncols;
     }
+    jj[m] = j;
+    cc[m] = col;
     const int k = j % ns;
-    cx<T> v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = a[(j + r * nb) * sm + col * sc];
+    for (int r = 0; r < R; ++r) v[m * R + r] = buf[(j + r * nb) * sm + col * sc];
     if (ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
         cx<T> w = tw[r * k * twstep];
         if (inv) w.y = -w.y;
-        v[r] = cmul(v[r], w);
+        v[m * R + r] = cmul(v[m * R + r], w);
       }
     }
-    if (R == 2) dft2(v);
-    if (R == 3) dft3(v, inv);
-    if (R == 4) dft4(v, inv);
-    if (R == 5) dft5(v, inv);
-    if (R == 8) dft8(v, inv);
-    const int d = (j / ns) * ns * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) b[(d + r * ns) * sm + col * sc] = v[r];
+    dftR<T, R>(&v[m * R], inv);
   }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    const int j = jj[m], col = cc[m];
+    const int d = (j / ns) * ns * R + (j % ns);
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf[(d + r * ns) * sm + col * sc] = v[m * R + r];
+  }
+  __syncthreads();
 }
 
-// Runs all stages; a and b are LDS buffers of the same shape. Returns the
-// buffer holding the result. tw: LDS or global table exp(-2 pi i m / n).
-template <typename T>
-__device__ cx<T> *lds_fft(cx<T> *a, cx<T> *b, const FftPlan &pl, int ncols,
-                          int sm, int sc, const cx<T> *tw, bool inv) {
+// All stages of the plan, in place in buf. Needs blockDim.x = n*ncols/E.
+template <typename T, int E>
+__device__ void lds_fft(cx<T> *buf, const FftPlan &pl, int ncols, int sm,
+                        int sc, const cx<T> *tw, bool inv) {
   int ns = 1;
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.rad[s];
-    switch (R) {
-      case 2: fft_stage<T, 2>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
-      case 3: fft_stage<T, 3>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
-      case 4: fft_stage<T, 4>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
-      case 5: fft_stage<T, 5>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
-      default: fft_stage<T, 8>(a, b, pl.n, ns, ncols, sm, sc, tw, inv); break;
-    }
-    __syncthreads();
-    cx<T> *t = a;
-    a = b;
-    b = t;
+    if constexpr (E % 8 == 0)
+      if (R == 8) fft_stage<T, 8, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
+    if constexpr (E % 4 == 0)
+      if (R == 4) fft_stage<T, 4, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
+    if constexpr (E % 2 == 0)
+      if (R == 2) fft_stage<T, 2, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
+    if constexpr (E % 3 == 0)
+      if (R == 3) fft_stage<T, 3, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
+    if constexpr (E % 5 == 0)
+      if (R == 5) fft_stage<T, 5, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
     ns *= R;
   }
-  return a;
 }
 
 }  // namespace fs

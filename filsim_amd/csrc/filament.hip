@@ -789,6 +789,125 @@ __global__ void __launch_bounds__(1024)
   (void)idt;
 }
 
+// Register-resident variant for n <= 32*NB: 1024 threads as a 32 x 32
+// grid; thread (ty, tx) owns A[i][j] for i = ty + 32a, j = tx + 32b.
+template <int NB>
+__device__ __forceinline__ double pick(const double (&row)[NB], int b) {
+  double v = 0.0;
+#pragma unroll
+  for (int q = 0; q < NB; ++q)
+    if (q == b) v = row[q];
+  return v;
+}
+
+template <int NB>
+__global__ void __launch_bounds__(1024)
+    k_gj_inverse_reg(SysDev S, const double *__restrict__ Jall,
+                     double *__restrict__ JTall, int *__restrict__ singular) {
+  __shared__ double colbuf[32 * NB], rowbuf[32 * NB];
+  __shared__ int used[32 * NB], pivrow[32 * NB], rowstep[32 * NB];
+  __shared__ int s_p;
+  const int f = blockIdx.x;
+  const int nw = S.nw[f];
+  const int n = 6 * nw;
+  const double *J = Jall + S.joff[f];
+  double *JT = JTall + S.joff[f];
+  const int tid = threadIdx.x, ty = tid >> 5, tx = tid & 31;
+  double A[NB][NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int i = ty + 32 * a, j = tx + 32 * b;
+      A[a][b] = (i < n && j < n) ? J[(size_t)i * n + j] : 0.0;
+    }
+  for (int i = tid; i < n; i += blockDim.x) used[i] = 0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const int kb = k >> 5, kt = k & 31;
+    if (tx == kt) {
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        int i = ty + 32 * a;
+        if (i < n) colbuf[i] = pick<NB>(A[a], kb);
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {
+      double best = -1.0;
+      int bi = n;
+      for (int i = tid; i < n; i += 64) {
+        double v = used[i] ? -1.0 : fabs(colbuf[i]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        double ob = __shfl_xor(best, off, 64);
+        int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (tid == 0) {
+        if (!(best > 0.0)) atomicExch(singular, 1);
+        s_p = bi;
+        used[bi] = 1;
+        pivrow[k] = bi;
+        rowstep[bi] = k;
+      }
+    }
+    __syncthreads();
+    const int p = s_p, pb = p >> 5, pt = p & 31;
+    if (ty == pt) {
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+        if (a == pb) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            int j = tx + 32 * b;
+            if (j < n) rowbuf[j] = A[a][b];
+          }
+        }
+    }
+    __syncthreads();
+    const double inv = 1.0 / rowbuf[k];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      const int i = ty + 32 * a;
+      if (i >= n) continue;
+      const double fct = colbuf[i];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int j = tx + 32 * b;
+        if (j >= n) continue;
+        if (i == p) {
+          A[a][b] = (j == k) ? inv : rowbuf[j] * inv;
+        } else if (j == k) {
+          A[a][b] = -fct * inv;
+        } else if (fct != 0.0) {
+          A[a][b] = A[a][b] - fct * (rowbuf[j] * inv);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int i = ty + 32 * a, j = tx + 32 * b;
+      if (i < n && j < n) {
+        int r = rowstep[i], c = pivrow[j];
+        double v = A[a][b];
+        if (r >= 3 * nw + 3) v /= S.dt;
+        JT[(size_t)c * n + r] = v;
+      }
+    }
+}
+
 // ---------------------------------------------------------------------
 // Broyden solve (FilamentJacobianSolver.hpp:49-73)
 // ---------------------------------------------------------------------
@@ -1011,7 +1130,8 @@ int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
   d.props = propsb.as<fs_fil_props>();
   FS_TRY(Jb.alloc(sizeof(double) * (size_t)hjoff[nf]));
   FS_TRY(JTb.alloc(sizeof(double) * (size_t)hjoff[nf]));
-  FS_TRY(Mb.alloc(sizeof(double) * (size_t)hjoff[nf]));
+  max_nw = *std::max_element(hnw.begin(), hnw.end());
+  if (6 * max_nw > 32 * 6) FS_TRY(Mb.alloc(sizeof(double) * (size_t)hjoff[nf]));
   FS_TRY(Cm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
   FS_TRY(Dm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
   dot_chunk = 16384;
@@ -1097,11 +1217,23 @@ int SystemImpl::build_jacobian(int nt) {
   FS_LAUNCHED();
   int *sing = flags.as<int>() + 4;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, sing, 1, 0);
-  // J is kept for inspection; the inverse works on a scratch copy
-  FS_HIP(hipMemcpyAsync(Mb.p, Jb.p, sizeof(double) * nJ,
-                        hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_gj_inverse, dim3(nf), dim3(1024), 0, st, d,
-                     Mb.as<double>(), Jinv_cm(), sing);
+  const int nmax = 6 * max_nw;
+  const int NB = (nmax + 31) / 32;
+  if (NB <= 6) {  // whole matrix in registers of one 1024-thread workgroup
+    switch (NB) {
+      case 1:
+      case 2: hipLaunchKernelGGL(k_gj_inverse_reg<2>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+      case 3: hipLaunchKernelGGL(k_gj_inverse_reg<3>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+      case 4: hipLaunchKernelGGL(k_gj_inverse_reg<4>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+      case 5: hipLaunchKernelGGL(k_gj_inverse_reg<5>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+      default: hipLaunchKernelGGL(k_gj_inverse_reg<6>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+    }
+  } else {  // larger filaments: the matrix stays in global memory (L2)
+    FS_HIP(hipMemcpyAsync(Mb.p, Jb.p, sizeof(double) * nJ,
+                          hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_gj_inverse, dim3(nf), dim3(1024), 0, st, d,
+                       Mb.as<double>(), Jinv_cm(), sing);
+  }
   FS_LAUNCHED();
   // history reset (FilamentJacobianSolver.hpp:40-41)
   size_t nh = 6 * (size_t)np * p.broyden_maxiter;

@@ -36,6 +36,7 @@ struct SystemImpl {
          *dE = nullptr, *scratch6 = nullptr;
   unsigned long long *hflags = nullptr;
   int dot_chunk = 0, dot_blocks = 0;
+  int max_nw = 0;
   double last_res0 = 0, last_res = 0;
   FcmImpl fcm;
   ColbarImpl cb;

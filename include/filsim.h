@@ -83,6 +83,7 @@ typedef struct fs_fcm_tuning {
   int fft_batch;    /* columns per workgroup in the strided FFT passes */
   int spread_tile;  /* tile edge (grid points) of the gather spread   */
   int interp_wave;  /* 1: one wavefront per segment in interpolation  */
+  int fft_xlines;   /* x lines (per component) per workgroup, x pass  */
 } fs_fcm_tuning;
 int fs_fcm_get_tuning(fs_fcm *h, fs_fcm_tuning *t);
 int fs_fcm_set_tuning(fs_fcm *h, const fs_fcm_tuning *t);
@@ -117,6 +118,10 @@ int fs_colbar_link(fs_colbar *h, const double *Xs, void *stream);
  * minimum squared distance over all visited pairs. */
 int fs_colbar_apply(fs_colbar *h, const double *Y, int nt, double *F,
                     double *min_dist2, void *stream);
+/* Host-array variants (copy in/out; synchronous): the shim's LinkedList. */
+int fs_colbar_link_host(fs_colbar *h, const double *Xs);
+int fs_colbar_apply_host(fs_colbar *h, const double *Y, int nt, double *F,
+                         double *min_dist2);
 /* The reference's head[ncell] and list[np] arrays (host out). */
 int fs_colbar_get_lists(fs_colbar *h, int *head, int *list);
 /* Unordered pairs (i<j) inside the cutoff r^2 < 1.21(2a)^2 found through the

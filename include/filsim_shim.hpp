@@ -1,0 +1,204 @@
+// filsim_shim.hpp -- header-only C++ layer over filsim.h that keeps the
+// reference's call surface, so the driver (main.cpp) swaps its solver by
+// changing includes, not call sites.
+//
+//   filsim::Fcm          struct Fcm          (FCMfunctions.hpp:7-59)
+//   filsim::LinkedList   struct LinkedList   (CollisionBarrierFilament.hpp:19-57)
+//   filsim::Filaments    the filaments vector + filamentJsolver + the
+//                        per-filament calls of main.cpp:334-352, 518-547,
+//                        640-663, 704-707 (FilamentJacobianSolver.hpp,
+//                        Filament.hpp), batched over every filament
+//   filsim::ErrorCheckFCM (MobilityConstraints.cpp:447)
+//
+// Differences from the reference, all forced by running on one GPU:
+//   - sizes are runtime parameters (fs_fcm_params / fs_sim_params) instead
+//     of config.hpp macros; Np is fixed at initialise;
+//   - mpi_broadcast() is a no-op (one process owns the device state);
+//   - errors throw filsim::Error (the reference calls MPI_Abort or prints).
+// The host arrays Y/F/T/V/W are double** over one contiguous Np x 3 block,
+// like double_array_2D (c_array_functions.cpp:12-21).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "filsim.h"
+
+namespace filsim {
+
+struct Error : std::runtime_error {
+  explicit Error(const std::string &m) : std::runtime_error(m) {}
+};
+
+inline void check(int rc, const char *what) {
+  if (rc != FS_OK)
+    throw Error(std::string(what) + ": " + fs_last_error());
+}
+
+// Np x 3 contiguous array with double** row pointers.
+struct Array2D {
+  std::vector<double> data;
+  std::vector<double *> rows;
+  void allocate(int n, int m) {
+    data.assign((size_t)n * m, 0.0);
+    rows.resize(n);
+    for (int i = 0; i < n; ++i) rows[i] = data.data() + (size_t)i * m;
+  }
+  double **ptr() { return rows.data(); }
+};
+
+// struct Fcm: host-array interface; each solve copies Y/F/T in and V/W out.
+struct Fcm {
+  double **Y = nullptr, **F = nullptr, **T = nullptr, **V = nullptr,
+         **W = nullptr;
+  int Np = 0;
+  fs_fcm_params params{};
+  int precision = FS_FP64;
+  fs_fcm *h = nullptr;
+  Array2D aY, aF, aT, aV, aW;
+
+  Fcm() = default;
+  Fcm(const fs_fcm_params &p, int np, int prec = FS_FP64)
+      : Np(np), params(p), precision(prec) {}
+
+  // Fcm::initialise (FCMfunctions.cpp:10-168)
+  void initialise(int myrank = 0) {
+    (void)myrank;
+    check(fs_fcm_create(&params, Np, precision, &h), "fs_fcm_create");
+    aY.allocate(Np, 3);
+    aF.allocate(Np, 3);
+    aT.allocate(Np, 3);
+    aV.allocate(Np, 3);
+    aW.allocate(Np, 3);
+    Y = aY.ptr();
+    F = aF.ptr();
+    T = aT.ptr();
+    V = aV.ptr();
+    W = aW.ptr();
+  }
+  // Fcm::mobility_solve (FCMfunctions.cpp:175-464)
+  void mobility_solve() {
+    check(fs_fcm_mobility_host(h, Np, &Y[0][0], &F[0][0], &T[0][0], &V[0][0],
+                               &W[0][0]),
+          "fs_fcm_mobility_host");
+  }
+  // Fcm::friction_mobility_solve (FCMfunctions.cpp:865-880)
+  void friction_mobility_solve() {
+    const double PI = 3.14159265358979323846264338327950288;
+    double a1 = 1. / (6 * PI * params.a * params.mu);
+    double a2 = 1. / (8 * PI * params.a * params.a * params.a * params.mu);
+    for (int i = 0; i < Np; ++i)
+      for (int d = 0; d < 3; ++d) {
+        V[i][d] = a1 * F[i][d];
+        W[i][d] = a2 * T[i][d];
+      }
+  }
+  void mpi_broadcast() {}
+  // Fcm::free_memory (FCMfunctions.cpp:468-519)
+  void free_memory() {
+    if (h) fs_fcm_destroy(h);
+    h = nullptr;
+  }
+};
+
+// struct LinkedList over host positions (copied to the device per call).
+struct LinkedList {
+  double min_dist2 = 999999;
+  fs_colbar *h = nullptr;
+  int Np = 0;
+  fs_colbar_params params{};
+
+  // LinkedList::initialise + set_global_segment_number
+  // (CollisionBarrierFilament.cpp:274-367)
+  void initialise(const fs_colbar_params &p, const std::vector<int> &seg_fil,
+                  const std::vector<int> &seg_idx) {
+    params = p;
+    Np = (int)seg_fil.size();
+    check(fs_colbar_create(&p, Np, seg_fil.data(), seg_idx.data(), &h),
+          "fs_colbar_create");
+  }
+  // LinkedList::link (CollisionBarrierFilament.cpp:370-433), from the
+  // unwrapped positions Xs (Np x 3)
+  void link(double **Xs) { check(fs_colbar_link_host(h, &Xs[0][0]), "link"); }
+  void mpi_broadcast() {}
+  // LinkedList::apply_collision_barrier (CollisionBarrierFilament.cpp:24-271)
+  void apply_collision_barrier(Fcm &fcm, int nt) {
+    check(fs_colbar_apply_host(h, &fcm.Y[0][0], nt, &fcm.F[0][0], &min_dist2),
+          "apply_collision_barrier");
+  }
+  void free_memory() {
+    if (h) fs_colbar_destroy(h);
+    h = nullptr;
+  }
+};
+
+// The filament set with its device-resident state: the reference's
+// std::vector<Filament> + filamentJsolver, one handle.
+struct Filaments {
+  fs_system *h = nullptr;
+  int nfil = 0, Np = 0, Nlam = 0;
+
+  void initialise(const fs_sim_params &p, const std::vector<int> &nw,
+                  const std::vector<fs_fil_props> &props) {
+    check(fs_system_create(&p, (int)nw.size(), nw.data(), props.data(), &h),
+          "fs_system_create");
+    check(fs_system_sizes(h, &nfil, &Np, &Nlam), "fs_system_sizes");
+  }
+  void set_state(const fs_state_ptrs &s) {
+    check(fs_system_set_state(h, &s), "fs_system_set_state");
+  }
+  void get_state(fs_state_ptrs &s) {
+    check(fs_system_get_state(h, &s), "fs_system_get_state");
+  }
+  // for nn: initialGuess(); unRobotArmify();  (main.cpp:334-336)
+  void initialGuess() { check(fs_fil_initial_guess(h), "initialGuess"); }
+  // setZeroForcesTorques .. applyExternalForcesTorques + assign_filament_data
+  void applyForcesTorques(int nt) { check(fs_fil_forces(h, nt), "forces"); }
+  // Jsolver.buildJacobian(filaments, nt) (main.cpp:369)
+  void buildJacobian(int nt) {
+    check(fs_fil_build_jacobian(h, nt), "buildJacobian");
+  }
+  // Jsolver.solve(filaments, b, x, iter) (device vectors of 6 Np)
+  void solve(const double *b, double *x, int iter) {
+    check(fs_fil_jsolve(h, b, x, iter), "solve");
+  }
+  // bad-Broyden columns (main.cpp:640-663)
+  void broydenUpdate(const double *E, const double *Enew, const double *upd,
+                     int iter) {
+    check(fs_fil_broyden_update(h, E, Enew, upd, iter), "broydenUpdate");
+  }
+  // for nn: update(Update); unRobotArmify();  (main.cpp:528-531)
+  void update(const double *upd) { check(fs_fil_update(h, upd), "update"); }
+  // for nn: step(nt)  (main.cpp:704-707)
+  void step(int nt) { check(fs_fil_step(h, nt), "step"); }
+  // fcm.mobility_solve() / friction_mobility_solve() on the device arrays
+  void mobility(int nt) { check(fs_system_mobility(h, nt), "mobility"); }
+  // linked_list.link + apply_collision_barrier on the device arrays
+  double barrier(int nt, bool link) {
+    double md = 0;
+    check(fs_system_barrier(h, nt, link ? 1 : 0, &md), "barrier");
+    return md;
+  }
+  // the whole of main.cpp:284-754 for one nt
+  int timestep(int nt) {
+    int it = 0;
+    check(fs_system_timestep(h, nt, &it), "timestep");
+    return it;
+  }
+  void free_memory() {
+    if (h) fs_system_destroy(h);
+    h = nullptr;
+  }
+};
+
+// ErrorCheckFCM (MobilityConstraints.cpp:447): Error (device, 6 Np) from the
+// device V/W; returns check.
+inline bool ErrorCheckFCM(Filaments &f, double *Error, const double *V,
+                          const double *W, int nt, double *maxabs = nullptr) {
+  int chk = 0;
+  check(fs_fil_residual(f.h, nt, V, W, Error, &chk, maxabs), "ErrorCheckFCM");
+  return chk != 0;
+}
+
+}  // namespace filsim

@@ -1,0 +1,39 @@
+"""The C++ driver (main.cpp's loop over include/filsim_shim.hpp) against
+the oracle run from the same seeded configuration."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from filsim_amd import api, params
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "filsim_amd", "filsim_driver")
+
+
+def test_driver_matches_oracle(cuda, tmp_path):
+    grid, nfil, nw, steps, seed, tol = 48, 6, 10, 2, 5, 1e-8
+    out = tmp_path / "state.bin"
+    r = subprocess.run([DRIVER, str(grid), str(nfil), str(nw), str(steps),
+                        "0", str(seed), repr(tol), str(out)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    raw = np.fromfile(out)
+    npt = nfil * nw
+    X, q = raw[:3 * npt].reshape(npt, 3), raw[3 * npt:].reshape(npt, 4)
+    fp = params.fcm_params(grid)
+    sp = params.sim_params(fp, dt=0.01, tol=tol, weight=(0, 0, -1.0))
+    props = params.fil_props(nfil, kap=1.0)
+    X0, q0 = api.make_filaments(params.box(fp), [nw] * nfil, min_sep=2.2,
+                                seed=seed)
+    o = O.Sim(sp, [nw] * nfil, props)
+    o.set_state(X=X0, Xs=X0, Xt=X0, q=q0, qs=q0, qt=q0)
+    for nt in range(1, steps + 1):
+        o.timestep(nt)
+    so = o.get_state()
+    assert np.abs(X - so["X"]).max() <= 1e-8 * np.abs(so["X"]).max()
+    assert np.abs(q - so["q"]).max() <= 1e-8

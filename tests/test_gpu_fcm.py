@@ -14,6 +14,7 @@ CASES = [
     ((48, 32, 40), 12, None),
     ((64, 64, 64), 12, None),
     ((30, 36, 50), 12, None),
+    ((54, 24, 64), 12, None),
     ((40, 40, 40), 20, "ref"),
 ]
 

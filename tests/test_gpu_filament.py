@@ -131,7 +131,9 @@ def test_whole_timesteps_friction(cuda):
     Lb = np.abs(so["X"]).max()
     assert np.abs(sg["X"] - so["X"]).max() <= 1e-8 * Lb
     assert np.abs(sg["q"] - so["q"]).max() <= 1e-8
-    assert np.abs(sg["lam"] - so["lam"]).max() <= 1e-6 * np.abs(so["lam"]).max()
+    # the residual pins lambda only to ~ TOL * 9 pi mu a / dt (the BlockD
+    # sensitivity, Filament.hpp:913-946)
+    assert np.abs(sg["lam"] - so["lam"]).max() <= 30 * sp.tol / sp.dt
 
 
 def test_whole_timesteps_fcm(cuda):
