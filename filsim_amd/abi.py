@@ -109,6 +109,20 @@ PROTOS = {
     "fs_system_mobility": [P, I],
     "fs_system_barrier": [P, I, I, _DP],
     "fs_make_filaments": [_DP, I, IP, D, D, C.c_ulonglong, I, _DP, _DP],
+    "fs_mlp_create": [I, I, I, C.c_ulonglong, C.POINTER(P)],
+    "fs_mlp_destroy": [P],
+    "fs_mlp_num_params": [P, IP],
+    "fs_mlp_set_norm": [P, FP, FP, FP],
+    "fs_mlp_forward": [P, I, P, P, P],
+    "fs_mlp_train_step": [P, I, P, P, C.c_float, FP, P],
+    "fs_mlp_gradients": [P, I, P, P, FP, I, FP],
+    "fs_mlp_get_params": [P, FP, I],
+    "fs_mlp_set_params": [P, FP, I],
+    "fs_system_set_warmstart": [P, P],
+    "fs_system_warmstart_dims": [P, IP, IP],
+    "fs_system_ws_record": [P, P, P, I],
+    "fs_system_ws_count": [P, IP],
+    "fs_system_last_step_info": [P, IP, _DP, _DP],
 }
 
 _lib = None

@@ -287,3 +287,97 @@ def device_count():
     n = C.c_int()
     abi.check(lib.fs_device_count(C.byref(n)))
     return n.value
+
+
+class Mlp:
+    """The learned warm start's network (fs_mlp_*)."""
+
+    def __init__(self, n_in, n_out, width=256, seed=1):
+        self.lib = abi.load()
+        self.n_in, self.n_out, self.width = n_in, n_out, width
+        self.h = C.c_void_p()
+        abi.check(self.lib.fs_mlp_create(n_in, width, n_out, seed,
+                                         C.byref(self.h)))
+        n = C.c_int()
+        abi.check(self.lib.fs_mlp_num_params(self.h, C.byref(n)))
+        self.nparams = n.value
+        self.dims = [n_in, width, width, width, n_out]
+
+    def close(self):
+        if self.h:
+            self.lib.fs_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def set_norm(self, mu, sigma, tau):
+        mu, sigma, tau = (np.ascontiguousarray(a, dtype=np.float32)
+                          for a in (mu, sigma, tau))
+        abi.check(self.lib.fs_mlp_set_norm(self.h, abi.fptr(mu),
+                                           abi.fptr(sigma), abi.fptr(tau)))
+
+    def forward(self, X, Y, stream=None):
+        abi.check(self.lib.fs_mlp_forward(self.h, X.shape[0], _ptr(X),
+                                          _ptr(Y), _stream(stream)))
+
+    def train_step(self, X, T, lr, want_loss=True, stream=None):
+        loss = C.c_float()
+        abi.check(self.lib.fs_mlp_train_step(
+            self.h, X.shape[0], _ptr(X), _ptr(T), lr,
+            C.byref(loss) if want_loss else None, _stream(stream)))
+        return loss.value
+
+    def gradients(self, X, T):
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_float()
+        abi.check(self.lib.fs_mlp_gradients(self.h, X.shape[0], _ptr(X),
+                                            _ptr(T), abi.fptr(g),
+                                            self.nparams, C.byref(loss)))
+        return loss.value, g
+
+    def get_params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        abi.check(self.lib.fs_mlp_get_params(self.h, abi.fptr(p),
+                                             self.nparams))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        abi.check(self.lib.fs_mlp_set_params(self.h, abi.fptr(p),
+                                             self.nparams))
+
+
+def _sys_ws_dims(system):
+    a, b = C.c_int(), C.c_int()
+    abi.check(system.lib.fs_system_warmstart_dims(system.h, C.byref(a),
+                                                  C.byref(b)))
+    return a.value, b.value
+
+
+def system_warmstart_dims(system):
+    return _sys_ws_dims(system)
+
+
+def system_set_warmstart(system, mlp):
+    abi.check(system.lib.fs_system_set_warmstart(
+        system.h, mlp.h if mlp is not None else None))
+
+
+def system_record(system, X, T, capacity):
+    abi.check(system.lib.fs_system_ws_record(
+        system.h, _ptr(X) if X is not None else None,
+        _ptr(T) if T is not None else None, capacity))
+
+
+def system_record_count(system):
+    n = C.c_int()
+    abi.check(system.lib.fs_system_ws_count(system.h, C.byref(n)))
+    return n.value
+
+
+def system_last_step_info(system):
+    u = C.c_int()
+    a, b = C.c_double(), C.c_double()
+    abi.check(system.lib.fs_system_last_step_info(system.h, C.byref(u),
+                                                  C.byref(a), C.byref(b)))
+    return u.value, a.value, b.value

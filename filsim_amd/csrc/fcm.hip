@@ -158,8 +158,8 @@ __global__ void __launch_bounds__(256)
   const int cx = min(G.hx + 1, G.nbx), cy = min(G.hy + 1, G.nby),
             cz = min(G.hz + 1, G.nbz);
   const int nbins = cx * cy * cz;
-  if (threadIdx.x < nbins) {
-    int t = threadIdx.x;
+  // the workgroup may have fewer threads (tx*ty) than bins: stride
+  for (int t = threadIdx.x; t < nbins; t += blockDim.x) {
     int dz = t / (cx * cy), dy = (t / cx) % cy, dx = t % cx;
     int ix = ((bx - G.hx + dx) % G.nbx + G.nbx) % G.nbx;
     int iy = ((by - G.hy + dy) % G.nby + G.nby) % G.nby;

@@ -991,6 +991,26 @@ __global__ void k_broyden_cols(const double *__restrict__ tmp,
   Dcol[r] = dE[r] / nrm;
 }
 
+// Fused bad-Broyden step. With g = H_k E_new (history columns < k), the
+// reference's second solve H_k dE equals g - Update, so its column
+// C_k = (-H_k dE - Update)/|dE| = -g/|dE|, D_k = dE/|dE|, and the next
+// update H_{k+1} E_new = g + C_k (D_k . E_new) = g (1 - dE.E_new/|dE|^2).
+__global__ void k_broyden_fused(const double *__restrict__ g,
+                                const double *__restrict__ dE,
+                                const double *__restrict__ dots, size_t n,
+                                double *__restrict__ Ccol,
+                                double *__restrict__ Dcol,
+                                double *__restrict__ upd) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double nrm2 = dots[0], dEN = dots[1];
+  const double nrm = sqrt(nrm2);
+  const double gr = g[r];
+  Ccol[r] = -gr / nrm;
+  Dcol[r] = dE[r] / nrm;
+  upd[r] = gr * (1.0 - dEN / nrm2);
+}
+
 // Filament::step (Filament.hpp:455-478) + Bead::step (Bead.hpp:266-276)
 __global__ void k_step(SysDev S, int nt) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1137,7 +1157,7 @@ int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
   dot_chunk = 16384;
   dot_blocks = ceil_div((long)n6, dot_chunk);
   FS_TRY(partial.alloc(sizeof(double) * (size_t)dot_blocks * p.broyden_maxiter));
-  FS_TRY(coefb.alloc(sizeof(double) * (p.broyden_maxiter + 1)));
+  FS_TRY(coefb.alloc(sizeof(double) * (p.broyden_maxiter + 2)));
   FS_TRY(flags.alloc(64));
   FS_HIP(hipHostMalloc((void **)&hflags, 64, hipHostMallocDefault));
   d.np = np;
@@ -1333,31 +1353,61 @@ int SystemImpl::barrier(int nt, int link, double *min_dist2) {
   return cb.apply(d.Ya, nt, d.Fa, min_dist2, st);
 }
 
-// main.cpp:284-754
+// main.cpp:284-754, with the two Jacobian solves of each Broyden iteration
+// (main.cpp:518, 650) fused into one (k_broyden_fused).
 int SystemImpl::timestep(int nt, int *iters) {
-  const size_t n6 = 6 * (size_t)np;
+  FS_TRY(begin_step(nt));
+  FS_TRY(broyden_loop(nt, iters));
+  FS_TRY(step(nt));
+  FS_HIP(hipStreamSynchronize(st));
+  return FS_OK;
+}
+
+// Initial guess, forces, Jacobian, link + barrier, mobility, residual E.
+int SystemImpl::begin_step(int nt) {
   FS_TRY(initial_guess());
   FS_TRY(forces(nt));
   FS_TRY(build_jacobian(nt));
   FS_TRY(barrier(nt, 1, nullptr));
   FS_TRY(mobility(nt));
-  int check = 0;
-  FS_TRY(residual(nt, d.V, d.W, E, &check, &last_res0));
+  return residual(nt, d.V, d.W, E, &check0, &last_res0);
+}
+
+int SystemImpl::broyden_loop(int nt, int *iters) {
+  const size_t n6 = 6 * (size_t)np;
+  int check = check0;
   int iter = 0;
+  last_res = last_res0;
+  if (check) FS_TRY(jsolve(E, Upd, 0));
   while (check && iter < p.broyden_maxiter) {
-    FS_TRY(jsolve(E, Upd, iter));
     FS_TRY(update(Upd));
     FS_TRY(forces(nt));
     FS_TRY(barrier(nt, 0, nullptr));
     FS_TRY(mobility(nt));
     FS_TRY(residual(nt, d.V, d.W, NewE, &check, &last_res));
-    FS_TRY(broyden_update(E, NewE, Upd, iter));
-    FS_HIP(hipMemcpyAsync(E, NewE, sizeof(double) * n6,
-                          hipMemcpyDeviceToDevice, st));
     ++iter;
+    if (!check || iter >= p.broyden_maxiter) break;
+    // dE = NewE - E; |dE|^2 and dE.NewE; g = H_{iter-1} NewE
+    hipLaunchKernelGGL(k_sub, dim3(ceil_div((long)n6, 256)), dim3(256), 0, st,
+                       NewE, E, dE, n6);
+    double *dots = coefb.as<double>() + p.broyden_maxiter;  // 2 scalars
+    hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, 1), dim3(256), 0, st,
+                       dE, n6, 1, dE, n6, dot_chunk, partial.as<double>());
+    hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st,
+                       partial.as<double>(), dot_blocks, 1, dots);
+    hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, 1), dim3(256), 0, st,
+                       dE, n6, 1, NewE, n6, dot_chunk, partial.as<double>());
+    hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st,
+                       partial.as<double>(), dot_blocks, 1, dots + 1);
+    FS_LAUNCHED();
+    FS_TRY(jsolve(NewE, tmp, iter - 1));
+    hipLaunchKernelGGL(k_broyden_fused, dim3(ceil_div((long)n6, 256)),
+                       dim3(256), 0, st, tmp, dE, dots, n6,
+                       Cm.as<double>() + n6 * (iter - 1),
+                       Dm.as<double>() + n6 * (iter - 1), Upd);
+    FS_LAUNCHED();
+    std::swap(E, NewE);
   }
-  FS_TRY(step(nt));
-  FS_HIP(hipStreamSynchronize(st));
   if (iters) *iters = iter;
   return FS_OK;
 }

@@ -1,0 +1,528 @@
+// mlp.hip -- the learned warm start's network: 4 dense layers (n_in -> W ->
+// W -> W -> n_out, ReLU between) trained and evaluated with hand-written
+// BF16 MFMA kernels (v_mfma_f32_16x16x16_bf16, FP32 accumulation).
+//
+// No reference counterpart: the network replaces the linear extrapolation of
+// Bead::initialGuess (Bead.hpp:202-217) as the first Broyden iterate.
+//
+// Numerics (mirrored by the CPU oracle oracle/mlp_oracle.py):
+//   forward   x_n = (x - mu)/sigma rounded to BF16; weights rounded to BF16;
+//             z_l = h_{l-1} W_l^T + b_l in FP32; h_l = BF16(relu(z_l));
+//             y = tau * (h_3 W_3^T + b_3)
+//   loss      mean over batch and outputs of (y_n - t/tau)^2, y_n the
+//             normalised output
+//   backward  g = BF16(dL/dz) feeds both MFMA products (dW = g^T h,
+//             dh = g W); bias gradients from the FP32 dL/dz
+//   optimiser Adam on FP32 master weights (beta 0.9/0.999, eps 1e-8)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+
+namespace fs {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) {
+  return __uint_as_float(((uint32_t)b) << 16);
+}
+
+enum { EPI_STORE = 0, EPI_HIDDEN = 1, EPI_OUT = 2, EPI_DRELU = 3 };
+
+// C(m,n) = sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) =
+// B[k*sbk + n*sbn] (BF16). One wavefront per 32x32 output tile (2x2 MFMA
+// 16x16 blocks), four wavefronts per workgroup. MFMA 16x16x16 lane layout:
+// lane l feeds A[l%16][4(l/16)..+3] and B[4(l/16)..+3][l%16] and receives
+// D[4(l/16)+i][l%16], i < 4.
+// Epilogues: STORE  Cf = acc
+//            HIDDEN Zf = acc + bias, Hb = BF16(relu(Zf))
+//            OUT    Cf = acc + bias
+//            DRELU  g = acc * (Zf > 0): Cf = g, Hb = BF16(g)
+__global__ void __launch_bounds__(256)
+    k_gemm_bf16(int M, int N, int K, const uint16_t *__restrict__ A, long sam,
+                long sak, const uint16_t *__restrict__ B, long sbk, long sbn,
+                int epi, const float *__restrict__ bias, float *__restrict__ Cf,
+                long ldc, uint16_t *__restrict__ Hb, long ldh,
+                const float *__restrict__ Zin, long ldz) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tilesN = (N + 31) / 32;
+  const int tile = blockIdx.x * 4 + wave;
+  const int tm = tile / tilesN, tn = tile - tm * tilesN;
+  if (tm * 32 >= M) return;  // whole wavefront leaves together
+  const int m0 = tm * 32, n0 = tn * 32;
+  const int r = lane & 15, kq = (lane >> 4) * 4;
+  f4v acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    s4v a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int m = m0 + 16 * i + r;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k0 + kq + q;
+        a[i][q] = (m < M && k < K) ? (short)A[m * sam + k * sak] : (short)0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + 16 * j + r;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k0 + kq + q;
+        b[j][q] = (n < N && k < K) ? (short)B[k * sbk + n * sbn] : (short)0;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j],
+                                                             acc[i][j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + 16 * i + kq + q, n = n0 + 16 * j + r;
+        if (m >= M || n >= N) continue;
+        float v = acc[i][j][q];
+        if (epi == EPI_HIDDEN) {
+          v += bias[n];
+          Cf[(long)m * ldc + n] = v;
+          Hb[(long)m * ldh + n] = f2bf(v > 0.f ? v : 0.f);
+        } else if (epi == EPI_OUT) {
+          Cf[(long)m * ldc + n] = v + bias[n];
+        } else if (epi == EPI_DRELU) {
+          float g = Zin[(long)m * ldz + n] > 0.f ? v : 0.f;
+          Cf[(long)m * ldc + n] = g;
+          Hb[(long)m * ldh + n] = f2bf(g);
+        } else {
+          Cf[(long)m * ldc + n] = v;
+        }
+      }
+}
+
+__global__ void k_to_bf16(const float *__restrict__ x, long n,
+                          uint16_t *__restrict__ y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// x_n = BF16((x - mu) / sigma) per column
+__global__ void k_normalise_in(const float *__restrict__ x, int rows, int cols,
+                               const float *__restrict__ mu,
+                               const float *__restrict__ sig,
+                               uint16_t *__restrict__ y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)rows * cols) return;
+  int c = (int)(i % cols);
+  y[i] = f2bf((x[i] - mu[c]) / sig[c]);
+}
+
+__global__ void k_scale_out(float *__restrict__ y, int rows, int cols,
+                            const float *__restrict__ tau) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)rows * cols) return;
+  y[i] *= tau[i % cols];
+}
+
+// dL/dy_n for L = mean((y_n - t/tau)^2); partial loss sums per block.
+__global__ void __launch_bounds__(256)
+    k_mse_grad(const float *__restrict__ yn, const float *__restrict__ t,
+               const float *__restrict__ tau, int rows, int cols,
+               float *__restrict__ g, uint16_t *__restrict__ gb,
+               float *__restrict__ partial) {
+  __shared__ float red[256];
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)rows * cols;
+  float l = 0.f;
+  if (i < n) {
+    float d = yn[i] - t[i] / tau[i % cols];
+    float gi = 2.f * d / (float)n;
+    g[i] = gi;
+    gb[i] = f2bf(gi);
+    l = d * d;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// column sums (bias gradients), fixed order
+__global__ void k_colsum(const float *__restrict__ g, int rows, int cols,
+                         float *__restrict__ out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += g[(long)r * cols + c];
+  out[c] = s;
+}
+
+__global__ void k_sum(const float *__restrict__ p, int n,
+                      float *__restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    float s = 0.f;
+    for (int i = 0; i < n; ++i) s += p[i];
+    *out = s / 1.0f;
+  }
+}
+
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, long n,
+                       float lr, float b1, float b2, float eps, float c1,
+                       float c2, uint16_t *__restrict__ wb) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float mi = b1 * m[i] + (1.f - b1) * gi;
+  float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+struct MlpImpl {
+  int nin = 0, width = 0, nout = 0;
+  int dims[5];
+  long woff[5], boff[5];  // offsets of W_l (out x in) and b_l in params
+  long nparams = 0;
+  dbuf params, grads, adam_m, adam_v, wbf;  // fp32 master + bf16 copies
+  dbuf norm;                                 // mu[nin], sigma[nin], tau[nout]
+  dbuf act;                                  // activations / scratch
+  long act_rows = 0;
+  long step = 0;
+  float *Zf[4];
+  uint16_t *Hb[4];
+  float *Y, *G, *Gtmp;
+  uint16_t *Gb, *Gb2;
+  float *partial, *lossd;
+  hipStream_t st = nullptr;
+
+  int init(int n_in, int w, int n_out, unsigned long long seed) {
+    FS_CHECK(n_in >= 1 && w >= 16 && n_out >= 1 && n_in <= 65536 &&
+                 w <= 4096 && n_out <= 65536,
+             FS_ERR_ARG, "bad MLP sizes %d %d %d", n_in, w, n_out);
+    nin = n_in;
+    width = w;
+    nout = n_out;
+    dims[0] = n_in;
+    dims[1] = dims[2] = dims[3] = w;
+    dims[4] = n_out;
+    long o = 0;
+    for (int l = 0; l < 4; ++l) {
+      woff[l] = o;
+      o += (long)dims[l + 1] * dims[l];
+      boff[l] = o;
+      o += dims[l + 1];
+    }
+    nparams = o;
+    FS_TRY(params.alloc(sizeof(float) * nparams));
+    FS_TRY(grads.alloc(sizeof(float) * nparams));
+    FS_TRY(adam_m.alloc(sizeof(float) * nparams));
+    FS_TRY(adam_v.alloc(sizeof(float) * nparams));
+    FS_TRY(wbf.alloc(sizeof(uint16_t) * nparams));
+    FS_TRY(norm.alloc(sizeof(float) * (2 * nin + nout)));
+    // He-uniform initialisation from a splitmix64 stream, zero biases
+    std::vector<float> h(nparams, 0.f);
+    uint64_t s = seed;
+    auto next = [&]() {
+      uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      return (z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int l = 0; l < 4; ++l) {
+      double lim = std::sqrt(6.0 / dims[l]) * (l == 3 ? 0.1 : 1.0);
+      for (long i = 0; i < (long)dims[l + 1] * dims[l]; ++i)
+        h[woff[l] + i] = (float)((2.0 * next() - 1.0) * lim);
+    }
+    FS_HIP(hipMemcpy(params.p, h.data(), sizeof(float) * nparams,
+                     hipMemcpyHostToDevice));
+    FS_HIP(hipMemset(adam_m.p, 0, sizeof(float) * nparams));
+    FS_HIP(hipMemset(adam_v.p, 0, sizeof(float) * nparams));
+    std::vector<float> nm(2 * nin + nout, 0.f);
+    for (int i = 0; i < nin; ++i) nm[nin + i] = 1.f;
+    for (int i = 0; i < nout; ++i) nm[2 * nin + i] = 1.f;
+    FS_HIP(hipMemcpy(norm.p, nm.data(), sizeof(float) * nm.size(),
+                     hipMemcpyHostToDevice));
+    FS_TRY(requantise());
+    FS_HIP(hipDeviceSynchronize());
+    return FS_OK;
+  }
+  void release() {
+    params.release();
+    grads.release();
+    adam_m.release();
+    adam_v.release();
+    wbf.release();
+    norm.release();
+    act.release();
+  }
+  int requantise() {
+    hipLaunchKernelGGL(k_to_bf16, dim3(ceil_div(nparams, 256)), dim3(256), 0,
+                       st, params.as<float>(), nparams, wbf.as<uint16_t>());
+    FS_LAUNCHED();
+    return FS_OK;
+  }
+  int ensure(long rows) {
+    if (rows <= act_rows) return FS_OK;
+    long w = width, mx = std::max<long>(std::max<long>(nin, nout), width);
+    size_t bytes = 0;
+    bytes += 4 * rows * w * sizeof(float);         // Zf
+    bytes += 4 * rows * mx * sizeof(uint16_t);     // Hb (Hb[0] = x_n)
+    bytes += 3 * rows * mx * sizeof(float);        // Y, G, Gtmp
+    bytes += 2 * rows * mx * sizeof(uint16_t);     // Gb, Gb2
+    bytes += (rows * mx / 256 + 8) * sizeof(float);  // partial + loss
+    bytes += 64 * 16;
+    FS_TRY(act.alloc(bytes));
+    char *p = (char *)act.p;
+    auto take = [&](size_t b) {
+      char *r = p;
+      p += (b + 255) & ~size_t(255);
+      return r;
+    };
+    for (int l = 0; l < 4; ++l) Zf[l] = (float *)take(rows * w * sizeof(float));
+    for (int l = 0; l < 4; ++l) Hb[l] = (uint16_t *)take(rows * mx * 2);
+    Y = (float *)take(rows * mx * sizeof(float));
+    G = (float *)take(rows * mx * sizeof(float));
+    Gtmp = (float *)take(rows * mx * sizeof(float));
+    Gb = (uint16_t *)take(rows * mx * 2);
+    Gb2 = (uint16_t *)take(rows * mx * 2);
+    partial = (float *)take((rows * mx / 256 + 4) * sizeof(float));
+    lossd = (float *)take(16);
+    act_rows = rows;
+    return FS_OK;
+  }
+  void gemm(int M, int N, int K, const uint16_t *A, long sam, long sak,
+            const uint16_t *B, long sbk, long sbn, int epi, const float *bias,
+            float *Cf, long ldc, uint16_t *Hbo, long ldh, const float *Z,
+            long ldz) {
+    long tiles = (long)((M + 31) / 32) * ((N + 31) / 32);
+    hipLaunchKernelGGL(k_gemm_bf16, dim3(ceil_div(tiles, 4)), dim3(256), 0, st,
+                       M, N, K, A, sam, sak, B, sbk, sbn, epi, bias, Cf, ldc,
+                       Hbo, ldh, Z, ldz);
+  }
+  const float *mu() { return norm.as<float>(); }
+  const float *sig() { return norm.as<float>() + nin; }
+  const float *tau() { return norm.as<float>() + 2 * nin; }
+  // forward into Y (normalised output), activations kept for backward
+  int forward_n(int B, const float *X) {
+    FS_TRY(ensure(B));
+    hipLaunchKernelGGL(k_normalise_in, dim3(ceil_div((long)B * nin, 256)),
+                       dim3(256), 0, st, X, B, nin, mu(), sig(), Hb[0]);
+    const uint16_t *W = wbf.as<uint16_t>();
+    const float *P = params.as<float>();
+    const uint16_t *h = Hb[0];
+    long ldh = nin;
+    for (int l = 0; l < 3; ++l) {
+      // z = h W_l^T: B(k=in, n=out) = W_l[out][in]
+      gemm(B, dims[l + 1], dims[l], h, ldh, 1, W + woff[l], 1, dims[l],
+           EPI_HIDDEN, P + boff[l], Zf[l], width, Hb[l + 1], width, nullptr, 0);
+      h = Hb[l + 1];
+      ldh = width;
+    }
+    gemm(B, nout, width, h, width, 1, W + woff[3], 1, width, EPI_OUT,
+         P + boff[3], Y, nout, nullptr, 0, nullptr, 0);
+    FS_LAUNCHED();
+    return FS_OK;
+  }
+  int forward(int B, const float *X, float *Yout) {
+    FS_TRY(forward_n(B, X));
+    FS_HIP(hipMemcpyAsync(Yout, Y, sizeof(float) * B * nout,
+                          hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_scale_out, dim3(ceil_div((long)B * nout, 256)),
+                       dim3(256), 0, st, Yout, B, nout, tau());
+    FS_LAUNCHED();
+    return FS_OK;
+  }
+  // loss and gradients into grads
+  int backward(int B, const float *X, const float *T, float *loss_host) {
+    FS_TRY(forward_n(B, X));
+    const long n = (long)B * nout;
+    const int nblk = ceil_div(n, 256);
+    hipLaunchKernelGGL(k_mse_grad, dim3(nblk), dim3(256), 0, st, Y, T, tau(),
+                       B, nout, G, Gb, partial);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1), 0, st, partial, nblk, lossd);
+    float *g = grads.as<float>();
+    const uint16_t *W = wbf.as<uint16_t>();
+    // layer 3: dW3 = Gb^T H3 (out x width), db3 = colsum(G), dH3 = Gb W3
+    float *gcur = G;
+    uint16_t *gb = Gb;
+    int gcols = nout;
+    for (int l = 3; l >= 0; --l) {
+      const uint16_t *hin = Hb[l];
+      const int in = dims[l], out = dims[l + 1];
+      const long ldin = (l == 0) ? nin : width;
+      // dW_l(o, i) = sum_b g(b, o) h(b, i)
+      gemm(out, in, B, gb, 1, gcols, hin, ldin, 1, EPI_STORE, nullptr,
+           g + woff[l], in, nullptr, 0, nullptr, 0);
+      hipLaunchKernelGGL(k_colsum, dim3(ceil_div(out, 64)), dim3(64), 0, st,
+                         gcur, B, out, g + boff[l]);
+      if (l > 0) {
+        // dh(b, i) = sum_o g(b, o) W_l[o][i]; dz = dh * (z_{l-1} > 0)
+        uint16_t *gb_next = (gb == Gb) ? Gb2 : Gb;
+        float *g_next = (gcur == G) ? Gtmp : G;
+        gemm(B, in, out, gb, gcols, 1, W + woff[l], in, 1, EPI_DRELU, nullptr,
+             g_next, width, gb_next, width, Zf[l - 1], width);
+        gcur = g_next;
+        gb = gb_next;
+        gcols = width;
+      }
+    }
+    FS_LAUNCHED();
+    if (loss_host) {
+      float lv;
+      FS_HIP(hipMemcpyAsync(&lv, lossd, sizeof(float), hipMemcpyDeviceToHost,
+                            st));
+      FS_HIP(hipStreamSynchronize(st));
+      *loss_host = lv / (float)n;
+    }
+    return FS_OK;
+  }
+  int adam(float lr) {
+    ++step;
+    const float b1 = 0.9f, b2 = 0.999f;
+    float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(ceil_div(nparams, 256)), dim3(256), 0, st,
+                       params.as<float>(), grads.as<float>(), adam_m.as<float>(),
+                       adam_v.as<float>(), nparams, lr, b1, b2, 1e-8f, c1, c2,
+                       wbf.as<uint16_t>());
+    FS_LAUNCHED();
+    return FS_OK;
+  }
+};
+
+}  // namespace fs
+
+using fs::MlpImpl;
+
+struct fs_mlp {
+  MlpImpl impl;
+};
+
+namespace fs {
+MlpImpl *mlp_impl(fs_mlp *h) { return h ? &h->impl : nullptr; }
+int mlp_forward(MlpImpl *m, int B, const float *X, float *Y, hipStream_t st) {
+  m->st = st;
+  return m->forward(B, X, Y);
+}
+int mlp_dims(MlpImpl *m, int *nin, int *nout) {
+  *nin = m->nin;
+  *nout = m->nout;
+  return FS_OK;
+}
+}  // namespace fs
+
+extern "C" int fs_mlp_create(int n_in, int width, int n_out,
+                             unsigned long long seed, fs_mlp **out) {
+  FS_CHECK(out, FS_ERR_ARG, "null argument");
+  *out = nullptr;
+  fs_mlp *h = new fs_mlp();
+  int rc = h->impl.init(n_in, width, n_out, seed);
+  if (rc != FS_OK) {
+    h->impl.release();
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return FS_OK;
+}
+
+extern "C" int fs_mlp_destroy(fs_mlp *h) {
+  if (!h) return FS_OK;
+  (void)hipDeviceSynchronize();
+  h->impl.release();
+  delete h;
+  return FS_OK;
+}
+
+extern "C" int fs_mlp_num_params(fs_mlp *h, int *n) {
+  FS_CHECK(h && n, FS_ERR_ARG, "null argument");
+  *n = (int)h->impl.nparams;
+  return FS_OK;
+}
+
+extern "C" int fs_mlp_forward(fs_mlp *h, int batch, const float *X, float *Y,
+                              void *stream) {
+  FS_CHECK(h && X && Y && batch >= 1, FS_ERR_ARG, "bad argument");
+  h->impl.st = (hipStream_t)stream;
+  return h->impl.forward(batch, X, Y);
+}
+
+extern "C" int fs_mlp_train_step(fs_mlp *h, int batch, const float *X,
+                                 const float *T, float lr, float *loss,
+                                 void *stream) {
+  FS_CHECK(h && X && T && batch >= 1 && lr > 0, FS_ERR_ARG, "bad argument");
+  h->impl.st = (hipStream_t)stream;
+  FS_TRY(h->impl.backward(batch, X, T, loss));
+  return h->impl.adam(lr);
+}
+
+extern "C" int fs_mlp_gradients(fs_mlp *h, int batch, const float *X,
+                                const float *T, float *grad, int n_grad,
+                                float *loss) {
+  FS_CHECK(h && X && T && grad && batch >= 1, FS_ERR_ARG, "bad argument");
+  FS_CHECK(n_grad == h->impl.nparams, FS_ERR_SIZE, "n_grad %d != %ld", n_grad,
+           h->impl.nparams);
+  h->impl.st = nullptr;
+  FS_TRY(h->impl.backward(batch, X, T, loss));
+  FS_HIP(hipMemcpy(grad, h->impl.grads.p, sizeof(float) * n_grad,
+                   hipMemcpyDeviceToHost));
+  return FS_OK;
+}
+
+extern "C" int fs_mlp_get_params(fs_mlp *h, float *p, int n) {
+  FS_CHECK(h && p && n == h->impl.nparams, FS_ERR_ARG, "bad argument");
+  FS_HIP(hipDeviceSynchronize());
+  FS_HIP(hipMemcpy(p, h->impl.params.p, sizeof(float) * n,
+                   hipMemcpyDeviceToHost));
+  return FS_OK;
+}
+
+extern "C" int fs_mlp_set_params(fs_mlp *h, const float *p, int n) {
+  FS_CHECK(h && p && n == h->impl.nparams, FS_ERR_ARG, "bad argument");
+  FS_HIP(hipMemcpy(h->impl.params.p, p, sizeof(float) * n,
+                   hipMemcpyHostToDevice));
+  h->impl.st = nullptr;
+  FS_TRY(h->impl.requantise());
+  FS_HIP(hipDeviceSynchronize());
+  return FS_OK;
+}
+
+extern "C" int fs_mlp_set_norm(fs_mlp *h, const float *mu, const float *sigma,
+                               const float *tau) {
+  FS_CHECK(h && mu && sigma && tau, FS_ERR_ARG, "null argument");
+  const int nin = h->impl.nin, nout = h->impl.nout;
+  for (int i = 0; i < nin; ++i)
+    FS_CHECK(sigma[i] > 0, FS_ERR_ARG, "sigma[%d] must be > 0", i);
+  for (int i = 0; i < nout; ++i)
+    FS_CHECK(tau[i] > 0, FS_ERR_ARG, "tau[%d] must be > 0", i);
+  float *d = h->impl.norm.as<float>();
+  FS_HIP(hipMemcpy(d, mu, sizeof(float) * nin, hipMemcpyHostToDevice));
+  FS_HIP(hipMemcpy(d + nin, sigma, sizeof(float) * nin, hipMemcpyHostToDevice));
+  FS_HIP(hipMemcpy(d + 2 * nin, tau, sizeof(float) * nout,
+                   hipMemcpyHostToDevice));
+  return FS_OK;
+}

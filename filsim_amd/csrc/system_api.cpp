@@ -125,6 +125,47 @@ extern "C" int fs_system_timestep(fs_system *h, int nt, int *iters) {
   return h->impl.timestep(nt, iters);
 }
 
+namespace fs {
+struct MlpImpl;
+MlpImpl *mlp_impl(fs_mlp *h);
+}  // namespace fs
+
+extern "C" int fs_system_set_warmstart(fs_system *h, fs_mlp *m) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->ws.attach(h->impl, fs::mlp_impl(m));
+}
+
+extern "C" int fs_system_warmstart_dims(fs_system *h, int *n_in, int *n_out) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  for (int f = 1; f < h->impl.nf; ++f)
+    FS_CHECK(h->impl.hnw[f] == h->impl.hnw[0], FS_ERR_ARG,
+             "the warm start needs one segment count for every filament");
+  if (n_in) *n_in = 15 * h->impl.hnw[0] - 4;
+  if (n_out) *n_out = 6 * h->impl.hnw[0];
+  return FS_OK;
+}
+
+extern "C" int fs_system_ws_record(fs_system *h, float *X, float *T,
+                                   int capacity) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->ws.record(h->impl, X, T, capacity);
+}
+
+extern "C" int fs_system_ws_count(fs_system *h, int *count) {
+  FS_CHECK(h && count, FS_ERR_ARG, "null argument");
+  *count = h->ws.rec_count;
+  return FS_OK;
+}
+
+extern "C" int fs_system_last_step_info(fs_system *h, int *used_mlp,
+                                        double *res_plain, double *res_mlp) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  if (used_mlp) *used_mlp = h->ws.used_mlp;
+  if (res_plain) *res_plain = h->ws.tracking ? h->ws.res_plain : h->impl.last_res0;
+  if (res_mlp) *res_mlp = h->ws.res_mlp;
+  return FS_OK;
+}
+
 extern "C" int fs_fil_initial_guess(fs_system *h) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
   return h->impl.initial_guess();

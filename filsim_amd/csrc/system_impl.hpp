@@ -59,6 +59,9 @@ struct SystemImpl {
   int mobility(int nt);
   int barrier(int nt, int link, double *min_dist2);
   int timestep(int nt, int *iters);
+  int begin_step(int nt);
+  int broyden_loop(int nt, int *iters);
+  int check0 = 0;
 };
 
 }  // namespace fs

@@ -1,14 +1,33 @@
-// warmstart.hpp -- optional learned warm start attached to a system.
+// warmstart.hpp -- learned first Broyden iterate attached to a system.
+//
+// No reference counterpart: replaces the extrapolated guess of
+// Bead::initialGuess (Bead.hpp:202-217) by guess + MLP correction when that
+// lowers the residual's infinity norm, else keeps the extrapolation.
 #pragma once
 
 #include "system_impl.hpp"
 
 namespace fs {
 
+struct MlpImpl;
+
 struct WarmStart {
-  bool attached() const { return false; }
-  int timestep(SystemImpl &, int, int *) { return FS_ERR_STATE; }
-  void release() {}
+  MlpImpl *mlp = nullptr;
+  bool recording = false;
+  bool tracking = false;  // velocity history kept (attached or recording)
+  int nw = 0, nin = 0, nout = 0;
+  dbuf V1, V2, feat, pred, u, sXs, sUs, sqs, slam;
+  float *recX = nullptr, *recT = nullptr;
+  int rec_cap = 0, rec_count = 0;
+  int used_mlp = 0;
+  double res_plain = 0, res_mlp = 0;
+
+  bool attached() const { return tracking; }
+  int setup(SystemImpl &S);  // sizes + buffers (uniform segment count)
+  int attach(SystemImpl &S, MlpImpl *m);
+  int record(SystemImpl &S, float *X, float *T, int capacity);
+  int timestep(SystemImpl &S, int nt, int *iters);
+  void release();
 };
 
 }  // namespace fs

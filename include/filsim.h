@@ -232,6 +232,50 @@ int fs_make_filaments(const double box[3], int nfil, const int *nw,
                       double dL, double min_sep, unsigned long long seed,
                       int max_tries, double *X, double *q);
 
+/* ------------------------------------------------------------------ */
+/* Learned warm start (no reference counterpart: replaces the linear   */
+/* extrapolation of Bead::initialGuess, Bead.hpp:202-217, by an MLP    */
+/* correction, kept only when it lowers the residual's infinity norm)  */
+/* ------------------------------------------------------------------ */
+typedef struct fs_mlp fs_mlp;
+/* MLP n_in -> width -> width -> width -> n_out (ReLU), BF16 weights on
+ * MFMA with FP32 accumulation, FP32 master weights, Adam. */
+int fs_mlp_create(int n_in, int width, int n_out, unsigned long long seed,
+                  fs_mlp **out);
+int fs_mlp_destroy(fs_mlp *h);
+int fs_mlp_num_params(fs_mlp *h, int *n);
+/* Feature/target normalisation: x_n = (x - mu)/sigma, y = tau * y_n. */
+int fs_mlp_set_norm(fs_mlp *h, const float *mu, const float *sigma,
+                    const float *tau);
+/* X (device, batch x n_in) -> Y (device, batch x n_out), fp32. */
+int fs_mlp_forward(fs_mlp *h, int batch, const float *X, float *Y,
+                   void *stream);
+/* One Adam step on the mean squared normalised error against T (device,
+ * batch x n_out); *loss (host, may be NULL) receives the loss. */
+int fs_mlp_train_step(fs_mlp *h, int batch, const float *X, const float *T,
+                      float lr, float *loss, void *stream);
+/* Loss gradient w.r.t. all parameters (host out, fp32) packed as
+ * W1,b1,W2,b2,W3,b3,W4,b4 with W_l row-major [out][in]. */
+int fs_mlp_gradients(fs_mlp *h, int batch, const float *X, const float *T,
+                     float *grad, int n_grad, float *loss);
+int fs_mlp_get_params(fs_mlp *h, float *p, int n);
+int fs_mlp_set_params(fs_mlp *h, const float *p, int n);
+
+/* Attach a network to a system (NULL detaches). Needs one segment count
+ * Nw for every filament; the network maps 15 Nw - 4 features to 6 Nw
+ * outputs (fs_system_warmstart_dims). */
+int fs_system_set_warmstart(fs_system *h, fs_mlp *m);
+int fs_system_warmstart_dims(fs_system *h, int *n_in, int *n_out);
+/* Record (features, target) of every following timestep into device
+ * buffers X (capacity x n_in) and T (capacity x n_out), one row per
+ * filament per step, until capacity rows are filled. NULL stops. */
+int fs_system_ws_record(fs_system *h, float *X, float *T, int capacity);
+int fs_system_ws_count(fs_system *h, int *count);
+/* Whether the last timestep started from the network's guess, and the
+ * infinity norms of the residuals of both candidate guesses. */
+int fs_system_last_step_info(fs_system *h, int *used_mlp, double *res_plain,
+                             double *res_mlp);
+
 #ifdef __cplusplus
 }
 #endif
