@@ -8,11 +8,15 @@ BDF2 + bad Broyden with TOL 1e-6. A step is one whole implicit timestep
 (initial guess, forces, Jacobian, cell list, barrier, Broyden iterations of
 mobility solves, state step), all resident in HBM.
 
-Reported `value`: timesteps/s of the mixed-precision path (FCM stages in
-FP32, Broyden/tensions/BDF2 state in FP64). Also reported: the plain FP64
-path on the same system, mobility solves/s of the 256^3 / 30,000-segment
-FCM microbenchmark in FP64 and FP32, and the roofline of the dominant
-kernel measured with HIP events on the solver's own stream.
+Reported `value`: timesteps/s of the mixed-precision + learned-warm-start
+path (FCM stages in FP32, Broyden/tensions/BDF2 state in FP64, first
+Broyden iterate from an MLP trained on the GPU on --record steps of the
+plain solver's own trajectory, kept only when its residual is lower).
+Also reported, from the same state and TOL: plain mixed precision and
+plain FP64 timesteps/s, Broyden iterations, the position deviation from the
+FP64 path, mobility solves/s of the 256^3 / 30,000-segment FCM
+microbenchmark in FP64 and FP32, and the roofline of the dominant kernel
+measured with HIP events on the solver's own stream.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -22,6 +26,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -128,10 +134,13 @@ def cpu_baseline(fp_iters):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mob-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--record", type=int, default=50,
+                    help="plain steps recorded to train the warm start")
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--only", choices=["all", "mixed", "fp64", "mobility"],
                     default="all")
     args = ap.parse_args()
@@ -171,25 +180,78 @@ def main():
     xk_ms = float(st32[4])
 
     res = {}
+    info = {}
     if args.only in ("all", "fp64", "mixed"):
-        for name, prec in (("mixed", abi.FS_FP32), ("fp64", abi.FS_FP64)):
-            if args.only not in ("all", name):
-                continue
-            s, _ = build_system(prec)
-            el, its, nt = time_steps(s, args.warmup, args.steps, torch)
-            s.close()
+        from filsim_amd import api, warmstart
+        # 1) the mixed-precision solver generates the training trajectory
+        s_l, _ = build_system(abi.FS_FP32)
+        t0 = time.perf_counter()
+        X, T, its_rec = warmstart.record(s_l, args.record, nt0=1, torch=torch)
+        t_rec = time.perf_counter() - t0
+        log("recorded %d samples in %.1f s (iters %s)" %
+            (X.shape[0], t_rec, its_rec[-5:]))
+        t0 = time.perf_counter()
+        mlp, hist = warmstart.train(X, T, torch, epochs=args.epochs,
+                                    batch=1024, lr=1e-3, log=log)
+        t_train = time.perf_counter() - t0
+        del X, T
+        nt0 = args.record + 1
+        state = s_l.get_state()
+        keep = {k: state[k] for k in ("X", "Xt", "U", "Ut", "q", "qt", "lam",
+                                      "lam1", "lam2")}
+        keep.update(Xs=state["X"], Us=state["U"], qs=state["q"])
+        api.system_set_warmstart(s_l, mlp)
+        info["train"] = {"samples": args.record * 1000, "record_s": t_rec,
+                         "train_s": t_train, "epochs": args.epochs,
+                         "holdout_loss": [h[2] for h in hist]}
+        # 2) the same state, three paths, same TOL
+        finals = {}
+        for name, prec in (("learned", None), ("mixed", abi.FS_FP32),
+                           ("fp64", abi.FS_FP64)):
+            if name == "learned":
+                s = s_l
+            else:
+                s, _ = build_system(prec)
+                s.set_state(**keep)
+            its, used = [], 0
+            nt = nt0
+            for _ in range(args.warmup):
+                s.timestep(nt)
+                nt += 1
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                its.append(s.timestep(nt))
+                if name == "learned":
+                    used += api.system_last_step_info(s)[0]
+                nt += 1
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            finals[name] = s.get_state()["X"]
             res[name] = (el, its)
-            log("%s: %d steps in %.3f s -> %.3f steps/s, iters %s" %
-                (name, args.steps, el, args.steps / el, its))
+            log("%s: %d steps in %.3f s -> %.3f steps/s, iters %s%s" %
+                (name, args.steps, el, args.steps / el, its,
+                 " (network guess used %d/%d)" % (used, args.steps)
+                 if name == "learned" else ""))
+            if name == "learned":
+                info["learned_used"] = used
+            s.close()
+        ref = finals["fp64"]
+        for name in ("learned", "mixed"):
+            info["pos_rel_err_vs_fp64_" + name] = float(
+                np.abs(finals[name] - ref).max() / np.abs(ref).max())
+        mlp.close()
 
-    value = args.steps / res["mixed"][0] if "mixed" in res else None
-    if "fp64" in res:
-        out["fp64_timesteps_per_s"] = args.steps / res["fp64"][0]
-        out["fp64_iters"] = res["fp64"][1]
-    if "mixed" in res:
-        out["mixed_iters"] = res["mixed"][1]
+    value = args.steps / res["learned"][0] if "learned" in res else None
+    for name in ("mixed", "fp64", "learned"):
+        if name in res:
+            out[name + "_timesteps_per_s"] = args.steps / res[name][0]
+            out[name + "_iters"] = res[name][1]
     if "fp64" in res and value:
         out["speedup_vs_fp64"] = value / out["fp64_timesteps_per_s"]
+        out["mixed_speedup_vs_fp64"] = (out["mixed_timesteps_per_s"] /
+                                        out["fp64_timesteps_per_s"])
+    out.update(info)
 
     cpu = None
     if not args.no_cpu:
@@ -204,14 +266,14 @@ def main():
                           (t_solve, sps))}
         log("cpu baseline", cpu)
 
-    ms = 1e3 * res["mixed"][0] / args.steps if "mixed" in res else None
+    ms = 1e3 * res["learned"][0] / args.steps if "learned" in res else None
     line = {
         "metric": "timesteps/s at 256^3, 30k segs: mixed+learned vs plain "
                   "FP64 on GPU, at fixed error",
         "value": value, "unit": "timesteps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 FCM + f64 Broyden", "data": "synthetic",
+        "dtype": "f32 FCM + f64 Broyden + bf16 MLP", "data": "synthetic",
         "config": {"workload": "1000 swimming filaments x 30 segments, "
                                "256^3 periodic FCM grid, dx=a/1.86, ngd=12, "
                                "KB=1000, dt=0.005, TOL=1e-6, barrier on",

@@ -131,7 +131,7 @@ __device__ __forceinline__ int cb_wrap(int v, int m) {
 
 // mode 0: F[i] += sum of pair forces, min distance; mode 1: count pairs
 // (i<j) inside the cutoff into *npairs and write them when room.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
     k_cb_apply(int np, const double *__restrict__ Y, CbGeom g, CbConst c,
                const int *__restrict__ cell, const int *__restrict__ start,
                const int *__restrict__ ids, const int *__restrict__ sfil,
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(256)
                unsigned long long *__restrict__ min_bits, int mode,
                int *__restrict__ npairs, int max_pairs,
                int *__restrict__ pairs) {
-  __shared__ double s_min[256];
+  __shared__ double s_min[64];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double mymin = 100000.0;
   if (i < np) {
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256)
   }
   s_min[threadIdx.x] = mymin;
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
+  for (int off = 32; off > 0; off >>= 1) {
     if (threadIdx.x < off)
       s_min[threadIdx.x] = fmin(s_min[threadIdx.x], s_min[threadIdx.x + off]);
     __syncthreads();
@@ -292,7 +292,7 @@ int ColbarImpl::apply(const double *Y, int nt, double *F, double *min_dist2,
   hipLaunchKernelGGL(k_cb_init_min, dim3(1), dim3(1), 0, st,
                      minb.as<unsigned long long>(), ib);
   if (np > 0)
-    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div(np, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div(np, 64)), dim3(64), 0, st,
                        np, Y, g, consts(nt), cell.as<int>(), start.as<int>(),
                        ids.as<int>(), sfil.as<int>(), sseg.as<int>(), F,
                        minb.as<unsigned long long>(), 0, (int *)nullptr, 0,
@@ -333,7 +333,7 @@ int ColbarImpl::pairs(const double *Y, int nt, int max_pairs, int *out,
   int *dn = d.as<int>(), *dp = dn + 1;
   hipLaunchKernelGGL(k_cb_fill, dim3(1), dim3(1), 0, 0, dn, 1, 0);
   if (np > 0)
-    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div(np, 256)), dim3(256), 0, 0, np,
+    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div(np, 64)), dim3(64), 0, 0, np,
                        Y, g, consts(nt), cell.as<int>(), start.as<int>(),
                        ids.as<int>(), sfil.as<int>(), sseg.as<int>(),
                        (double *)nullptr, (unsigned long long *)nullptr, 1, dn,

@@ -50,39 +50,41 @@ struct GaussConst {
 };
 
 template <typename T>
+__global__ void k_prepare_tab(int np, const double *__restrict__ Y,
+                              GaussConst g, T *__restrict__ tab) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)np * 3 * g.ngd) return;
+  const int i = (int)(t % g.ngd), d = (int)((t / g.ngd) % 3),
+            s = (int)(t / (3 * g.ngd));
+  const double y = Y[3 * s + d];
+  const int c = (int)(y / g.dx);  // truncation, as FCMfunctions.cpp:565-567
+  const int xg = c - g.ngdh + 1 + i;
+  const double xx = ((double)xg) * g.dx - y;
+  T *ts = tab + (size_t)s * 9 * g.ngd;
+  ts[(0 + d) * g.ngd + i] = (T)(g.anorm * exp(-xx * xx / g.anorm2));
+  ts[(3 + d) * g.ngd + i] = (T)(g.anormdip * exp(-xx * xx / g.anormdip2));
+  ts[(6 + d) * g.ngd + i] = (T)(-xx / g.sigmadipsq);
+}
+
+template <typename T>
 __global__ void k_prepare(int np, const double *__restrict__ Y,
                           const double *__restrict__ F,
                           const double *__restrict__ Tq, GaussConst g,
                           int tx, int ty, int tz, int nbx, int nby,
-                          T *__restrict__ tab, int *__restrict__ origin,
-                          T *__restrict__ coef, int *__restrict__ bin,
-                          int *__restrict__ bin_count, int *__restrict__ slot) {
+                          int *__restrict__ origin, T *__restrict__ coef,
+                          int *__restrict__ bin, int *__restrict__ bin_count,
+                          int *__restrict__ slot) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= np) return;
   const int nn[3] = {g.nx, g.ny, g.nz};
   int o[3];
-  T *ts = tab + (size_t)s * 9 * g.ngd;
   for (int d = 0; d < 3; ++d) {
-    double y = Y[3 * s + d];
-    int c = (int)(y / g.dx);  // truncation, as FCMfunctions.cpp:565-567
-    int x0 = c - g.ngdh + 1;
-    int w = x0 % nn[d];
+    int c = (int)(Y[3 * s + d] / g.dx);
+    int w = (c - g.ngdh + 1) % nn[d];
     if (w < 0) w += nn[d];
     o[d] = w;
-    for (int i = 0; i < g.ngd; ++i) {
-      int xg = x0 + i;
-      double xx = ((double)xg) * g.dx - y;
-      double e1 = g.anorm * exp(-xx * xx / g.anorm2);
-      double e2 = g.anormdip * exp(-xx * xx / g.anormdip2);
-      double gr = -xx / g.sigmadipsq;
-      ts[(0 + d) * g.ngd + i] = (T)e1;
-      ts[(3 + d) * g.ngd + i] = (T)e2;
-      ts[(6 + d) * g.ngd + i] = (T)gr;
-    }
+    origin[3 * s + d] = w;
   }
-  origin[3 * s + 0] = o[0];
-  origin[3 * s + 1] = o[1];
-  origin[3 * s + 2] = o[2];
   // spread coefficients: F and half torque (GA_setup: GA = T/2 pattern)
   for (int d = 0; d < 3; ++d) {
     coef[6 * s + d] = (T)F[3 * s + d];
@@ -141,7 +143,10 @@ __global__ void __launch_bounds__(256)
   __shared__ int s_bstart[kMaxBinsPerTile + 1];
   __shared__ int s_bfirst[kMaxBinsPerTile];
   __shared__ int s_ntot;
-  __shared__ T s_tab[kCandChunk][9 * kMaxNgd];
+  // candidate tables during the sum, the output tile during the store
+  constexpr int kTabSz = kCandChunk * 9 * kMaxNgd;
+  constexpr int kBufSz = kTabSz > 256 * TZ ? kTabSz : 256 * TZ;
+  __shared__ T s_buf[kBufSz];
   __shared__ int s_org[kCandChunk][2];
   __shared__ int s_z[kCandChunk][3];  // zlo, zhi, stencil z index of z0
   __shared__ T s_coef[kCandChunk][6];
@@ -191,7 +196,7 @@ __global__ void __launch_bounds__(256)
       int t = 0;
       while (gc >= s_bstart[t + 1]) ++t;
       int s = ids[s_bfirst[t] + gc - s_bstart[t]];
-      s_tab[c][r] = tab[(size_t)s * 9 * ngd + r];
+      s_buf[c * (9 * kMaxNgd) + r] = tab[(size_t)s * 9 * ngd + r];
       if (r < 2) s_org[c][r] = origin[3 * s + r];
       if (r < 6) s_coef[c][r] = coef[6 * s + r];
       if (r == 0) {
@@ -224,7 +229,7 @@ __global__ void __launch_bounds__(256)
         if (iy < 0) iy += G.ny;
         if (ix >= ngd || iy >= ngd) continue;
         const int zlo = s_z[c][0], zhi = s_z[c][1], izb = s_z[c][2];
-        const T *tb = s_tab[c];
+        const T *tb = s_buf + c * (9 * kMaxNgd);
         const T gxy = tb[0 * ngd + ix] * tb[1 * ngd + iy];
         const T dxy = tb[3 * ngd + ix] * tb[4 * ngd + iy];
         const T xx = tb[6 * ngd + ix], yy = tb[7 * ngd + iy];
@@ -247,15 +252,24 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
   }
-  if (active) {
-    const size_t N = (size_t)G.nx * G.ny * G.nz;
-    const size_t o = ((size_t)px * G.ny + py) * G.nz + pz0;
+  // stage each component's tile in LDS (the table buffer is free now) and
+  // write it with consecutive threads on consecutive z: runs of TZ values
+  // instead of one address per lane per store
+  const size_t N = (size_t)G.nx * G.ny * G.nz;
+  T *stage = s_buf;
+  const int nrow = G.tx * G.ty;
+  for (int c = 0; c < 3; ++c) {
+    if (active) {
 #pragma unroll
-    for (int k = 0; k < TZ; ++k) {
-      R[o + k] = acc[k][0];
-      R[N + o + k] = acc[k][1];
-      R[2 * N + o + k] = acc[k][2];
+      for (int k = 0; k < TZ; ++k) stage[threadIdx.x * TZ + k] = acc[k][c];
     }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nrow * TZ; e += blockDim.x) {
+      const int row = e / TZ, k = e - row * TZ;
+      const int rx = bx * G.tx + row / G.ty, ry = by * G.ty + row % G.ty;
+      R[(size_t)c * N + ((size_t)rx * G.ny + ry) * G.nz + pz0 + k] = stage[e];
+    }
+    __syncthreads();
   }
 }
 
@@ -448,56 +462,79 @@ __global__ void __launch_bounds__(256)
              const int *__restrict__ origin, const int *__restrict__ ids,
              const T *__restrict__ R, double *__restrict__ V,
              double *__restrict__ W) {
-  __shared__ T s_tab[4][9 * NGD];
+  constexpr int NC = NGD * NGD;          // (x, y) columns of a stencil
+  constexpr int NP = NC * NGD;           // stencil points
+  constexpr int PER = (NP + 63) / 64;    // points per lane
+  __shared__ T s_gxy[4][NC], s_dxy[4][NC];
+  __shared__ T s_z[4][3 * NGD];          // g_z, gD_z, grad_z
+  __shared__ T s_xx[4][NGD], s_yy[4][NGD];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int w = blockIdx.x * 4 + wave;
-  const bool valid = w < np;
+  // XCD-aware order: the 8 XCDs take contiguous runs of the bin-sorted
+  // segment list, so neighbouring stencils share an L2
+  const int nblk = (np + 3) / 4;
+  const int q8 = (nblk + 7) / 8;
+  const int b = (blockIdx.x & 7) * q8 + (blockIdx.x >> 3);
+  const int w = b * 4 + wave;
+  const bool valid = (b < nblk) && (w < np);
   const int s = valid ? ids[w] : 0;
-  if (valid)
-    for (int r = lane; r < 9 * NGD; r += 64)
-      s_tab[wave][r] = tab[(size_t)s * 9 * NGD + r];
+  const T *ts = tab + (size_t)s * 9 * NGD;
+  if (valid) {
+    for (int c = lane; c < NC; c += 64) {
+      int i = c / NGD, j = c - (c / NGD) * NGD;
+      s_gxy[wave][c] = ts[0 * NGD + i] * ts[1 * NGD + j];
+      s_dxy[wave][c] = ts[3 * NGD + i] * ts[4 * NGD + j];
+    }
+    for (int k = lane; k < NGD; k += 64) {
+      s_z[wave][k] = ts[2 * NGD + k];
+      s_z[wave][NGD + k] = ts[5 * NGD + k];
+      s_z[wave][2 * NGD + k] = ts[8 * NGD + k];
+      s_xx[wave][k] = ts[6 * NGD + k];
+      s_yy[wave][k] = ts[7 * NGD + k];
+    }
+  }
   const int ox = valid ? origin[3 * s] : 0, oy = valid ? origin[3 * s + 1] : 0,
             oz = valid ? origin[3 * s + 2] : 0;
   __syncthreads();
-  const T *tb = s_tab[wave];
   const size_t N = (size_t)G.nx * G.ny * G.nz;
-  // wrapped z indices of the stencil, shared by every column
-  int kz[NGD];
-#pragma unroll
-  for (int k = 0; k < NGD; ++k) {
-    int z = oz + k;
-    kz[k] = (z >= G.nz) ? z - G.nz : z;
-  }
+  // chunks of CH points per lane: all CH loads of a chunk are issued before
+  // they are used (3 CH independent loads in flight), consecutive lanes on
+  // consecutive z
+  constexpr int CH = PER < 9 ? PER : 9;
   T v0 = 0, v1 = 0, v2 = 0, w0 = 0, w1 = 0, w2 = 0;
-  const int ncol = valid ? NGD * NGD : 0;
-  for (int q = lane; q < ncol; q += 64) {
-    const int i = q / NGD, j = q - (q / NGD) * NGD;
-    int ii = ox + i;
-    if (ii >= G.nx) ii -= G.nx;
-    int jj = oy + j;
-    if (jj >= G.ny) jj -= G.ny;
-    const size_t rb = ((size_t)ii * G.ny + jj) * G.nz;
-    T ux[NGD], uy[NGD], uz[NGD];
+  for (int t0 = 0; t0 < PER; t0 += CH) {
+    T ux[CH], uy[CH], uz[CH];
 #pragma unroll
-    for (int k = 0; k < NGD; ++k) {
-      ux[k] = R[rb + kz[k]];
-      uy[k] = R[N + rb + kz[k]];
-      uz[k] = R[2 * N + rb + kz[k]];
+    for (int t = 0; t < CH; ++t) {
+      const int p = lane + 64 * (t0 + t);
+      const bool ok = valid && p < NP;
+      const int c = ok ? p / NGD : 0, k = ok ? p - (p / NGD) * NGD : 0;
+      const int i = c / NGD, j = c - (c / NGD) * NGD;
+      int ii = ox + i;
+      if (ii >= G.nx) ii -= G.nx;
+      int jj = oy + j;
+      if (jj >= G.ny) jj -= G.ny;
+      int z = oz + k;
+      if (z >= G.nz) z -= G.nz;
+      const size_t o = ((size_t)ii * G.ny + jj) * G.nz + z;
+      ux[t] = ok ? R[o] : (T)0;
+      uy[t] = ok ? R[N + o] : (T)0;
+      uz[t] = ok ? R[2 * N + o] : (T)0;
     }
-    const T gxy = tb[0 * NGD + i] * tb[1 * NGD + j];
-    const T dxy = tb[3 * NGD + i] * tb[4 * NGD + j];
-    const T xx = tb[6 * NGD + i], yy = tb[7 * NGD + j];
 #pragma unroll
-    for (int k = 0; k < NGD; ++k) {
-      const T temp = gxy * tb[2 * NGD + k];
-      const T temp2 = dxy * tb[5 * NGD + k];
-      const T zz = tb[8 * NGD + k];
-      v0 += ux[k] * temp;
-      v1 += uy[k] * temp;
-      v2 += uz[k] * temp;
-      w0 += (uz[k] * yy - uy[k] * zz) * temp2;
-      w1 += (ux[k] * zz - uz[k] * xx) * temp2;
-      w2 += (uy[k] * xx - ux[k] * yy) * temp2;
+    for (int t = 0; t < CH; ++t) {
+      const int p = lane + 64 * (t0 + t);
+      const bool ok = valid && p < NP;
+      const int c = ok ? p / NGD : 0, k = ok ? p - (p / NGD) * NGD : 0;
+      const int i = c / NGD, j = c - (c / NGD) * NGD;
+      const T temp = s_gxy[wave][c] * s_z[wave][k];
+      const T temp2 = s_dxy[wave][c] * s_z[wave][NGD + k];
+      const T xx = s_xx[wave][i], yy = s_yy[wave][j], zz = s_z[wave][2 * NGD + k];
+      v0 += ux[t] * temp;
+      v1 += uy[t] * temp;
+      v2 += uz[t] * temp;
+      w0 += (uz[t] * yy - uy[t] * zz) * temp2;
+      w1 += (ux[t] * zz - uz[t] * xx) * temp2;
+      w2 += (uy[t] * xx - ux[t] * yy) * temp2;
     }
   }
   v0 = wave_sum(v0);
@@ -796,16 +833,18 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   // bin counting sort
   hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbins, 256)), dim3(256), 0, st,
                      bin_count.as<int>(), nbins, 0);
-  hipLaunchKernelGGL(k_prepare<T>, dim3(ceil_div(np, 128)), dim3(128), 0, st,
-                     np, Y, F, Tq, g, geom.tx, geom.ty, geom.tz, geom.nbx,
-                     geom.nby, tab.as<T>(), origin.as<int>(), coef.as<T>(),
-                     bin.as<int>(), bin_count.as<int>(), slot.as<int>());
+  hipLaunchKernelGGL(k_prepare_tab<T>, dim3(ceil_div((long)np * 3 * p.ngd, 256)),
+                     dim3(256), 0, st, np, Y, g, tab.as<T>());
+  hipLaunchKernelGGL(k_prepare<T>, dim3(ceil_div(np, 64)), dim3(64), 0, st, np,
+                     Y, F, Tq, g, geom.tx, geom.ty, geom.tz, geom.nbx, geom.nby,
+                     origin.as<int>(), coef.as<T>(), bin.as<int>(),
+                     bin_count.as<int>(), slot.as<int>());
   FS_TRY(device_scan(bin_count.as<int>(), nbins, bin_start.as<int>(),
                      scan_tmp.as<int>(), st));
   hipLaunchKernelGGL(k_scatter, dim3(ceil_div(np, 256)), dim3(256), 0, st, np,
                      bin.as<int>(), slot.as<int>(), bin_start.as<int>(),
                      ids.as<int>());
-  hipLaunchKernelGGL(k_sort_bins, dim3(ceil_div(nbins, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_sort_bins, dim3(ceil_div(nbins, 64)), dim3(64), 0, st,
                      nbins, bin_start.as<int>(), ids.as<int>());
   mark(1);
   const int ntiles = nbins;
@@ -874,7 +913,8 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   double dx3 = p.dx * p.dx * p.dx;
   bool okn = with_NGD(p.ngd, [&](auto g) {
     constexpr int NG = decltype(g)::value;
-    hipLaunchKernelGGL((k_interp<T, NG>), dim3(ceil_div(np, 4)), dim3(256), 0,
+    hipLaunchKernelGGL((k_interp<T, NG>), dim3(8 * ceil_div(ceil_div(np, 4), 8)),
+                       dim3(256), 0,
                        st, np, geom, dx3, tab.as<T>(), origin.as<int>(),
                        ids.as<int>(), R.as<T>(), V, W);
   });

@@ -991,6 +991,38 @@ __global__ void k_broyden_cols(const double *__restrict__ tmp,
   Dcol[r] = dE[r] / nrm;
 }
 
+// dE = Enew - E and the per-block partial sums of dE.dE and dE.Enew
+// (fixed partition: deterministic).
+__global__ void __launch_bounds__(256)
+    k_diff_dots(const double *__restrict__ En, const double *__restrict__ Eo,
+                double *__restrict__ dE, size_t n, int chunk,
+                double *__restrict__ partial) {
+  __shared__ double r1[256], r2[256];
+  const size_t lo = (size_t)blockIdx.x * chunk;
+  const size_t hi = min(n, lo + chunk);
+  double a = 0, b = 0;
+  for (size_t r = lo + threadIdx.x; r < hi; r += 256) {
+    double e = En[r], d = e - Eo[r];
+    dE[r] = d;
+    a += d * d;
+    b += d * e;
+  }
+  r1[threadIdx.x] = a;
+  r2[threadIdx.x] = b;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      r1[threadIdx.x] += r1[threadIdx.x + off];
+      r2[threadIdx.x] += r2[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = r1[0];
+    partial[2 * blockIdx.x + 1] = r2[0];
+  }
+}
+
 // Fused bad-Broyden step. With g = H_k E_new (history columns < k), the
 // reference's second solve H_k dE equals g - Update, so its column
 // C_k = (-H_k dE - Update)/|dE| = -g/|dE|, D_k = dE/|dE|, and the next
@@ -1156,7 +1188,8 @@ int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
   FS_TRY(Dm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
   dot_chunk = 16384;
   dot_blocks = ceil_div((long)n6, dot_chunk);
-  FS_TRY(partial.alloc(sizeof(double) * (size_t)dot_blocks * p.broyden_maxiter));
+  FS_TRY(partial.alloc(sizeof(double) * (size_t)dot_blocks *
+                       std::max(2, p.broyden_maxiter)));
   FS_TRY(coefb.alloc(sizeof(double) * (p.broyden_maxiter + 2)));
   FS_TRY(flags.alloc(64));
   FS_HIP(hipHostMalloc((void **)&hflags, 64, hipHostMallocDefault));
@@ -1388,17 +1421,11 @@ int SystemImpl::broyden_loop(int nt, int *iters) {
     ++iter;
     if (!check || iter >= p.broyden_maxiter) break;
     // dE = NewE - E; |dE|^2 and dE.NewE; g = H_{iter-1} NewE
-    hipLaunchKernelGGL(k_sub, dim3(ceil_div((long)n6, 256)), dim3(256), 0, st,
-                       NewE, E, dE, n6);
     double *dots = coefb.as<double>() + p.broyden_maxiter;  // 2 scalars
-    hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, 1), dim3(256), 0, st,
-                       dE, n6, 1, dE, n6, dot_chunk, partial.as<double>());
+    hipLaunchKernelGGL(k_diff_dots, dim3(dot_blocks), dim3(256), 0, st, NewE,
+                       E, dE, n6, dot_chunk, partial.as<double>());
     hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st,
-                       partial.as<double>(), dot_blocks, 1, dots);
-    hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, 1), dim3(256), 0, st,
-                       dE, n6, 1, NewE, n6, dot_chunk, partial.as<double>());
-    hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st,
-                       partial.as<double>(), dot_blocks, 1, dots + 1);
+                       partial.as<double>(), dot_blocks, 2, dots);
     FS_LAUNCHED();
     FS_TRY(jsolve(NewE, tmp, iter - 1));
     hipLaunchKernelGGL(k_broyden_fused, dim3(ceil_div((long)n6, 256)),
