@@ -138,10 +138,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mob-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--record", type=int, default=50,
+    ap.add_argument("--record", type=int, default=100,
                     help="plain steps recorded to train the warm start")
-    ap.add_argument("--epochs", type=int, default=20)
-    ap.add_argument("--only", choices=["all", "mixed", "fp64", "mobility"],
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--skip", type=int, default=10,
+                    help="start-up steps run before recording")
+    ap.add_argument("--only", choices=["all", "mixed", "fp64", "mobility",
+                                       "copy"],
                     default="all")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -156,6 +159,17 @@ def main():
         sys.exit(2)
     from filsim_amd import abi
     out = {}
+    if args.only == "copy":
+        # calibration for HBM byte counters: a plain 1 GiB FP32 scale-by-one
+        # copy (reads and writes exactly 2^30 bytes per dispatch)
+        src = torch.rand(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 1 << 30,
+                          "copy_bytes_written": 1 << 30}), flush=True)
+        return
 
     # FCM mobility microbenchmark (256^3, 30,000 segments)
     t64, st64, fp = mobility_bench(abi.FS_FP64, args.mob_reps, torch)
@@ -186,7 +200,8 @@ def main():
         # 1) the mixed-precision solver generates the training trajectory
         s_l, _ = build_system(abi.FS_FP32)
         t0 = time.perf_counter()
-        X, T, its_rec = warmstart.record(s_l, args.record, nt0=1, torch=torch)
+        X, T, its_rec = warmstart.record(s_l, args.record, nt0=1, torch=torch,
+                                         skip=args.skip)
         t_rec = time.perf_counter() - t0
         log("recorded %d samples in %.1f s (iters %s)" %
             (X.shape[0], t_rec, its_rec[-5:]))
@@ -195,15 +210,17 @@ def main():
                                     batch=1024, lr=1e-3, log=log)
         t_train = time.perf_counter() - t0
         del X, T
-        nt0 = args.record + 1
+        nt0 = args.skip + args.record + 1
         state = s_l.get_state()
         keep = {k: state[k] for k in ("X", "Xt", "U", "Ut", "q", "qt", "lam",
                                       "lam1", "lam2")}
         keep.update(Xs=state["X"], Us=state["U"], qs=state["q"])
         api.system_set_warmstart(s_l, mlp)
-        info["train"] = {"samples": args.record * 1000, "record_s": t_rec,
+        info["train"] = {"samples": args.record * 1000, "skip": args.skip,
+                         "record_s": t_rec,
                          "train_s": t_train, "epochs": args.epochs,
-                         "holdout_loss": [h[2] for h in hist]}
+                         "holdout_loss_every_10_epochs":
+                             [h[2] for h in hist[::10]] + [hist[-1][2]]}
         # 2) the same state, three paths, same TOL
         finals = {}
         for name, prec in (("learned", None), ("mixed", abi.FS_FP32),

@@ -91,9 +91,9 @@ class Fcm:
         abi.check(self.lib.fs_fcm_get_tuning(self.h, C.byref(t)))
         return t
 
-    def set_tuning(self, fft_batch=-1, spread_tile=-1, interp_wave=-1,
+    def set_tuning(self, fft_batch=-1, spread_tile=-1, spread_zrun=-1,
                    fft_xlines=-1):
-        t = abi.FcmTuning(fft_batch, spread_tile, interp_wave, fft_xlines)
+        t = abi.FcmTuning(fft_batch, spread_tile, spread_zrun, fft_xlines)
         abi.check(self.lib.fs_fcm_set_tuning(self.h, C.byref(t)))
 
 

@@ -150,6 +150,7 @@ __global__ void __launch_bounds__(256)
   __shared__ int s_org[kCandChunk][2];
   __shared__ int s_z[kCandChunk][3];  // zlo, zhi, stencil z index of z0
   __shared__ T s_coef[kCandChunk][6];
+  __shared__ int s_cid[kCandChunk];
 
   const int ntiles = G.nbx * G.nby * G.nbz;
   // XCD-aware order: consecutive tiles land on the same XCD's L2.
@@ -190,34 +191,46 @@ __global__ void __launch_bounds__(256)
 
   for (int c0 = 0; c0 < ntot; c0 += kCandChunk) {
     const int nc = min(kCandChunk, ntot - c0);
-    for (int e = threadIdx.x; e < nc * 9 * ngd; e += blockDim.x) {
-      int c = e / (9 * ngd), r = e - c * (9 * ngd);
-      int gc = c0 + c;
+    // candidate ids, origins, z overlap and coefficients: one thread each
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const int gc = c0 + c;
       int t = 0;
       while (gc >= s_bstart[t + 1]) ++t;
-      int s = ids[s_bfirst[t] + gc - s_bstart[t]];
-      s_buf[c * (9 * kMaxNgd) + r] = tab[(size_t)s * 9 * ngd + r];
-      if (r < 2) s_org[c][r] = origin[3 * s + r];
-      if (r < 6) s_coef[c][r] = coef[6 * s + r];
-      if (r == 0) {
-        // z overlap of the stencil [oz, oz+ngd) with [pz0, pz0+TZ) (mod nz;
-        // nz >= TZ + ngd so at most one interval)
-        int oz = origin[3 * s + 2];
-        int o = pz0 - oz;
-        if (o < 0) o += G.nz;
-        int zlo = 0, zhi = 0, izb = 0;
-        if (o < ngd) {
-          zlo = 0;
-          zhi = min(TZ, ngd - o);
-          izb = o;
-        } else if (o + TZ > G.nz) {
-          zlo = G.nz - o;
-          zhi = min(TZ, G.nz - o + ngd);
-          izb = o - G.nz;
-        }
-        s_z[c][0] = zlo;
-        s_z[c][1] = zhi;
-        s_z[c][2] = izb;
+      const int s = ids[s_bfirst[t] + gc - s_bstart[t]];
+      s_cid[c] = s;
+      s_org[c][0] = origin[3 * s + 0];
+      s_org[c][1] = origin[3 * s + 1];
+      for (int r = 0; r < 6; ++r) s_coef[c][r] = coef[6 * s + r];
+      // z overlap of the stencil [oz, oz+ngd) with [pz0, pz0+TZ) (mod nz;
+      // nz >= TZ + ngd so at most one interval)
+      const int oz = origin[3 * s + 2];
+      int o = pz0 - oz;
+      if (o < 0) o += G.nz;
+      int zlo = 0, zhi = 0, izb = 0;
+      if (o < ngd) {
+        zhi = min(TZ, ngd - o);
+        izb = o;
+      } else if (o + TZ > G.nz) {
+        zlo = G.nz - o;
+        zhi = min(TZ, G.nz - o + ngd);
+        izb = o - G.nz;
+      }
+      s_z[c][0] = zlo;
+      s_z[c][1] = zhi;
+      s_z[c][2] = izb;
+    }
+    __syncthreads();
+    // tables: flat copy with independent loads in flight. e / tl via a
+    // float reciprocal is exact here: (e + 0.5) / tl stays >= 0.5 / tl from
+    // an integer and e < 2^13.
+    {
+      const int tl = 9 * ngd, ne = nc * tl;
+      const float inv_tl = 1.0f / (float)tl;
+#pragma unroll 4
+      for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const int c = (int)(((float)e + 0.5f) * inv_tl);
+        const int r = e - c * tl;
+        s_buf[c * (9 * kMaxNgd) + r] = tab[(size_t)s_cid[c] * tl + r];
       }
     }
     __syncthreads();
@@ -707,7 +720,7 @@ int FcmImpl::init(const fs_fcm_params &pp, int maxnp, int prec) {
   FS_TRY(ids.alloc((size_t)maxnp * sizeof(int)));
   tune.fft_batch = (prec == FS_FP64) ? 8 : 16;
   tune.spread_tile = 16;
-  tune.interp_wave = 1;
+  tune.spread_zrun = 8;
   tune.fft_xlines = 2;
   FS_TRY(set_geometry());
   for (int i = 0; i < 8; ++i) FS_HIP(hipEventCreate(&ev[i]));
@@ -728,13 +741,14 @@ int FcmImpl::set_geometry() {
   int t = tune.spread_tile;
   FS_CHECK(t == 4 || t == 8 || t == 16, FS_ERR_ARG,
            "spread_tile must be 4, 8 or 16");
-  // z run per thread: the largest instantiated length dividing nz with
-  // nz >= run + ngd (one overlap interval per stencil)
+  FS_CHECK(tune.spread_zrun >= 4 && tune.spread_zrun <= 16, FS_ERR_ARG,
+           "spread_zrun must be in [4, 16]");
+  // tile z extent: the largest instantiated length <= spread_zrun dividing
+  // nz with nz >= run + ngd
   static const int runs[] = {16, 12, 10, 8, 6, 5, 4};
   int tz = 0;
   for (int r : runs)
-    if (r <= (precision == FS_FP64 ? 8 : 16) && p.nz % r == 0 &&
-        p.nz >= r + p.ngd) {
+    if (r <= tune.spread_zrun && p.nz % r == 0 && p.nz >= r + p.ngd) {
       tz = r;
       break;
     }
@@ -1063,7 +1077,7 @@ extern "C" int fs_fcm_set_tuning(fs_fcm *h, const fs_fcm_tuning *t) {
   fs_fcm_tuning nt = old;
   if (t->fft_batch > 0) nt.fft_batch = t->fft_batch;
   if (t->spread_tile > 0) nt.spread_tile = t->spread_tile;
-  if (t->interp_wave >= 0) nt.interp_wave = t->interp_wave;
+  if (t->spread_zrun > 0) nt.spread_zrun = t->spread_zrun;
   if (t->fft_xlines > 0) nt.fft_xlines = t->fft_xlines;
   FS_CHECK(nt.fft_batch >= 1 && nt.fft_batch <= 64 && nt.fft_xlines >= 1 &&
                nt.fft_xlines <= 16,

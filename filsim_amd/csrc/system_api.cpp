@@ -140,8 +140,8 @@ extern "C" int fs_system_warmstart_dims(fs_system *h, int *n_in, int *n_out) {
   for (int f = 1; f < h->impl.nf; ++f)
     FS_CHECK(h->impl.hnw[f] == h->impl.hnw[0], FS_ERR_ARG,
              "the warm start needs one segment count for every filament");
-  if (n_in) *n_in = 15 * h->impl.hnw[0] - 4;
-  if (n_out) *n_out = 6 * h->impl.hnw[0];
+  if (n_in) *n_in = fs::ws_inputs(h->impl.hnw[0]);
+  if (n_out) *n_out = fs::ws_outputs(h->impl.hnw[0]);
   return FS_OK;
 }
 

@@ -3,10 +3,11 @@
 // warmstart.hpp).
 //
 // Per filament, everything is expressed in the frame of bead 0's current
-// orientation q_0 (columns tangent, normal, binormal): features are the Nw
-// tangents, the converged velocities of the last two steps, the tensions of
-// the last two steps (lam1, lam2) and cos/sin of the driving phase at the
-// new time; the target is (converged - extrapolated) for the unknowns
+// orientation q_0 (columns tangent, normal, binormal). Features (24 Nw - 4):
+// per segment the tangent, the last converged velocity V1 and V1 - V2, the
+// last converged angular velocity W1 and W1 - W2, and U - Ut (the change of
+// the rotation increment over the last step); per link lam1 and
+// lam1 - lam2; cos/sin of the driving phase at the new time. The target is (converged - extrapolated) for the unknowns
 // [X_0, U_0..U_{Nw-1}, lam_0..lam_{Nw-2}], the Broyden state layout of
 // Filament::update (Filament.hpp:425-452).
 #include <hip/hip_runtime.h>
@@ -57,24 +58,42 @@ __device__ __forceinline__ Frame3 bead0_frame(const SysDev &S, int f) {
   return F;
 }
 
+// Differences of nearly equal histories (V1 - V2, W1 - W2, U - Ut,
+// lam1 - lam2) are formed in FP64 before the FP32 store: after
+// normalisation and BF16 rounding the network could not recover them from
+// the two terms.
+__device__ __forceinline__ void local_diff(const Frame3 &F, const double *a,
+                                           const double *b, float *o) {
+  const double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  F.to_local(d, o);
+}
+
 __global__ void k_ws_features(SysDev S, int nt, int nw,
                               const double *__restrict__ V1,
-                              const double *__restrict__ V2, int nin,
+                              const double *__restrict__ V2,
+                              const double *__restrict__ W1,
+                              const double *__restrict__ W2, int nin,
                               float *__restrict__ X) {
   const int f = blockIdx.x;
   const int b0 = S.boff[f], l0 = S.loff[f];
   const Frame3 F = bead0_frame(S, f);
   float *x = X + (size_t)f * nin;
   for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+    const int g = b0 + i;
     double t[3], n[3], b[3];
-    frame_of(S.q + 4 * (b0 + i), t, n, b);
+    frame_of(S.q + 4 * g, t, n, b);
     F.to_local(t, x + 3 * i);
-    F.to_local(V1 + 3 * (b0 + i), x + 3 * nw + 3 * i);
-    F.to_local(V2 + 3 * (b0 + i), x + 6 * nw + 3 * i);
+    F.to_local(V1 + 3 * g, x + 3 * nw + 3 * i);
+    local_diff(F, V1 + 3 * g, V2 + 3 * g, x + 6 * nw + 3 * i);
+    F.to_local(W1 + 3 * g, x + 9 * nw + 3 * i);
+    local_diff(F, W1 + 3 * g, W2 + 3 * g, x + 12 * nw + 3 * i);
+    local_diff(F, S.U + 3 * g, S.Ut + 3 * g, x + 15 * nw + 3 * i);
   }
   for (int l = threadIdx.x; l < nw - 1; l += blockDim.x) {
-    F.to_local(S.lam1 + 3 * (l0 + l), x + 9 * nw + 3 * l);
-    F.to_local(S.lam2 + 3 * (l0 + l), x + 9 * nw + 3 * (nw - 1) + 3 * l);
+    const int g = l0 + l;
+    F.to_local(S.lam1 + 3 * g, x + 18 * nw + 3 * l);
+    local_diff(F, S.lam1 + 3 * g, S.lam2 + 3 * g,
+               x + 18 * nw + 3 * (nw - 1) + 3 * l);
   }
   if (threadIdx.x == 0) {
     const fs_fil_props pr = S.props[f];
@@ -127,14 +146,18 @@ int WarmStart::setup(SystemImpl &S) {
     FS_CHECK(S.hnw[f] == S.hnw[0], FS_ERR_ARG,
              "the warm start needs one segment count for every filament");
   nw = S.hnw[0];
-  nin = 15 * nw - 4;
-  nout = 6 * nw;
+  nin = ws_inputs(nw);
+  nout = ws_outputs(nw);
   const size_t n3 = 3 * (size_t)S.np, n4 = 4 * (size_t)S.np,
                l3 = 3 * (size_t)S.nlam;
   FS_TRY(V1.alloc(sizeof(double) * n3));
   FS_TRY(V2.alloc(sizeof(double) * n3));
+  FS_TRY(W1.alloc(sizeof(double) * n3));
+  FS_TRY(W2.alloc(sizeof(double) * n3));
   FS_HIP(hipMemset(V1.p, 0, sizeof(double) * n3));
   FS_HIP(hipMemset(V2.p, 0, sizeof(double) * n3));
+  FS_HIP(hipMemset(W1.p, 0, sizeof(double) * n3));
+  FS_HIP(hipMemset(W2.p, 0, sizeof(double) * n3));
   FS_TRY(feat.alloc(sizeof(float) * (size_t)S.nf * nin));
   FS_TRY(pred.alloc(sizeof(float) * (size_t)S.nf * nout));
   FS_TRY(u.alloc(sizeof(double) * 6 * (size_t)S.np));
@@ -183,6 +206,8 @@ int WarmStart::record(SystemImpl &S, float *X, float *T, int capacity) {
 void WarmStart::release() {
   V1.release();
   V2.release();
+  W1.release();
+  W2.release();
   feat.release();
   pred.release();
   u.release();
@@ -201,8 +226,8 @@ int WarmStart::timestep(SystemImpl &S, int nt, int *iters) {
   res_mlp = 0;
   used_mlp = 0;
   hipLaunchKernelGGL(k_ws_features, dim3(S.nf), dim3(64), 0, st, S.d, nt,
-                     nw, V1.as<double>(), V2.as<double>(), nin,
-                     feat.as<float>());
+                     nw, V1.as<double>(), V2.as<double>(), W1.as<double>(),
+                     W2.as<double>(), nin, feat.as<float>());
   FS_LAUNCHED();
   // keep the extrapolated guess: fallback state and training reference
   FS_HIP(hipMemcpyAsync(sXs.p, S.d.Xs, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
@@ -245,9 +270,11 @@ int WarmStart::timestep(SystemImpl &S, int nt, int *iters) {
     FS_LAUNCHED();
     rec_count += S.nf;
   }
-  // velocity history for the next step's features
+  // velocity and angular-velocity history for the next step's features
   FS_HIP(hipMemcpyAsync(V2.p, V1.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
   FS_HIP(hipMemcpyAsync(V1.p, S.d.V, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+  FS_HIP(hipMemcpyAsync(W2.p, W1.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+  FS_HIP(hipMemcpyAsync(W1.p, S.d.W, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
   (void)n6;
   FS_TRY(S.step(nt));
   FS_HIP(hipStreamSynchronize(st));

@@ -11,12 +11,17 @@ namespace fs {
 
 struct MlpImpl;
 
+// network sizes for Nw segments per filament (feature layout in
+// warmstart.hip)
+inline int ws_inputs(int nw) { return 24 * nw - 4; }
+inline int ws_outputs(int nw) { return 6 * nw; }
+
 struct WarmStart {
   MlpImpl *mlp = nullptr;
   bool recording = false;
-  bool tracking = false;  // velocity history kept (attached or recording)
+  bool tracking = false;  // velocity/angular-velocity history kept (attached or recording)
   int nw = 0, nin = 0, nout = 0;
-  dbuf V1, V2, feat, pred, u, sXs, sUs, sqs, slam;
+  dbuf V1, V2, W1, W2, feat, pred, u, sXs, sUs, sqs, slam;
   float *recX = nullptr, *recT = nullptr;
   int rec_cap = 0, rec_count = 0;
   int used_mlp = 0;

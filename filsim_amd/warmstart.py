@@ -11,10 +11,14 @@ import numpy as np
 from . import api
 
 
-def record(system, steps, nt0=1, torch=None):
-    """Runs `steps` plain timesteps from nt0 while recording one sample per
-    filament per step. Returns device tensors X (rows x n_in), T and the
-    Broyden iteration counts."""
+def record(system, steps, nt0=1, torch=None, skip=0):
+    """Runs `skip` plain timesteps from nt0 (the start-up transient, kept out
+    of the data), then `steps` more while recording one sample per filament
+    per step. Returns device tensors X (rows x n_in), T and the Broyden
+    iteration counts of the recorded steps."""
+    for nt in range(nt0, nt0 + skip):
+        system.timestep(nt)
+    nt0 += skip
     n_in, n_out = api.system_warmstart_dims(system)
     cap = system.nfil * steps
     X = torch.zeros((cap, n_in), dtype=torch.float32, device="cuda")

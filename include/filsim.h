@@ -78,11 +78,11 @@ int fs_fcm_friction(fs_fcm *h, int np, const double *F, const double *T,
  * fs_fcm_set_timing(h,1) before the solve. */
 int fs_fcm_set_timing(fs_fcm *h, int on);
 int fs_fcm_stage_times(fs_fcm *h, float *ms7);
-/* Tunables chosen by the autotuner (tile/batch shapes); -1 keeps current. */
+/* Launch-shape tunables (tile/batch shapes); values <= 0 keep current. */
 typedef struct fs_fcm_tuning {
   int fft_batch;    /* columns per workgroup in the strided FFT passes */
-  int spread_tile;  /* tile edge (grid points) of the gather spread   */
-  int interp_wave;  /* 1: one wavefront per segment in interpolation  */
+  int spread_tile;  /* spread tile edge in x and y (4, 8 or 16)        */
+  int spread_zrun;  /* largest spread tile extent in z (4..16)         */
   int fft_xlines;   /* x lines (per component) per workgroup, x pass  */
 } fs_fcm_tuning;
 int fs_fcm_get_tuning(fs_fcm *h, fs_fcm_tuning *t);
@@ -262,7 +262,7 @@ int fs_mlp_get_params(fs_mlp *h, float *p, int n);
 int fs_mlp_set_params(fs_mlp *h, const float *p, int n);
 
 /* Attach a network to a system (NULL detaches). Needs one segment count
- * Nw for every filament; the network maps 15 Nw - 4 features to 6 Nw
+ * Nw for every filament; the network maps 24 Nw - 4 features to 6 Nw
  * outputs (fs_system_warmstart_dims). */
 int fs_system_set_warmstart(fs_system *h, fs_mlp *m);
 int fs_system_warmstart_dims(fs_system *h, int *n_in, int *n_out);

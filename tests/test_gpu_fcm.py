@@ -16,6 +16,9 @@ CASES = [
     ((30, 36, 50), 12, None),
     ((54, 24, 64), 12, None),
     ((40, 40, 40), 20, "ref"),
+    # stencils wrapping onto both ends of a spread tile
+    ((16, 16, 16), 12, None),
+    ((24, 24, 24), 20, "ref"),
 ]
 
 
@@ -79,13 +82,14 @@ def test_device_pointers_and_determinism(cuda):
     assert rel(V1.cpu().numpy(), Vo) <= 1e-12
 
 
-@pytest.mark.parametrize("batch,tile", [(1, 4), (2, 8), (8, 16), (16, 8)])
-def test_tunings_agree(cuda, batch, tile):
+@pytest.mark.parametrize("batch,tile,zrun", [(1, 4, 4), (2, 8, 8), (8, 16, 16),
+                                             (16, 8, 4), (4, 16, 8)])
+def test_tunings_agree(cuda, batch, tile, zrun):
     fp = params.fcm_params(64)
     Y, F, T = random_segments(fp, 500, seed=2)
     Vo, Wo = O.mobility(fp, Y, F, T)
     with api.Fcm(fp, 500) as fcm:
-        fcm.set_tuning(fft_batch=batch, spread_tile=tile)
+        fcm.set_tuning(fft_batch=batch, spread_tile=tile, spread_zrun=zrun)
         V, W = fcm.mobility_host(Y, F, T)
     assert rel(V, Vo) <= 1e-12 and rel(W, Wo) <= 1e-12
 
