@@ -790,7 +790,16 @@ __global__ void __launch_bounds__(1024)
 }
 
 // Register-resident variant for n <= 32*NB: 1024 threads as a 32 x 32
-// grid; thread (ty, tx) owns A[i][j] for i = ty + 32a, j = tx + 32b.
+// grid; thread (ty, tx) owns A[i][j] for i = ty + 32a, j = tx + 32b (rows
+// and columns past n are zero padding). Same pivot choice as k_gj_inverse
+// (largest |value| among unused rows, lowest row on ties). Per pivot step:
+//   1. every wave reduces column k (held in colbuf[k & 1]) to the same
+//      pivot p, so no barrier separates the search from its use; the
+//      owners of row p publish it in rowbuf; barrier;
+//   2. branch-free update A -= fct_i * r_j with r_j = A[p][j] / A[p][k];
+//      column k (base 0, r = 1/pivot) and row p (r_j) are fixed under
+//      wave-uniform conditions; the owners of column k + 1 publish it in
+//      the other colbuf; barrier.
 template <int NB>
 __device__ __forceinline__ double pick(const double (&row)[NB], int b) {
   double v = 0.0;
@@ -804,15 +813,14 @@ template <int NB>
 __global__ void __launch_bounds__(1024)
     k_gj_inverse_reg(SysDev S, const double *__restrict__ Jall,
                      double *__restrict__ JTall, int *__restrict__ singular) {
-  __shared__ double colbuf[32 * NB], rowbuf[32 * NB];
+  __shared__ double colbuf[2][32 * NB], rowbuf[32 * NB];
   __shared__ int used[32 * NB], pivrow[32 * NB], rowstep[32 * NB];
-  __shared__ int s_p;
   const int f = blockIdx.x;
   const int nw = S.nw[f];
   const int n = 6 * nw;
   const double *J = Jall + S.joff[f];
   double *JT = JTall + S.joff[f];
-  const int tid = threadIdx.x, ty = tid >> 5, tx = tid & 31;
+  const int tid = threadIdx.x, ty = tid >> 5, tx = tid & 31, lane = tid & 63;
   double A[NB][NB];
 #pragma unroll
   for (int a = 0; a < NB; ++a)
@@ -821,76 +829,79 @@ __global__ void __launch_bounds__(1024)
       int i = ty + 32 * a, j = tx + 32 * b;
       A[a][b] = (i < n && j < n) ? J[(size_t)i * n + j] : 0.0;
     }
-  for (int i = tid; i < n; i += blockDim.x) used[i] = 0;
+  for (int i = tid; i < 32 * NB; i += blockDim.x) used[i] = (i < n) ? 0 : 1;
+  if (tx == 0) {
+#pragma unroll
+    for (int a = 0; a < NB; ++a) colbuf[0][ty + 32 * a] = A[a][0];
+  }
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     const int kb = k >> 5, kt = k & 31;
-    if (tx == kt) {
-#pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        int i = ty + 32 * a;
-        if (i < n) colbuf[i] = pick<NB>(A[a], kb);
+    const double *cb = colbuf[k & 1];
+    double best = -1.0;
+    int bi = 32 * NB;
+    for (int i = lane; i < n; i += 64) {
+      const double v = used[i] ? -1.0 : fabs(cb[i]);
+      if (v > best) {
+        best = v;
+        bi = i;
       }
     }
-    __syncthreads();
-    if (tid < 64) {
-      double best = -1.0;
-      int bi = n;
-      for (int i = tid; i < n; i += 64) {
-        double v = used[i] ? -1.0 : fabs(colbuf[i]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        double ob = __shfl_xor(best, off, 64);
-        int oi = __shfl_xor(bi, off, 64);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if (tid == 0) {
-        if (!(best > 0.0)) atomicExch(singular, 1);
-        s_p = bi;
-        used[bi] = 1;
-        pivrow[k] = bi;
-        rowstep[bi] = k;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
       }
     }
-    __syncthreads();
-    const int p = s_p, pb = p >> 5, pt = p & 31;
+    const int p = __builtin_amdgcn_readfirstlane(bi);
+    const int pb = p >> 5, pt = p & 31;
     if (ty == pt) {
 #pragma unroll
       for (int a = 0; a < NB; ++a)
         if (a == pb) {
 #pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            int j = tx + 32 * b;
-            if (j < n) rowbuf[j] = A[a][b];
-          }
+          for (int b = 0; b < NB; ++b) rowbuf[tx + 32 * b] = A[a][b];
         }
+    }
+    if (tid == 0) {
+      if (!(best > 0.0)) atomicExch(singular, 1);
+      pivrow[k] = p;
+      rowstep[p] = k;
     }
     __syncthreads();
     const double inv = 1.0 / rowbuf[k];
+    double r[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) r[b] = rowbuf[tx + 32 * b] * inv;
+    const bool colk = (tx == kt);
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (b == kb) {
+        r[b] = colk ? inv : r[b];
+#pragma unroll
+        for (int a = 0; a < NB; ++a) A[a][b] = colk ? 0.0 : A[a][b];
+      }
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
-      const int i = ty + 32 * a;
-      if (i >= n) continue;
-      const double fct = colbuf[i];
+      const double fct = cb[ty + 32 * a];
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int j = tx + 32 * b;
-        if (j >= n) continue;
-        if (i == p) {
-          A[a][b] = (j == k) ? inv : rowbuf[j] * inv;
-        } else if (j == k) {
-          A[a][b] = -fct * inv;
-        } else if (fct != 0.0) {
-          A[a][b] = A[a][b] - fct * (rowbuf[j] * inv);
-        }
+      for (int b = 0; b < NB; ++b) A[a][b] = fma(-fct, r[b], A[a][b]);
+    }
+    const bool rowp = (ty == pt);
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+      if (a == pb) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) A[a][b] = rowp ? r[b] : A[a][b];
       }
+    if (tid == 0) used[p] = 1;
+    const int k1 = k + 1;
+    if (k1 < n && tx == (k1 & 31)) {
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+        colbuf[k1 & 1][ty + 32 * a] = pick<NB>(A[a], k1 >> 5);
     }
     __syncthreads();
   }
