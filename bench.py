@@ -33,6 +33,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch of k_fft_x_project<float> at 256^3, measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in separate runs, each scaled by
+# the same counter's reading for a 1 GiB copy (profiles/
+# fcm_mobility_256cubed_30k_hbm_bytes.txt): 203.1 MB read + 202.9 MB written
+X_PASS_FP32_TRAFFIC_BYTES = 406.0e6
 
 
 def log(*a):
@@ -134,7 +139,7 @@ def cpu_baseline(fp_iters):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mob-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
@@ -300,7 +305,8 @@ def main():
                      "achieved": xk_bytes / (xk_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": xk_bytes / (xk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": X_PASS_FP32_TRAFFIC_BYTES,
+                     "algorithmic_bytes": xk_bytes},
         "cpu_baseline": cpu,
         "extra": out,
     }

@@ -4,6 +4,7 @@ Record (features, target) pairs from the plain solver, normalise them, and
 train the network with the library's own forward/backward/Adam kernels.
 torch only allocates device buffers and draws minibatch indices.
 """
+import math
 import time
 
 import numpy as np
@@ -38,8 +39,10 @@ def normalisation(X, T, torch):
 
 
 def train(X, T, torch, width=256, epochs=20, batch=1024, lr=1e-3, seed=1,
-          holdout=0.1, log=None):
-    """Trains a fresh network on (X, T); returns (mlp, history)."""
+          holdout=0.1, log=None, lr_final=None):
+    """Trains a fresh network on (X, T); returns (mlp, history). With
+    lr_final the step size follows a cosine from lr down to lr_final over
+    all steps."""
     n = X.shape[0]
     rng = np.random.default_rng(seed)
     perm = rng.permutation(n)
@@ -52,13 +55,20 @@ def train(X, T, torch, width=256, epochs=20, batch=1024, lr=1e-3, seed=1,
     mlp.set_norm(mu, sigma, tau)
     hist = []
     t0 = time.time()
+    per_epoch = max(1, (len(tr) - batch) // batch + 1)
+    total, step = epochs * per_epoch, 0
     for ep in range(epochs):
         order = rng.permutation(tr)
         losses = []
         for i in range(0, len(order) - batch + 1, batch):
             idx = torch.as_tensor(order[i:i + batch], device="cuda")
+            lr_t = lr
+            if lr_final is not None:
+                lr_t = lr_final + 0.5 * (lr - lr_final) * (
+                    1.0 + math.cos(math.pi * step / total))
+            step += 1
             losses.append(mlp.train_step(X[idx].contiguous(),
-                                         T[idx].contiguous(), lr,
+                                         T[idx].contiguous(), lr_t,
                                          want_loss=(i == 0)))
         hl = None
         if nh:
