@@ -791,15 +791,17 @@ __global__ void __launch_bounds__(1024)
 
 // Register-resident variant for n <= 32*NB: 1024 threads as a 32 x 32
 // grid; thread (ty, tx) owns A[i][j] for i = ty + 32a, j = tx + 32b (rows
-// and columns past n are zero padding). Same pivot choice as k_gj_inverse
-// (largest |value| among unused rows, lowest row on ties). Per pivot step:
-//   1. every wave reduces column k (held in colbuf[k & 1]) to the same
-//      pivot p, so no barrier separates the search from its use; the
-//      owners of row p publish it in rowbuf; barrier;
+// and columns past n are zero padding). Pivot: largest |value| among unused
+// rows, compared on |value| with its 8 lowest mantissa bits dropped (ties
+// within 2^-44 go to the lowest row). Per pivot step:
+//   1. every wave reduces the 64-bit keys of column k (published with the
+//      column in colkey/colbuf[k & 1]) to the same pivot p with DPP row
+//      permutes and four lane reads, so no barrier separates the search from
+//      its use; the owners of row p publish it in rowbuf; barrier;
 //   2. branch-free update A -= fct_i * r_j with r_j = A[p][j] / A[p][k];
 //      column k (base 0, r = 1/pivot) and row p (r_j) are fixed under
-//      wave-uniform conditions; the owners of column k + 1 publish it in
-//      the other colbuf; barrier.
+//      wave-uniform conditions; the owners of column k + 1 publish it and
+//      its keys in the other buffers; barrier.
 template <int NB>
 __device__ __forceinline__ double pick(const double (&row)[NB], int b) {
   double v = 0.0;
@@ -809,11 +811,53 @@ __device__ __forceinline__ double pick(const double (&row)[NB], int b) {
   return v;
 }
 
+// pivot key of row i (< 256) holding v; 0 marks a used row
+__device__ __forceinline__ unsigned long long pivot_key(double v, int i,
+                                                        bool used) {
+  const unsigned long long m =
+      ((unsigned long long)__double_as_longlong(fabs(v)) & ~0xFFull) |
+      (unsigned long long)(255 - i);
+  return used ? 0ull : m;
+}
+
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_max_u64(
+    unsigned long long k) {
+  const int lo = (int)(unsigned)k, hi = (int)(unsigned)(k >> 32);
+  const unsigned lo2 =
+      (unsigned)__builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+  const unsigned hi2 =
+      (unsigned)__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  const unsigned long long k2 = ((unsigned long long)hi2 << 32) | lo2;
+  return k2 > k ? k2 : k;
+}
+
+// maximum over the 64 lanes, returned wave-uniform
+__device__ __forceinline__ unsigned long long wave_max_u64(
+    unsigned long long k) {
+  k = dpp_max_u64<0xB1>(k);   // quad_perm [1,0,3,2]
+  k = dpp_max_u64<0x4E>(k);   // quad_perm [2,3,0,1]
+  k = dpp_max_u64<0x141>(k);  // row_half_mirror
+  k = dpp_max_u64<0x140>(k);  // row_mirror: every row of 16 holds its max
+  const int lo = (int)(unsigned)k, hi = (int)(unsigned)(k >> 32);
+  unsigned long long best = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const unsigned long long v =
+        ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 16 * r)
+         << 32) |
+        (unsigned)__builtin_amdgcn_readlane(lo, 16 * r);
+    best = v > best ? v : best;
+  }
+  return best;
+}
+
 template <int NB>
 __global__ void __launch_bounds__(1024)
     k_gj_inverse_reg(SysDev S, const double *__restrict__ Jall,
                      double *__restrict__ JTall, int *__restrict__ singular) {
   __shared__ double colbuf[2][32 * NB], rowbuf[32 * NB];
+  __shared__ unsigned long long colkey[2][32 * NB];
   __shared__ int used[32 * NB], pivrow[32 * NB], rowstep[32 * NB];
   const int f = blockIdx.x;
   const int nw = S.nw[f];
@@ -832,30 +876,23 @@ __global__ void __launch_bounds__(1024)
   for (int i = tid; i < 32 * NB; i += blockDim.x) used[i] = (i < n) ? 0 : 1;
   if (tx == 0) {
 #pragma unroll
-    for (int a = 0; a < NB; ++a) colbuf[0][ty + 32 * a] = A[a][0];
+    for (int a = 0; a < NB; ++a) {
+      const int i = ty + 32 * a;
+      colbuf[0][i] = A[a][0];
+      colkey[0][i] = pivot_key(A[a][0], i, i >= n);
+    }
   }
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     const int kb = k >> 5, kt = k & 31;
     const double *cb = colbuf[k & 1];
-    double best = -1.0;
-    int bi = 32 * NB;
-    for (int i = lane; i < n; i += 64) {
-      const double v = used[i] ? -1.0 : fabs(cb[i]);
-      if (v > best) {
-        best = v;
-        bi = i;
-      }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_xor(best, off, 64);
-      const int oi = __shfl_xor(bi, off, 64);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
-      }
-    }
-    const int p = __builtin_amdgcn_readfirstlane(bi);
+    const unsigned long long *ck = colkey[k & 1];
+    unsigned long long key = 0;
+    for (int i = lane; i < 32 * NB; i += 64) key = ck[i] > key ? ck[i] : key;
+    key = wave_max_u64(key);
+    int p = 255 - (int)(key & 0xFF);
+    const bool bad = (key >> 8) == 0 || p >= n;  // no nonzero unused entry
+    p = bad ? n - 1 - k : p;  // keeps indices in range; result is flagged
     const int pb = p >> 5, pt = p & 31;
     if (ty == pt) {
 #pragma unroll
@@ -866,7 +903,7 @@ __global__ void __launch_bounds__(1024)
         }
     }
     if (tid == 0) {
-      if (!(best > 0.0)) atomicExch(singular, 1);
+      if (bad) atomicExch(singular, 1);
       pivrow[k] = p;
       rowstep[p] = k;
     }
@@ -900,8 +937,13 @@ __global__ void __launch_bounds__(1024)
     const int k1 = k + 1;
     if (k1 < n && tx == (k1 & 31)) {
 #pragma unroll
-      for (int a = 0; a < NB; ++a)
-        colbuf[k1 & 1][ty + 32 * a] = pick<NB>(A[a], k1 >> 5);
+      for (int a = 0; a < NB; ++a) {
+        const int i = ty + 32 * a;
+        const double v = pick<NB>(A[a], k1 >> 5);
+        colbuf[k1 & 1][i] = v;
+        // used[p] of this step may not be visible yet: test i == p too
+        colkey[k1 & 1][i] = pivot_key(v, i, i >= n || i == p || used[i]);
+      }
     }
     __syncthreads();
   }
@@ -961,13 +1003,16 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) partial[(size_t)blk * ncol + i] = red[0];
 }
 
-__global__ void k_dots_final(const double *__restrict__ partial, int nblk,
-                             int ncol, double *__restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ncol) return;
+// one 64-lane workgroup per column: strided lane sums, then a butterfly in
+// a fixed order (deterministic)
+__global__ void __launch_bounds__(64)
+    k_dots_final(const double *__restrict__ partial, int nblk, int ncol,
+                 double *__restrict__ out) {
+  const int i = blockIdx.x, lane = threadIdx.x;
   double s = 0;
-  for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * ncol + i];
-  out[i] = s;
+  for (int b = lane; b < nblk; b += 64) s += partial[(size_t)b * ncol + i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[i] = s;
 }
 
 __global__ void k_hist_axpy(const double *__restrict__ C, size_t lda, int ncol,
@@ -1197,7 +1242,7 @@ int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
   if (6 * max_nw > 32 * 6) FS_TRY(Mb.alloc(sizeof(double) * (size_t)hjoff[nf]));
   FS_TRY(Cm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
   FS_TRY(Dm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
-  dot_chunk = 16384;
+  dot_chunk = 2048;
   dot_blocks = ceil_div((long)n6, dot_chunk);
   FS_TRY(partial.alloc(sizeof(double) * (size_t)dot_blocks *
                        std::max(2, p.broyden_maxiter)));
@@ -1325,7 +1370,7 @@ int SystemImpl::jsolve(const double *b, double *x, int iter) {
     hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, iter), dim3(256), 0, st,
                        Dm.as<double>(), n6, iter, b, n6, dot_chunk,
                        partial.as<double>());
-    hipLaunchKernelGGL(k_dots_final, dim3(ceil_div(iter, 64)), dim3(64), 0, st,
+    hipLaunchKernelGGL(k_dots_final, dim3(iter), dim3(64), 0, st,
                        partial.as<double>(), dot_blocks, iter,
                        coefb.as<double>());
     hipLaunchKernelGGL(k_hist_axpy, dim3(ceil_div((long)n6, 256)), dim3(256), 0,
@@ -1435,7 +1480,7 @@ int SystemImpl::broyden_loop(int nt, int *iters) {
     double *dots = coefb.as<double>() + p.broyden_maxiter;  // 2 scalars
     hipLaunchKernelGGL(k_diff_dots, dim3(dot_blocks), dim3(256), 0, st, NewE,
                        E, dE, n6, dot_chunk, partial.as<double>());
-    hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st,
+    hipLaunchKernelGGL(k_dots_final, dim3(2), dim3(64), 0, st,
                        partial.as<double>(), dot_blocks, 2, dots);
     FS_LAUNCHED();
     FS_TRY(jsolve(NewE, tmp, iter - 1));
