@@ -297,7 +297,8 @@ __global__ void __launch_bounds__(256)
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
     k_fft_z_r2c(const T *__restrict__ R, cx<T> *__restrict__ C, FftPlan pl,
-                const cx<T> *__restrict__ twg, int nx, int ny, int P) {
+                const cx<T> *__restrict__ twg, int nx, int ny, int P,
+                FastDiv dXB) {
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n, nh = n / 2 + 1, XB = 2 * P;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -305,25 +306,26 @@ __global__ void __launch_bounds__(1024)
   const int comp = blockIdx.y;
   const int nxb = nx / XB;
   const int x0 = (blockIdx.x % nxb) * XB, y = blockIdx.x / nxb;
+  const int pn = fft_padded_len(n);
   const T *Rc = R + (size_t)comp * nx * ny * n;
   cx<T> *Cc = C + (size_t)comp * nh * ny * nx;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
   for (int e = threadIdx.x; e < XB * n; e += blockDim.x) {
-    int i = e / n, z = e - i * n;
+    int i = pl.dn.q(e), z = e - i * n;
     T v = Rc[((size_t)(x0 + i) * ny + y) * n + z];
     if (i & 1)
-      a[(i >> 1) * n + z].y = v;
+      a[(i >> 1) * pn + fft_pad(z)].y = v;
     else
-      a[(i >> 1) * n + z].x = v;
+      a[(i >> 1) * pn + fft_pad(z)].x = v;
   }
   __syncthreads();
-  lds_fft<T, E>(a, pl, P, 1, n, tw, false);
+  lds_fft<T, E, true>(a, pl, P, 1, pn, tw, false);
   const T half = (T)0.5;
   for (int e = threadIdx.x; e < nh * XB; e += blockDim.x) {
-    int kz = e / XB, i = e - kz * XB;
+    int kz = dXB.q(e), i = e - kz * XB;
     int p = i >> 1;
-    cx<T> z = a[p * n + kz];
-    cx<T> w = a[p * n + (kz == 0 ? 0 : n - kz)];
+    cx<T> z = a[p * pn + fft_pad(kz)];
+    cx<T> w = a[p * pn + fft_pad(kz == 0 ? 0 : n - kz)];
     cx<T> o = (i & 1) ? cx<T>{half * (z.y + w.y), half * (w.x - z.x)}
                       : cx<T>{half * (z.x + w.x), half * (z.y - w.y)};
     Cc[((size_t)kz * ny + y) * nx + x0 + i] = o;
@@ -334,7 +336,8 @@ __global__ void __launch_bounds__(1024)
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
     k_fft_z_c2r(const cx<T> *__restrict__ C, T *__restrict__ R, FftPlan pl,
-                const cx<T> *__restrict__ twg, int nx, int ny, int P) {
+                const cx<T> *__restrict__ twg, int nx, int ny, int P,
+                FastDiv dP) {
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n, nh = n / 2 + 1, XB = 2 * P;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -342,25 +345,26 @@ __global__ void __launch_bounds__(1024)
   const int comp = blockIdx.y;
   const int nxb = nx / XB;
   const int x0 = (blockIdx.x % nxb) * XB, y = blockIdx.x / nxb;
+  const int pn = fft_padded_len(n);
   T *Rc = R + (size_t)comp * nx * ny * n;
   const cx<T> *Cc = C + (size_t)comp * nh * ny * nx;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
   for (int e = threadIdx.x; e < nh * P; e += blockDim.x) {
-    int k = e / P, p = e - k * P;
+    int k = dP.q(e), p = e - k * P;
     const cx<T> *src = Cc + ((size_t)k * ny + y) * nx + x0 + 2 * p;
     cx<T> A = src[0], B = src[1];
     if (k == 0 || 2 * k == n) {  // DC / Nyquist of a real line are real
       A.y = 0;
       B.y = 0;
     }
-    a[p * n + k] = {A.x - B.y, A.y + B.x};
-    if (k > 0 && 2 * k < n) a[p * n + n - k] = {A.x + B.y, B.x - A.y};
+    a[p * pn + fft_pad(k)] = {A.x - B.y, A.y + B.x};
+    if (k > 0 && 2 * k < n) a[p * pn + fft_pad(n - k)] = {A.x + B.y, B.x - A.y};
   }
   __syncthreads();
-  lds_fft<T, E>(a, pl, P, 1, n, tw, true);
+  lds_fft<T, E, true>(a, pl, P, 1, pn, tw, true);
   for (int e = threadIdx.x; e < XB * n; e += blockDim.x) {
-    int i = e / n, z = e - i * n;
-    cx<T> v = a[(i >> 1) * n + z];
+    int i = pl.dn.q(e), z = e - i * n;
+    cx<T> v = a[(i >> 1) * pn + fft_pad(z)];
     Rc[((size_t)(x0 + i) * ny + y) * n + z] = (i & 1) ? v.y : v.x;
   }
 }
@@ -369,7 +373,7 @@ __global__ void __launch_bounds__(1024)
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
     k_fft_y(cx<T> *__restrict__ C, FftPlan pl, const cx<T> *__restrict__ twg,
-            int nx, int nh, int B, int inverse) {
+            int nx, int nh, int B, int inverse, FastDiv dB) {
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n;  // ny
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -380,13 +384,13 @@ __global__ void __launch_bounds__(1024)
   cx<T> *base = C + ((size_t)comp * nh + kz) * n * nx + x0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
   for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    int m = e / B, col = e - m * B;
+    int m = dB.q(e), col = e - m * B;
     a[e] = base[(size_t)m * nx + col];
   }
   __syncthreads();
-  lds_fft<T, E>(a, pl, B, B, 1, tw, inverse != 0);
+  lds_fft<T, E>(a, pl, B, B, 1, tw, inverse != 0, dB);
   for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    int m = e / B, col = e - m * B;
+    int m = dB.q(e), col = e - m * B;
     base[(size_t)m * nx + col] = a[e];
   }
 }
@@ -402,12 +406,14 @@ struct ProjConst {
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
     k_fft_x_project(cx<T> *__restrict__ C, FftPlan pl,
-                    const cx<T> *__restrict__ twg, ProjConst pc, int Q) {
+                    const cx<T> *__restrict__ twg, ProjConst pc, int Q,
+                    FastDiv dQ) {
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n;  // nx
   const int nh = pc.nh;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
-  cx<T> *a = tw + n;  // [3][Q][n]
+  cx<T> *a = tw + n;  // [3][Q][padded n]
+  const int pn = fft_padded_len(n);
   const int nyb = pc.ny / Q;
   const int y0 = (blockIdx.x % nyb) * Q, kz = blockIdx.x / nyb;
   const size_t comp_stride = (size_t)nh * pc.ny * n;
@@ -415,24 +421,25 @@ __global__ void __launch_bounds__(1024)
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
   const int nseq = 3 * Q;
   for (int e = threadIdx.x; e < nseq * n; e += blockDim.x) {
-    int sq = e / n, x = e - sq * n;
-    int c = sq / Q, ql = sq - c * Q;
-    a[e] = base[c * comp_stride + (size_t)ql * n + x];
+    int sq = pl.dn.q(e), x = e - sq * n;
+    int c = dQ.q(sq), ql = sq - c * Q;
+    a[sq * pn + fft_pad(x)] = base[c * comp_stride + (size_t)ql * n + x];
   }
   __syncthreads();
-  lds_fft<T, E>(a, pl, nseq, 1, n, tw, false);
+  lds_fft<T, E, true>(a, pl, nseq, 1, pn, tw, false);
   // flow_solve (FCMfunctions.cpp:648-736)
   const T q3 = (T)(kz * pc.kfz);
   const T sc = (T)pc.scale;
   for (int e = threadIdx.x; e < Q * n; e += blockDim.x) {
-    int ql = e / n, x = e - ql * n;
+    int ql = pl.dn.q(e), x = e - ql * n;
     int y = y0 + ql;
     T q1 = (T)(((x <= pc.nx / 2) ? x : x - pc.nx) * pc.kfx);
     T q2 = (T)(((y <= pc.ny / 2) ? y : y - pc.ny) * pc.kfy);
     T qq = q2 * q2 + q1 * q1 + (T)1e-18;
     T norm = (T)1 / (qq + q3 * q3);
-    cx<T> f1 = a[(0 * Q + ql) * n + x], f2 = a[(1 * Q + ql) * n + x],
-          f3 = a[(2 * Q + ql) * n + x];
+    const int px = fft_pad(x);
+    cx<T> f1 = a[(0 * Q + ql) * pn + px], f2 = a[(1 * Q + ql) * pn + px],
+          f3 = a[(2 * Q + ql) * pn + px];
     cx<T> u1, u2, u3;
     T kf = (q1 * f1.x + q2 * f2.x + q3 * f3.x) * norm;
     u1.x = norm * (f1.x - q1 * kf) * sc;
@@ -443,16 +450,16 @@ __global__ void __launch_bounds__(1024)
     u2.y = norm * (f2.y - q2 * kf) * sc;
     u3.y = norm * (f3.y - q3 * kf) * sc;
     if (x == 0 && y == 0 && kz == 0) u1 = u2 = u3 = cx<T>{0, 0};
-    a[(0 * Q + ql) * n + x] = u1;
-    a[(1 * Q + ql) * n + x] = u2;
-    a[(2 * Q + ql) * n + x] = u3;
+    a[(0 * Q + ql) * pn + px] = u1;
+    a[(1 * Q + ql) * pn + px] = u2;
+    a[(2 * Q + ql) * pn + px] = u3;
   }
   __syncthreads();
-  lds_fft<T, E>(a, pl, nseq, 1, n, tw, true);
+  lds_fft<T, E, true>(a, pl, nseq, 1, pn, tw, true);
   for (int e = threadIdx.x; e < nseq * n; e += blockDim.x) {
-    int sq = e / n, x = e - sq * n;
-    int c = sq / Q, ql = sq - c * Q;
-    base[c * comp_stride + (size_t)ql * n + x] = a[e];
+    int sq = pl.dn.q(e), x = e - sq * n;
+    int c = dQ.q(sq), ql = sq - c * Q;
+    base[c * comp_stride + (size_t)ql * n + x] = a[sq * pn + fft_pad(x)];
   }
 }
 
@@ -587,6 +594,7 @@ static bool with_E(int E, F f) {
     case 8: f(std::integral_constant<int, 8>{}); return true;
     case 10: f(std::integral_constant<int, 10>{}); return true;
     case 12: f(std::integral_constant<int, 12>{}); return true;
+    case 16: f(std::integral_constant<int, 16>{}); return true;
     case 20: f(std::integral_constant<int, 20>{}); return true;
     case 24: f(std::integral_constant<int, 24>{}); return true;
     case 30: f(std::integral_constant<int, 30>{}); return true;
@@ -652,7 +660,7 @@ static int set_fft_lds_attr(const FftPlan &plx, const FftPlan &ply,
 static bool good_size(int n) {
   if (n < 2 || n > 1024 || (n & 1)) return false;
   FftPlan p;
-  return make_plan(n, &p);
+  return make_plan(n, &p, 8);
 }
 
 static void twiddles_host(int n, std::vector<double> &re,
@@ -695,9 +703,11 @@ int FcmImpl::init(const fs_fcm_params &pp, int maxnp, int prec) {
   FS_CHECK(p.dx > 0 && p.sigma > 0 && p.sigma_dip > 0 && p.mu > 0 && p.a > 0,
            FS_ERR_ARG, "dx, sigma, sigma_dip, mu, a must be positive");
   FS_CHECK(maxnp >= 1, FS_ERR_ARG, "max_np must be >= 1");
-  make_plan(p.nx, &plx);
-  make_plan(p.ny, &ply);
-  make_plan(p.nz, &plz);
+  // radix 16 only for the FP32 fused x pass (measured at 256^3: x pass
+  // 162 -> 146 us with 4 lines per workgroup; z/y passes and FP64 slower)
+  make_plan(p.nx, &plx, prec == FS_FP32 ? 16 : 8);
+  make_plan(p.ny, &ply, 8);
+  make_plan(p.nz, &plz, 8);
   tsz = (prec == FS_FP64) ? 8 : 4;
   const size_t N = (size_t)p.nx * p.ny * p.nz;
   const size_t NC = (size_t)p.nx * p.ny * (p.nz / 2 + 1);
@@ -721,7 +731,7 @@ int FcmImpl::init(const fs_fcm_params &pp, int maxnp, int prec) {
   tune.fft_batch = (prec == FS_FP64) ? 8 : 16;
   tune.spread_tile = 16;
   tune.spread_zrun = 8;
-  tune.fft_xlines = 2;
+  tune.fft_xlines = (prec == FS_FP32) ? 4 : 2;
   FS_TRY(set_geometry());
   for (int i = 0; i < 8; ++i) FS_HIP(hipEventCreate(&ev[i]));
   if (prec == FS_FP64)
@@ -788,16 +798,20 @@ int FcmImpl::set_geometry() {
   FS_TRY(scan_tmp.alloc((size_t)(nbins / kScanBlock + 4) * sizeof(int)));
   // FFT pass geometry: z rows per workgroup (even divisor of nx), y-pass run
   // of consecutive x, x-pass lines per workgroup; workgroups stay <= 1024
+  const size_t cs = 2 * tsz, lds_max = 160 * 1024;
   zP = 0;
   for (int xb = std::min(16, p.nx); xb >= 2; --xb)
-    if (xb % 2 == 0 && p.nx % xb == 0 && p.nz / plz.E * (xb / 2) <= 1024) {
+    if (xb % 2 == 0 && p.nx % xb == 0 && p.nz / plz.E * (xb / 2) <= 1024 &&
+        cs * (p.nz + (size_t)(xb / 2) * fft_padded_len(p.nz)) <= lds_max) {
       zP = xb / 2;
       break;
     }
   yB = largest_divisor_le(p.nx, tune.fft_batch);
   while (yB > 1 && p.ny / ply.E * yB > 1024) yB = largest_divisor_le(p.nx, yB - 1);
   xQ = largest_divisor_le(p.ny, tune.fft_xlines);
-  while (xQ > 1 && p.nx / plx.E * 3 * xQ > 1024) xQ = largest_divisor_le(p.ny, xQ - 1);
+  while (xQ > 1 && (p.nx / plx.E * 3 * xQ > 1024 ||
+                    cs * (p.nx + 3 * (size_t)xQ * fft_padded_len(p.nx)) > lds_max))
+    xQ = largest_divisor_le(p.ny, xQ - 1);
   FS_CHECK(zP >= 1 && p.ny / ply.E * yB <= 1024 && p.nx / plx.E * 3 * xQ <= 1024,
            FS_ERR_ARG, "no FFT workgroup shape fits %dx%dx%d", p.nx, p.ny, p.nz);
   return FS_OK;
@@ -876,15 +890,17 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   bool ok = with_E(plz.E, [&](auto e) {
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_z_r2c<T, EE>), dim3(p.ny * (p.nx / (2 * zP)), 3),
-                       dim3(p.nz / EE * zP), csz * (p.nz + (size_t)zP * p.nz), st,
+                       dim3(p.nz / EE * zP),
+                       csz * (p.nz + (size_t)zP * fft_padded_len(p.nz)), st,
                        R.as<T>(), C.as<cx<T>>(), plz, twz.as<cx<T>>(), p.nx,
-                       p.ny, zP);
+                       p.ny, zP, make_div(2 * zP));
   });
   ok &= with_E(ply.E, [&](auto e) {
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_y<T, EE>), dim3(nh * (p.nx / yB), 3),
                        dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st,
-                       C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 0);
+                       C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 0,
+                       make_div(yB));
   });
   FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
@@ -902,8 +918,9 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_x_project<T, EE>), dim3(nh * (p.ny / xQ)),
                        dim3(p.nx / EE * 3 * xQ),
-                       csz * (p.nx + 3 * (size_t)xQ * p.nx), st, C.as<cx<T>>(),
-                       plx, twx.as<cx<T>>(), pc, xQ);
+                       csz * (p.nx + 3 * (size_t)xQ * fft_padded_len(p.nx)), st,
+                       C.as<cx<T>>(),
+                       plx, twx.as<cx<T>>(), pc, xQ, make_div(xQ));
   });
   FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
@@ -912,14 +929,16 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_y<T, EE>), dim3(nh * (p.nx / yB), 3),
                        dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st,
-                       C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 1);
+                       C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 1,
+                       make_div(yB));
   });
   ok &= with_E(plz.E, [&](auto e) {
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_z_c2r<T, EE>), dim3(p.ny * (p.nx / (2 * zP)), 3),
-                       dim3(p.nz / EE * zP), csz * (p.nz + (size_t)zP * p.nz), st,
+                       dim3(p.nz / EE * zP),
+                       csz * (p.nz + (size_t)zP * fft_padded_len(p.nz)), st,
                        C.as<cx<T>>(), R.as<T>(), plz, twz.as<cx<T>>(), p.nx,
-                       p.ny, zP);
+                       p.ny, zP, make_div(zP));
   });
   FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
@@ -1082,10 +1101,11 @@ extern "C" int fs_fcm_set_tuning(fs_fcm *h, const fs_fcm_tuning *t) {
   FS_CHECK(nt.fft_batch >= 1 && nt.fft_batch <= 64 && nt.fft_xlines >= 1 &&
                nt.fft_xlines <= 16,
            FS_ERR_ARG, "fft_batch must be in [1,64], fft_xlines in [1,16]");
-  // LDS: y pass holds (B+1)*ny complex, x pass (3Q+1)*nx
+  // LDS: y pass holds (B+1)*ny complex, x pass nx + 3Q*(padded nx)
   size_t cs = (size_t)h->impl.tsz * 2;
-  size_t lds = std::max(cs * (h->impl.p.ny + (size_t)nt.fft_batch * h->impl.p.ny),
-                        cs * (h->impl.p.nx + 3 * (size_t)nt.fft_xlines * h->impl.p.nx));
+  size_t lds = std::max(
+      cs * (h->impl.p.ny + (size_t)nt.fft_batch * h->impl.p.ny),
+      cs * (h->impl.p.nx + 3 * (size_t)nt.fft_xlines * fs::fft_padded_len(h->impl.p.nx)));
   FS_CHECK(lds <= 160 * 1024, FS_ERR_ARG, "tuning needs %zu B of LDS", lds);
   h->impl.tune = nt;
   int rc = h->impl.set_geometry();

@@ -19,39 +19,72 @@ namespace fs {
 
 constexpr int kMaxStages = 12;
 
+// Division by a launch constant d without the integer divide sequence:
+// q = (t * m) >> 32 with m = floor(2^32 / d) + 1 is exact for t < 2^16 and
+// 1 <= d <= 2^16 (the error t * (m - 2^32/d) / 2^32 < 2^-16 <= 1/d never
+// crosses an integer); every FFT index here is below 2^16.
+struct FastDiv {
+  unsigned long long m;
+  int d;
+  __device__ __forceinline__ int q(int t) const {
+    return (int)(((unsigned long long)(unsigned)t * m) >> 32);
+  }
+};
+inline FastDiv make_div(int d) {
+  return FastDiv{(1ull << 32) / (unsigned long long)d + 1ull, d};
+}
+
 struct FftPlan {
   int n;
   int nst;
   int E;  // elements per thread (multiple of every radix)
   int rad[kMaxStages];
+  FastDiv dn;               // by n
+  FastDiv dnb[kMaxStages];  // by n / rad[s]
+  FastDiv dns[kMaxStages];  // by the stage's span ns (product of earlier radices)
 };
 
 inline int gcd_i(int a, int b) { return b == 0 ? a : gcd_i(b, a % b); }
 inline int lcm_i(int a, int b) { return a / gcd_i(a, b) * b; }
 
-// Radices for n = 2^a 3^b 5^c: largest power-of-two radices first, reduced
-// until lcm(radices) <= 30 (the largest register tile instantiated).
-inline bool make_plan(int n, FftPlan *p) {
+// Radices for n = 2^a 3^b 5^c: largest power-of-two radices first (up to
+// max_radix: 16, 8, 4, 2), reduced until lcm(radices) <= 30 (the largest
+// register tile instantiated). Radix 16 halves the LDS passes of a
+// 256-point transform (two stages instead of three) but also halves the
+// threads of a workgroup; it pays only where the workgroup stays wide.
+inline bool make_plan(int n, FftPlan *p, int max_radix = 16) {
   int a = 0, b = 0, c = 0, m = n;
   while (m % 2 == 0) m /= 2, ++a;
   while (m % 3 == 0) m /= 3, ++b;
   while (m % 5 == 0) m /= 5, ++c;
   if (m != 1 || n < 2) return false;
-  for (int maxr2 = 8; maxr2 >= 2; maxr2 /= 2) {
+  for (int maxr2 = max_radix; maxr2 >= 2; maxr2 /= 2) {
     p->n = n;
     p->nst = 0;
     int r2 = a;
     while (r2 > 0 && p->nst < kMaxStages) {
-      int r = (r2 >= 3 && maxr2 >= 8) ? 8 : (r2 >= 2 && maxr2 >= 4) ? 4 : 2;
+      int r = (r2 >= 4 && maxr2 >= 16) ? 16
+              : (r2 >= 3 && maxr2 >= 8) ? 8
+              : (r2 >= 2 && maxr2 >= 4) ? 4
+                                        : 2;
       p->rad[p->nst++] = r;
-      r2 -= (r == 8 ? 3 : r == 4 ? 2 : 1);
+      r2 -= (r == 16 ? 4 : r == 8 ? 3 : r == 4 ? 2 : 1);
     }
     for (int i = 0; i < b && p->nst < kMaxStages; ++i) p->rad[p->nst++] = 3;
     for (int i = 0; i < c && p->nst < kMaxStages; ++i) p->rad[p->nst++] = 5;
     int L = 1;
     for (int s = 0; s < p->nst; ++s) L = lcm_i(L, p->rad[s]);
     p->E = (L < 6) ? 8 : L;  // n >= 8 with only 2s always has an 8
-    if (p->E <= 30) return true;
+    if (p->E <= 30) {
+      p->dn = make_div(n);
+      int ns = 1;
+      for (int s = 0; s < p->nst; ++s) {
+        p->dnb[s] = make_div(n / p->rad[s]);
+        p->dns[s] = make_div(ns);
+        ns *= p->rad[s];
+      }
+      return true;
+    }
   }
   return false;
 }
@@ -109,6 +142,42 @@ __device__ __forceinline__ void dft8(cx<T> *v, bool inv) {
   v[7] = csub(e[3], o3);
 }
 
+// 16 points as 4 x 4: DFT4 over n1 for each n2, twiddle by W16^(n2 k1),
+// DFT4 over n2; natural order in and out.
+template <typename T>
+__device__ __forceinline__ void dft16(cx<T> *v, bool inv) {
+  const T c = (T)0.92387953251128675612818318939679;  // cos(pi/8)
+  const T s = (T)0.38268343236508977172845998403040;  // sin(pi/8)
+  const T h = (T)0.70710678118654752440084436210485;
+  const T sg = inv ? (T)1 : (T)-1;  // sign of the imaginary parts
+  cx<T> y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    cx<T> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+    dft4(t, inv);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
+  }
+  const cx<T> w1 = {c, sg * s}, w2 = {h, sg * h}, w3 = {s, sg * c},
+              w6 = {-h, sg * h}, w9 = {-c, -sg * s};
+  y[1][1] = cmul(y[1][1], w1);
+  y[1][2] = cmul(y[1][2], w2);
+  y[1][3] = cmul(y[1][3], w3);
+  y[2][1] = cmul(y[2][1], w2);
+  y[2][2] = rot(y[2][2], inv);  // W16^4 = -i (forward)
+  y[2][3] = cmul(y[2][3], w6);
+  y[3][1] = cmul(y[3][1], w3);
+  y[3][2] = cmul(y[3][2], w6);
+  y[3][3] = cmul(y[3][3], w9);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cx<T> t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    dft4(t, inv);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void dft3(cx<T> *v, bool inv) {
   const T s3 = (T)0.86602540378443864676372317075294;
@@ -149,13 +218,32 @@ __device__ __forceinline__ void dftR(cx<T> *v, bool inv) {
   if (R == 4) dft4(v, inv);
   if (R == 5) dft5(v, inv);
   if (R == 8) dft8(v, inv);
+  if (R == 16) dft16(v, inv);
+}
+
+// Padded LDS layout for contiguous sequences: element i of a sequence sits at
+// i + i/8. Early Stockham stages write with a lane stride of R elements
+// (d = j*R + r); unpadded, 16 (FP32) or 32 (FP64) lanes of a wave would hit
+// one bank. One pad element per 8 spreads those strided writes over the
+// banks, and contiguous reads stay conflict-free.
+__host__ __device__ __forceinline__ int fft_pad(int i) { return i + (i >> 3); }
+__host__ __device__ __forceinline__ int fft_padded_len(int n) {
+  return fft_pad(n - 1) + 1;
+}
+
+// LDS index of element i of sequence col: sequences contiguous and padded
+// (PAD, stride sc = padded length) or element-major (i * sm + col * sc)
+template <bool PAD>
+__device__ __forceinline__ int lds_idx(int i, int col, int sm, int sc) {
+  return PAD ? col * sc + fft_pad(i) : i * sm + col * sc;
 }
 
 // One in-place stage; blockDim.x * (E/R) == (n/R) * ncols exactly.
-template <typename T, int R, int E>
+template <typename T, int R, int E, bool PAD>
 __device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int ns, int ncols,
                                           int sm, int sc, const cx<T> *tw,
-                                          bool inv) {
+                                          bool inv, FastDiv dnb, FastDiv dns,
+                                          FastDiv dcols) {
   constexpr int NB = E / R;
   const int nb = n / R;
   const int twstep = n / (ns * R);
@@ -166,17 +254,19 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int ns, int ncols,
     const int t = threadIdx.x + m * blockDim.x;
     int j, col;
     if (sm == 1) {  // sequences contiguous: butterflies fastest
-      j = t % nb;
-      col = t / nb;
+      col = dnb.q(t);
+      j = t - col * nb;
     } else {  // sequences interleaved: columns fastest
-      col = t % ncols;
-      j = t / ncols;
+      j = dcols.q(t);
+      col = t - j * ncols;
     }
     jj[m] = j;
     cc[m] = col;
-    const int k = j % ns;
+    const int jq = dns.q(j);
+    const int k = j - jq * ns;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[m * R + r] = buf[(j + r * nb) * sm + col * sc];
+    for (int r = 0; r < R; ++r)
+      v[m * R + r] = buf[lds_idx<PAD>(j + r * nb, col, sm, sc)];
     if (ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
@@ -191,30 +281,43 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int ns, int ncols,
 #pragma unroll
   for (int m = 0; m < NB; ++m) {
     const int j = jj[m], col = cc[m];
-    const int d = (j / ns) * ns * R + (j % ns);
+    const int jq = dns.q(j);
+    const int d = jq * ns * R + (j - jq * ns);
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[(d + r * ns) * sm + col * sc] = v[m * R + r];
+    for (int r = 0; r < R; ++r)
+      buf[lds_idx<PAD>(d + r * ns, col, sm, sc)] = v[m * R + r];
   }
   __syncthreads();
 }
 
 // All stages of the plan, in place in buf. Needs blockDim.x = n*ncols/E.
-template <typename T, int E>
+// PAD: contiguous padded sequences (sm = 1, sc = fft_padded_len(n)).
+template <typename T, int E, bool PAD = false>
 __device__ void lds_fft(cx<T> *buf, const FftPlan &pl, int ncols, int sm,
-                        int sc, const cx<T> *tw, bool inv) {
+                        int sc, const cx<T> *tw, bool inv,
+                        FastDiv dcols = FastDiv{0, 0}) {
   int ns = 1;
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.rad[s];
+    const FastDiv a = pl.dnb[s], b = pl.dns[s];
+    if constexpr (E % 16 == 0)
+      if (R == 16)
+        fft_stage<T, 16, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
     if constexpr (E % 8 == 0)
-      if (R == 8) fft_stage<T, 8, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
+      if (R == 8)
+        fft_stage<T, 8, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
     if constexpr (E % 4 == 0)
-      if (R == 4) fft_stage<T, 4, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
+      if (R == 4)
+        fft_stage<T, 4, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
     if constexpr (E % 2 == 0)
-      if (R == 2) fft_stage<T, 2, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
+      if (R == 2)
+        fft_stage<T, 2, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
     if constexpr (E % 3 == 0)
-      if (R == 3) fft_stage<T, 3, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
+      if (R == 3)
+        fft_stage<T, 3, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
     if constexpr (E % 5 == 0)
-      if (R == 5) fft_stage<T, 5, E>(buf, pl.n, ns, ncols, sm, sc, tw, inv);
+      if (R == 5)
+        fft_stage<T, 5, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
     ns *= R;
   }
 }
