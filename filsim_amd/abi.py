@@ -78,6 +78,7 @@ PROTOS = {
     "fs_fcm_stage_times": [P, FP],
     "fs_fcm_get_tuning": [P, C.POINTER(FcmTuning)],
     "fs_fcm_set_tuning": [P, C.POINTER(FcmTuning)],
+    "fs_fcm_autotune": [P, C.c_int, C.c_int, C.POINTER(FcmTuning)],
     "fs_colbar_create": [C.POINTER(ColbarParams), I, IP, IP, C.POINTER(P)],
     "fs_colbar_destroy": [P],
     "fs_colbar_dims": [P, IP],
@@ -123,6 +124,11 @@ PROTOS = {
     "fs_system_ws_record": [P, P, P, I],
     "fs_system_ws_count": [P, IP],
     "fs_system_last_step_info": [P, IP, _DP, _DP],
+    "fs_system_autotune": [P, C.c_int, C.POINTER(FcmTuning)],
+    "fs_batch_create": [C.POINTER(P), C.c_int, C.POINTER(P)],
+    "fs_batch_destroy": [P],
+    "fs_batch_size": [P, IP],
+    "fs_batch_timestep": [P, C.c_int, IP],
 }
 
 _lib = None

@@ -96,6 +96,13 @@ class Fcm:
         t = abi.FcmTuning(fft_batch, spread_tile, spread_zrun, fft_xlines)
         abi.check(self.lib.fs_fcm_set_tuning(self.h, C.byref(t)))
 
+    def autotune(self, np_, reps=3):
+        """Fastest launch shapes for np_ segments (fs_fcm_autotune); they
+        stay set. Returns the tuning."""
+        t = abi.FcmTuning()
+        abi.check(self.lib.fs_fcm_autotune(self.h, np_, reps, C.byref(t)))
+        return t
+
 
 class Colbar:
     """Collision barrier (struct LinkedList,
@@ -265,6 +272,38 @@ class System:
         abi.check(self.lib.fs_system_barrier(self.h, nt, 1 if link else 0,
                                              C.byref(md)))
         return md.value
+
+    def autotune(self, reps=3):
+        """Autotune the system's FCM launch shapes (fs_system_autotune)."""
+        t = abi.FcmTuning()
+        abi.check(self.lib.fs_system_autotune(self.h, reps, C.byref(t)))
+        return t
+
+
+class Batch:
+    """Independent systems advanced together through captured hipGraphs
+    (fs_batch_*); each reproduces its unbatched timesteps bit for bit. The
+    systems must outlive the batch."""
+
+    def __init__(self, systems):
+        self.lib = abi.load()
+        self.systems = list(systems)
+        arr = (C.c_void_p * len(self.systems))(*[s.h for s in self.systems])
+        self.h = C.c_void_p()
+        abi.check(self.lib.fs_batch_create(arr, len(self.systems),
+                                           C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.lib.fs_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def timestep(self, nt):
+        it = (C.c_int * len(self.systems))()
+        abi.check(self.lib.fs_batch_timestep(self.h, nt, it))
+        return list(it)
 
 
 def make_filaments(box, nw, dL=2.2, min_sep=2.2, seed=1, max_tries=100000):

@@ -132,13 +132,15 @@ __device__ __forceinline__ int cb_wrap(int v, int m) {
 // mode 0: F[i] += sum of pair forces, min distance; mode 1: count pairs
 // (i<j) inside the cutoff into *npairs and write them when room.
 __global__ void __launch_bounds__(64)
-    k_cb_apply(int np, const double *__restrict__ Y, CbGeom g, CbConst c,
+    k_cb_apply(const int *__restrict__ act, int np,
+               const double *__restrict__ Y, CbGeom g, CbConst c,
                const int *__restrict__ cell, const int *__restrict__ start,
                const int *__restrict__ ids, const int *__restrict__ sfil,
                const int *__restrict__ sseg, double *__restrict__ F,
                unsigned long long *__restrict__ min_bits, int mode,
                int *__restrict__ npairs, int max_pairs,
                int *__restrict__ pairs) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   __shared__ double s_min[64];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double mymin = 100000.0;
@@ -195,7 +197,9 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-__global__ void k_cb_init_min(unsigned long long *m, unsigned long long v) {
+__global__ void k_cb_init_min(const int *__restrict__ act,
+                              unsigned long long *m, unsigned long long v) {
+  if (act && !*act) return;
   *m = v;
 }
 
@@ -289,11 +293,11 @@ int ColbarImpl::apply(const double *Y, int nt, double *F, double *min_dist2,
   double init = 100000.0;
   unsigned long long ib;
   memcpy(&ib, &init, sizeof ib);
-  hipLaunchKernelGGL(k_cb_init_min, dim3(1), dim3(1), 0, st,
+  hipLaunchKernelGGL(k_cb_init_min, dim3(1), dim3(1), 0, st, act,
                      minb.as<unsigned long long>(), ib);
   if (np > 0)
     hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div(np, 64)), dim3(64), 0, st,
-                       np, Y, g, consts(nt), cell.as<int>(), start.as<int>(),
+                       act, np, Y, g, consts(nt), cell.as<int>(), start.as<int>(),
                        ids.as<int>(), sfil.as<int>(), sseg.as<int>(), F,
                        minb.as<unsigned long long>(), 0, (int *)nullptr, 0,
                        (int *)nullptr);
@@ -333,7 +337,8 @@ int ColbarImpl::pairs(const double *Y, int nt, int max_pairs, int *out,
   int *dn = d.as<int>(), *dp = dn + 1;
   hipLaunchKernelGGL(k_cb_fill, dim3(1), dim3(1), 0, 0, dn, 1, 0);
   if (np > 0)
-    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div(np, 64)), dim3(64), 0, 0, np,
+    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div(np, 64)), dim3(64), 0, 0,
+                       (const int *)nullptr, np,
                        Y, g, consts(nt), cell.as<int>(), start.as<int>(),
                        ids.as<int>(), sfil.as<int>(), sseg.as<int>(),
                        (double *)nullptr, (unsigned long long *)nullptr, 1, dn,

@@ -18,6 +18,9 @@ struct CbConst {
 struct ColbarImpl {
   fs_colbar_params p{};
   int np = 0;
+  // device flag read by the pair-force kernels of apply(): 0 = skip (a
+  // converged suspension of a batched sweep); nullptr = always run
+  const int *act = nullptr;
   int ncell = 0;
   CbGeom g{};
   bool linked = false;

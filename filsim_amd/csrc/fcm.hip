@@ -50,8 +50,9 @@ struct GaussConst {
 };
 
 template <typename T>
-__global__ void k_prepare_tab(int np, const double *__restrict__ Y,
+__global__ void k_prepare_tab(const int *__restrict__ act, int np, const double *__restrict__ Y,
                               GaussConst g, T *__restrict__ tab) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)np * 3 * g.ngd) return;
   const int i = (int)(t % g.ngd), d = (int)((t / g.ngd) % 3),
@@ -67,13 +68,14 @@ __global__ void k_prepare_tab(int np, const double *__restrict__ Y,
 }
 
 template <typename T>
-__global__ void k_prepare(int np, const double *__restrict__ Y,
+__global__ void k_prepare(const int *__restrict__ act, int np, const double *__restrict__ Y,
                           const double *__restrict__ F,
                           const double *__restrict__ Tq, GaussConst g,
                           int tx, int ty, int tz, int nbx, int nby,
                           int *__restrict__ origin, T *__restrict__ coef,
                           int *__restrict__ bin, int *__restrict__ bin_count,
                           int *__restrict__ slot) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= np) return;
   const int nn[3] = {g.nx, g.ny, g.nz};
@@ -95,7 +97,8 @@ __global__ void k_prepare(int np, const double *__restrict__ Y,
   slot[s] = atomicAdd(&bin_count[b], 1);
 }
 
-__global__ void k_fill_int(int *p, int n, int v) {
+__global__ void k_fill_int(const int *__restrict__ act, int *p, int n, int v) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -136,10 +139,11 @@ __global__ void k_sort_bins(int nbins, const int *__restrict__ start,
 // ---------------------------------------------------------------------
 template <typename T, int TZ>
 __global__ void __launch_bounds__(256)
-    k_spread(SpreadGeom G, const T *__restrict__ tab,
+    k_spread(const int *__restrict__ act, SpreadGeom G, const T *__restrict__ tab,
              const int *__restrict__ origin, const T *__restrict__ coef,
              const int *__restrict__ start, const int *__restrict__ ids,
              T *__restrict__ R) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   __shared__ int s_bstart[kMaxBinsPerTile + 1];
   __shared__ int s_bfirst[kMaxBinsPerTile];
   __shared__ int s_ntot;
@@ -296,9 +300,10 @@ __global__ void __launch_bounds__(256)
 // (row 2p in the real part, row 2p+1 in the imaginary part).
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
-    k_fft_z_r2c(const T *__restrict__ R, cx<T> *__restrict__ C, FftPlan pl,
+    k_fft_z_r2c(const int *__restrict__ act, const T *__restrict__ R, cx<T> *__restrict__ C, FftPlan pl,
                 const cx<T> *__restrict__ twg, int nx, int ny, int P,
                 FastDiv dXB) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n, nh = n / 2 + 1, XB = 2 * P;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -335,9 +340,10 @@ __global__ void __launch_bounds__(1024)
 // z inverse: the mirror of k_fft_z_r2c.
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
-    k_fft_z_c2r(const cx<T> *__restrict__ C, T *__restrict__ R, FftPlan pl,
+    k_fft_z_c2r(const int *__restrict__ act, const cx<T> *__restrict__ C, T *__restrict__ R, FftPlan pl,
                 const cx<T> *__restrict__ twg, int nx, int ny, int P,
                 FastDiv dP) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n, nh = n / 2 + 1, XB = 2 * P;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -372,8 +378,9 @@ __global__ void __launch_bounds__(1024)
 // y pass: one workgroup per (comp, kz, run of B consecutive x).
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
-    k_fft_y(cx<T> *__restrict__ C, FftPlan pl, const cx<T> *__restrict__ twg,
+    k_fft_y(const int *__restrict__ act, cx<T> *__restrict__ C, FftPlan pl, const cx<T> *__restrict__ twg,
             int nx, int nh, int B, int inverse, FastDiv dB) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n;  // ny
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -405,9 +412,10 @@ struct ProjConst {
 // consecutive y), the three components' contiguous x lines in LDS.
 template <typename T, int E>
 __global__ void __launch_bounds__(1024)
-    k_fft_x_project(cx<T> *__restrict__ C, FftPlan pl,
+    k_fft_x_project(const int *__restrict__ act, cx<T> *__restrict__ C, FftPlan pl,
                     const cx<T> *__restrict__ twg, ProjConst pc, int Q,
                     FastDiv dQ) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
   const int n = pl.n;  // nx
   const int nh = pc.nh;
@@ -478,10 +486,11 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 template <typename T, int NGD>
 __global__ void __launch_bounds__(256)
-    k_interp(int np, SpreadGeom G, double dx3, const T *__restrict__ tab,
+    k_interp(const int *__restrict__ act, int np, SpreadGeom G, double dx3, const T *__restrict__ tab,
              const int *__restrict__ origin, const int *__restrict__ ids,
              const T *__restrict__ R, double *__restrict__ V,
              double *__restrict__ W) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   constexpr int NC = NGD * NGD;          // (x, y) columns of a stencil
   constexpr int NP = NC * NGD;           // stencil points
   constexpr int PER = (NP + 63) / 64;    // points per lane
@@ -573,9 +582,10 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-__global__ void k_friction(int n3, const double *__restrict__ F,
+__global__ void k_friction(const int *__restrict__ act, int n3, const double *__restrict__ F,
                            const double *__restrict__ Tq, double a1, double a2,
                            double *__restrict__ V, double *__restrict__ W) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n3) {
     V[i] = a1 * F[i];
@@ -859,11 +869,11 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   };
   mark(0);
   // bin counting sort
-  hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbins, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbins, 256)), dim3(256), 0, st, act,
                      bin_count.as<int>(), nbins, 0);
   hipLaunchKernelGGL(k_prepare_tab<T>, dim3(ceil_div((long)np * 3 * p.ngd, 256)),
-                     dim3(256), 0, st, np, Y, g, tab.as<T>());
-  hipLaunchKernelGGL(k_prepare<T>, dim3(ceil_div(np, 64)), dim3(64), 0, st, np,
+                     dim3(256), 0, st, act, np, Y, g, tab.as<T>());
+  hipLaunchKernelGGL(k_prepare<T>, dim3(ceil_div(np, 64)), dim3(64), 0, st, act, np,
                      Y, F, Tq, g, geom.tx, geom.ty, geom.tz, geom.nbx, geom.nby,
                      origin.as<int>(), coef.as<T>(), bin.as<int>(),
                      bin_count.as<int>(), slot.as<int>());
@@ -879,7 +889,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   bool okz = with_TZ(geom.tz, [&](auto z) {
     constexpr int TZ = decltype(z)::value;
     hipLaunchKernelGGL((k_spread<T, TZ>), dim3(ntiles), dim3(geom.tx * geom.ty),
-                       0, st, geom, tab.as<T>(), origin.as<int>(), coef.as<T>(),
+                       0, st, act, geom, tab.as<T>(), origin.as<int>(), coef.as<T>(),
                        bin_start.as<int>(), ids.as<int>(), R.as<T>());
   });
   FS_CHECK(okz, FS_ERR_ARG, "spread dispatch failed");
@@ -891,14 +901,14 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_z_r2c<T, EE>), dim3(p.ny * (p.nx / (2 * zP)), 3),
                        dim3(p.nz / EE * zP),
-                       csz * (p.nz + (size_t)zP * fft_padded_len(p.nz)), st,
+                       csz * (p.nz + (size_t)zP * fft_padded_len(p.nz)), st, act,
                        R.as<T>(), C.as<cx<T>>(), plz, twz.as<cx<T>>(), p.nx,
                        p.ny, zP, make_div(2 * zP));
   });
   ok &= with_E(ply.E, [&](auto e) {
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_y<T, EE>), dim3(nh * (p.nx / yB), 3),
-                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st,
+                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st, act,
                        C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 0,
                        make_div(yB));
   });
@@ -918,7 +928,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_x_project<T, EE>), dim3(nh * (p.ny / xQ)),
                        dim3(p.nx / EE * 3 * xQ),
-                       csz * (p.nx + 3 * (size_t)xQ * fft_padded_len(p.nx)), st,
+                       csz * (p.nx + 3 * (size_t)xQ * fft_padded_len(p.nx)), st, act,
                        C.as<cx<T>>(),
                        plx, twx.as<cx<T>>(), pc, xQ, make_div(xQ));
   });
@@ -928,7 +938,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   ok = with_E(ply.E, [&](auto e) {
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_y<T, EE>), dim3(nh * (p.nx / yB), 3),
-                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st,
+                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st, act,
                        C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 1,
                        make_div(yB));
   });
@@ -936,7 +946,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
     constexpr int EE = decltype(e)::value;
     hipLaunchKernelGGL((k_fft_z_c2r<T, EE>), dim3(p.ny * (p.nx / (2 * zP)), 3),
                        dim3(p.nz / EE * zP),
-                       csz * (p.nz + (size_t)zP * fft_padded_len(p.nz)), st,
+                       csz * (p.nz + (size_t)zP * fft_padded_len(p.nz)), st, act,
                        C.as<cx<T>>(), R.as<T>(), plz, twz.as<cx<T>>(), p.nx,
                        p.ny, zP, make_div(zP));
   });
@@ -948,7 +958,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
     constexpr int NG = decltype(g)::value;
     hipLaunchKernelGGL((k_interp<T, NG>), dim3(8 * ceil_div(ceil_div(np, 4), 8)),
                        dim3(256), 0,
-                       st, np, geom, dx3, tab.as<T>(), origin.as<int>(),
+                       st, act, np, geom, dx3, tab.as<T>(), origin.as<int>(),
                        ids.as<int>(), R.as<T>(), V, W);
   });
   FS_CHECK(okn, FS_ERR_ARG, "interp dispatch failed");
@@ -975,7 +985,7 @@ int FcmImpl::friction(int np, const double *F, const double *Tq, double *V,
   double a1 = 1. / (6 * kPi * p.a * p.mu);
   double a2 = 1. / (8 * kPi * p.a * p.a * p.a * p.mu);
   hipLaunchKernelGGL(k_friction, dim3(ceil_div(3 * np, 256)), dim3(256), 0,
-                     st, 3 * np, F, Tq, a1, a2, V, W);
+                     st, act, 3 * np, F, Tq, a1, a2, V, W);
   FS_LAUNCHED();
   return FS_OK;
 }
@@ -1003,6 +1013,121 @@ int FcmImpl::stage_times(float *ms7) {
 // C-ABI
 // ---------------------------------------------------------------------
 using fs::FcmImpl;
+
+int FcmImpl::set_tuning(const fs_fcm_tuning &t) {
+  fs_fcm_tuning old = tune;
+  fs_fcm_tuning nt = old;
+  if (t.fft_batch > 0) nt.fft_batch = t.fft_batch;
+  if (t.spread_tile > 0) nt.spread_tile = t.spread_tile;
+  if (t.spread_zrun > 0) nt.spread_zrun = t.spread_zrun;
+  if (t.fft_xlines > 0) nt.fft_xlines = t.fft_xlines;
+  FS_CHECK(nt.fft_batch >= 1 && nt.fft_batch <= 64 && nt.fft_xlines >= 1 &&
+               nt.fft_xlines <= 16,
+           FS_ERR_ARG, "fft_batch must be in [1,64], fft_xlines in [1,16]");
+  // LDS: y pass holds (B+1)*ny complex, x pass nx + 3Q*(padded nx)
+  size_t cs = (size_t)tsz * 2;
+  size_t lds = std::max(cs * (p.ny + (size_t)nt.fft_batch * p.ny),
+                        cs * (p.nx + 3 * (size_t)nt.fft_xlines *
+                                         fft_padded_len(p.nx)));
+  FS_CHECK(lds <= 160 * 1024, FS_ERR_ARG, "tuning needs %zu B of LDS", lds);
+  tune = nt;
+  int rc = set_geometry();
+  if (rc != FS_OK) {
+    tune = old;
+    set_geometry();
+  }
+  return rc;
+}
+
+// synthetic segments for the autotuner: positions uniform in the box,
+// forces and torques uniform in [-1, 1] (splitmix64 of the element index)
+__global__ void k_synth(int np, double Lx, double Ly, double Lz,
+                        double *__restrict__ Y, double *__restrict__ F,
+                        double *__restrict__ T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 9 * np) return;
+  unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ull +
+                         0x2545F4914F6CDD1Dull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  const int arr = i / (3 * np), k = i - arr * 3 * np, d = k % 3;
+  if (arr == 0)
+    Y[k] = u * (d == 0 ? Lx : d == 1 ? Ly : Lz);
+  else
+    (arr == 1 ? F : T)[k] = 2.0 * u - 1.0;
+}
+
+int FcmImpl::autotune(int np, int reps, fs_fcm_tuning *best) {
+  FS_CHECK(np >= 1 && np <= max_np, FS_ERR_SIZE, "np=%d outside [1, %d]", np,
+           max_np);
+  FS_CHECK(reps >= 1, FS_ERR_ARG, "reps must be >= 1");
+  dbuf buf;
+  FS_TRY(buf.alloc(sizeof(double) * 15 * (size_t)np));
+  double *Y = buf.as<double>(), *F = Y + 3 * (size_t)np,
+         *T = F + 3 * (size_t)np, *V = T + 3 * (size_t)np,
+         *W = V + 3 * (size_t)np;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  FS_HIP(hipStreamCreate(&st));
+  int rc = FS_OK;
+  auto cleanup = [&]() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  };
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    cleanup();
+    FS_CHECK(false, FS_ERR_HIP, "event creation failed");
+  }
+  hipLaunchKernelGGL(k_synth, dim3(ceil_div(9 * np, 256)), dim3(256), 0, st, np,
+                     p.nx * p.dx, p.ny * p.dx, p.nz * p.dx, Y, F, T);
+  auto time_it = [&](float *ms) -> int {
+    FS_TRY(mobility(np, Y, F, T, V, W, st));  // warm-up
+    FS_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) FS_TRY(mobility(np, Y, F, T, V, W, st));
+    FS_HIP(hipEventRecord(e1, st));
+    FS_HIP(hipEventSynchronize(e1));
+    FS_HIP(hipEventElapsedTime(ms, e0, e1));
+    return FS_OK;
+  };
+  fs_fcm_tuning cur = tune;
+  float tbest = 0;
+  rc = time_it(&tbest);
+  static const int tiles[] = {4, 8, 16}, zruns[] = {4, 8, 16},
+                   batches[] = {4, 8, 16, 32}, xlines[] = {1, 2, 4, 8};
+  struct Field {
+    int fs_fcm_tuning::*m;
+    const int *v;
+    int n;
+  } fields[] = {{&fs_fcm_tuning::spread_tile, tiles, 3},
+                {&fs_fcm_tuning::spread_zrun, zruns, 3},
+                {&fs_fcm_tuning::fft_batch, batches, 4},
+                {&fs_fcm_tuning::fft_xlines, xlines, 4}};
+  for (const Field &fd : fields) {
+    if (rc != FS_OK) break;
+    for (int k = 0; k < fd.n && rc == FS_OK; ++k) {
+      if (fd.v[k] == cur.*fd.m) continue;
+      fs_fcm_tuning trial = cur;
+      trial.*fd.m = fd.v[k];
+      if (set_tuning(trial) != FS_OK) continue;  // shape does not fit
+      float t = 0;
+      rc = time_it(&t);
+      if (rc == FS_OK && t < tbest) {
+        tbest = t;
+        cur = tune;
+      }
+    }
+    if (set_tuning(cur) != FS_OK) rc = FS_ERR_STATE;
+  }
+  cleanup();
+  if (rc != FS_OK) return rc;
+  if (best) *best = tune;
+  set_error("");
+  return FS_OK;
+}
 
 struct fs_fcm {
   FcmImpl impl;
@@ -1092,26 +1217,10 @@ extern "C" int fs_fcm_get_tuning(fs_fcm *h, fs_fcm_tuning *t) {
 
 extern "C" int fs_fcm_set_tuning(fs_fcm *h, const fs_fcm_tuning *t) {
   FS_CHECK(h && t, FS_ERR_ARG, "null argument");
-  fs_fcm_tuning old = h->impl.tune;
-  fs_fcm_tuning nt = old;
-  if (t->fft_batch > 0) nt.fft_batch = t->fft_batch;
-  if (t->spread_tile > 0) nt.spread_tile = t->spread_tile;
-  if (t->spread_zrun > 0) nt.spread_zrun = t->spread_zrun;
-  if (t->fft_xlines > 0) nt.fft_xlines = t->fft_xlines;
-  FS_CHECK(nt.fft_batch >= 1 && nt.fft_batch <= 64 && nt.fft_xlines >= 1 &&
-               nt.fft_xlines <= 16,
-           FS_ERR_ARG, "fft_batch must be in [1,64], fft_xlines in [1,16]");
-  // LDS: y pass holds (B+1)*ny complex, x pass nx + 3Q*(padded nx)
-  size_t cs = (size_t)h->impl.tsz * 2;
-  size_t lds = std::max(
-      cs * (h->impl.p.ny + (size_t)nt.fft_batch * h->impl.p.ny),
-      cs * (h->impl.p.nx + 3 * (size_t)nt.fft_xlines * fs::fft_padded_len(h->impl.p.nx)));
-  FS_CHECK(lds <= 160 * 1024, FS_ERR_ARG, "tuning needs %zu B of LDS", lds);
-  h->impl.tune = nt;
-  int rc = h->impl.set_geometry();
-  if (rc != FS_OK) {
-    h->impl.tune = old;
-    h->impl.set_geometry();
-  }
-  return rc;
+  return h->impl.set_tuning(*t);
+}
+
+extern "C" int fs_fcm_autotune(fs_fcm *h, int np, int reps, fs_fcm_tuning *best) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.autotune(np, reps, best);
 }

@@ -28,6 +28,9 @@ struct FcmImpl {
   SpreadGeom geom{};
   int nbins = 0;
   int zP = 1, yB = 1, xQ = 1;  // FFT pass workgroup shapes
+  // device flag read by every kernel of a solve: 0 = skip (a converged
+  // suspension of a batched sweep); nullptr = always run
+  const int *act = nullptr;
   dbuf R, C, twx, twy, twz;
   dbuf tab, origin, coef, bin, slot, ids, bin_count, bin_start, scan_tmp;
   hipEvent_t ev[8] = {};
@@ -40,6 +43,8 @@ struct FcmImpl {
   int friction(int np, const double *F, const double *Tq, double *V,
                double *W, hipStream_t st);
   int stage_times(float *ms7);
+  int set_tuning(const fs_fcm_tuning &t);
+  int autotune(int np, int reps, fs_fcm_tuning *best);
   template <typename T>
   int run(int np, const double *Y, const double *F, const double *Tq,
           double *V, double *W, hipStream_t st);

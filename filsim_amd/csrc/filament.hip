@@ -228,6 +228,7 @@ __global__ void k_guess(SysDev S) {
 
 // Filament::update (Filament.hpp:425-452, Bead.hpp:236-256) + unRobotArmify
 __global__ void k_update(SysDev S, const double *__restrict__ u) {
+  if (S.act && !*S.act) return;
   const int f = blockIdx.x;
   const int b0 = S.boff[f], l0 = S.loff[f], nw = S.nw[f];
   const size_t so = 6 * (size_t)b0;
@@ -256,6 +257,8 @@ __global__ void k_update(SysDev S, const double *__restrict__ u) {
 // + applyExternalForcesTorques (Filament.hpp:271-402) + assign_filament_data
 // (FCMfunctions.cpp:882-918). One workgroup per filament.
 __global__ void k_forces(SysDev S, int nt) {
+  if (S.act && !*S.act) return;
+  if (S.ntp) nt = *S.ntp;
   __shared__ double s_v[kMaxNw][3];  // elastic torque of link l
   __shared__ double s_t[kMaxNw][3];  // tangent of bead i
   const int f = blockIdx.x;
@@ -343,6 +346,8 @@ __global__ void __launch_bounds__(256)
     k_residual(SysDev S, int nt, const double *__restrict__ V,
                const double *__restrict__ W, double *__restrict__ err,
                unsigned long long *__restrict__ maxbits) {
+  if (S.act && !*S.act) return;
+  if (S.ntp) nt = *S.ntp;
   __shared__ double s_m[256];
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   double mx = 0.0;
@@ -382,7 +387,11 @@ __global__ void __launch_bounds__(256)
     atomicMax(maxbits, (unsigned long long)__double_as_longlong(s_m[0]));
 }
 
-__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+__global__ void k_set_u64(const int *__restrict__ act, unsigned long long *p,
+                          unsigned long long v) {
+  if (act && !*act) return;
+  *p = v;
+}
 
 __global__ void k_fill_i32(int *p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -642,6 +651,7 @@ __global__ void k_fill_f64(double *p, size_t n, double v) {
 // buildApproximateJacobian (Filament.hpp:803-867), one workgroup per
 // filament, one lane per bead / link.
 __global__ void k_jac_build(SysDev S, int nt, double *__restrict__ Jall) {
+  if (S.ntp) nt = *S.ntp;
   const int f = blockIdx.x;
   const int b0 = S.boff[f], l0 = S.loff[f], nw = S.nw[f];
   const int n = 6 * nw;
@@ -967,6 +977,7 @@ __global__ void __launch_bounds__(1024)
 __global__ void __launch_bounds__(256)
     k_jsolve_block(SysDev S, const double *__restrict__ JTall,
                    const double *__restrict__ b, double *__restrict__ x) {
+  if (S.act && !*S.act) return;
   __shared__ double sb[6 * kMaxNw];
   const int f = blockIdx.x;
   const int nw = S.nw[f], n = 6 * nw;
@@ -981,14 +992,21 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// partial[blk * ncol + i] = sum over rows of chunk blk of A_i[r] * b[r]
-// (A column-major, column stride lda). Fixed partition: deterministic.
+// partial[blk * ldp + i] = sum over rows of chunk blk of A_i[r] * b[r]
+// (A column-major, column stride lda) for i < ncol. Fixed partition:
+// deterministic. With ncolp the column count is *ncolp + noff, read on the
+// device (batched sweep: the grid covers the largest count, blocks past it
+// return); act: skip the launch when *act == 0.
 __global__ void __launch_bounds__(256)
-    k_dots_partial(const double *__restrict__ A, size_t lda, int ncol,
-                   const double *__restrict__ b, size_t nrow, int chunk,
-                   double *__restrict__ partial) {
+    k_dots_partial(const int *__restrict__ act, const double *__restrict__ A,
+                   size_t lda, int ncol, const int *__restrict__ ncolp,
+                   int noff, int ldp, const double *__restrict__ b,
+                   size_t nrow, int chunk, double *__restrict__ partial) {
+  if (act && !*act) return;
+  if (ncolp) ncol = *ncolp + noff;
   __shared__ double red[256];
   const int blk = blockIdx.x, i = blockIdx.y;
+  if (i >= ncol) return;
   const size_t lo = (size_t)blk * chunk;
   const size_t hi = min(nrow, lo + chunk);
   const double *Ai = A + lda * i;
@@ -1000,24 +1018,32 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[(size_t)blk * ncol + i] = red[0];
+  if (threadIdx.x == 0) partial[(size_t)blk * ldp + i] = red[0];
 }
 
 // one 64-lane workgroup per column: strided lane sums, then a butterfly in
 // a fixed order (deterministic)
 __global__ void __launch_bounds__(64)
-    k_dots_final(const double *__restrict__ partial, int nblk, int ncol,
-                 double *__restrict__ out) {
+    k_dots_final(const int *__restrict__ act, const double *__restrict__ partial,
+                 int nblk, int ldp, int ncol, const int *__restrict__ ncolp,
+                 int noff, double *__restrict__ out) {
+  if (act && !*act) return;
+  if (ncolp) ncol = *ncolp + noff;
   const int i = blockIdx.x, lane = threadIdx.x;
+  if (i >= ncol) return;
   double s = 0;
-  for (int b = lane; b < nblk; b += 64) s += partial[(size_t)b * ncol + i];
+  for (int b = lane; b < nblk; b += 64) s += partial[(size_t)b * ldp + i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (lane == 0) out[i] = s;
 }
 
-__global__ void k_hist_axpy(const double *__restrict__ C, size_t lda, int ncol,
+__global__ void k_hist_axpy(const int *__restrict__ act,
+                            const double *__restrict__ C, size_t lda, int ncol,
+                            const int *__restrict__ ncolp, int noff,
                             const double *__restrict__ coef, size_t nrow,
                             double *__restrict__ x) {
+  if (act && !*act) return;
+  if (ncolp) ncol = *ncolp + noff;
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrow) return;
   double v = x[r];
@@ -1050,9 +1076,10 @@ __global__ void k_broyden_cols(const double *__restrict__ tmp,
 // dE = Enew - E and the per-block partial sums of dE.dE and dE.Enew
 // (fixed partition: deterministic).
 __global__ void __launch_bounds__(256)
-    k_diff_dots(const double *__restrict__ En, const double *__restrict__ Eo,
-                double *__restrict__ dE, size_t n, int chunk,
-                double *__restrict__ partial) {
+    k_diff_dots(const int *__restrict__ act, const double *__restrict__ En,
+                const double *__restrict__ Eo, double *__restrict__ dE,
+                size_t n, int chunk, double *__restrict__ partial) {
+  if (act && !*act) return;
   __shared__ double r1[256], r2[256];
   const size_t lo = (size_t)blockIdx.x * chunk;
   const size_t hi = min(n, lo + chunk);
@@ -1083,12 +1110,18 @@ __global__ void __launch_bounds__(256)
 // reference's second solve H_k dE equals g - Update, so its column
 // C_k = (-H_k dE - Update)/|dE| = -g/|dE|, D_k = dE/|dE|, and the next
 // update H_{k+1} E_new = g + C_k (D_k . E_new) = g (1 - dE.E_new/|dE|^2).
-__global__ void k_broyden_fused(const double *__restrict__ g,
+// Columns col of C and D (column stride n); with iterp, col = *iterp - 1.
+__global__ void k_broyden_fused(const int *__restrict__ act,
+                                const double *__restrict__ g,
                                 const double *__restrict__ dE,
                                 const double *__restrict__ dots, size_t n,
-                                double *__restrict__ Ccol,
-                                double *__restrict__ Dcol,
+                                double *__restrict__ Cm,
+                                double *__restrict__ Dm, int col,
+                                const int *__restrict__ iterp,
                                 double *__restrict__ upd) {
+  if (act && !*act) return;
+  if (iterp) col = *iterp - 1;
+  double *Ccol = Cm + n * (size_t)col, *Dcol = Dm + n * (size_t)col;
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const double nrm2 = dots[0], dEN = dots[1];
@@ -1099,8 +1132,45 @@ __global__ void k_broyden_fused(const double *__restrict__ g,
   upd[r] = gr * (1.0 - dEN / nrm2);
 }
 
+__global__ void k_copy_f64(const int *__restrict__ act,
+                           const double *__restrict__ src,
+                           double *__restrict__ dst, size_t n) {
+  if (act && !*act) return;
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[r] = src[r];
+}
+
+// Device control block of a suspension in a batched sweep: ctl[0] = Broyden
+// iterations done this step, ctl[1] = still iterating. The loop test of
+// main.cpp:494-503, 684-687 on the device: iterate while the residual's
+// infinity norm exceeds TOL and fewer than broyden_maxiter iterations ran.
+__global__ void k_ctl_begin(int *__restrict__ ctl,
+                            const unsigned long long *__restrict__ maxbits,
+                            double tol, int maxiter) {
+  const double m = __longlong_as_double((long long)*maxbits);
+  ctl[0] = 0;
+  ctl[1] = (m > tol && maxiter > 0) ? 1 : 0;
+}
+
+__global__ void k_ctl_after(int *__restrict__ ctl,
+                            const unsigned long long *__restrict__ maxbits,
+                            double tol, int maxiter) {
+  if (!ctl[1]) return;
+  const double m = __longlong_as_double((long long)*maxbits);
+  const int it = ctl[0] + 1;
+  ctl[0] = it;
+  ctl[1] = (m > tol && it < maxiter) ? 1 : 0;
+}
+
+// publish ints of a suspension into the batch's status array
+__global__ void k_publish(const int *__restrict__ src, int n,
+                          int *__restrict__ dst) {
+  if (threadIdx.x < n) dst[threadIdx.x] = src[threadIdx.x];
+}
+
 // Filament::step (Filament.hpp:455-478) + Bead::step (Bead.hpp:266-276)
 __global__ void k_step(SysDev S, int nt) {
+  if (S.ntp) nt = *S.ntp;
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < (size_t)S.np) {
     size_t g = t;
@@ -1248,6 +1318,9 @@ int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
                        std::max(2, p.broyden_maxiter)));
   FS_TRY(coefb.alloc(sizeof(double) * (p.broyden_maxiter + 2)));
   FS_TRY(flags.alloc(64));
+  FS_TRY(ctl.alloc(4 * sizeof(int)));
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(4), 0, st, ctl.as<int>(), 4, 0);
+  FS_LAUNCHED();
   FS_HIP(hipHostMalloc((void **)&hflags, 64, hipHostMallocDefault));
   d.np = np;
   d.nlam = nlam;
@@ -1291,6 +1364,7 @@ void SystemImpl::release() {
   partial.release();
   coefb.release();
   flags.release();
+  ctl.release();
   if (hflags) (void)hipHostFree(hflags);
   hflags = nullptr;
   fcm.release();
@@ -1317,7 +1391,8 @@ int SystemImpl::update(const double *u) {
   return FS_OK;
 }
 
-int SystemImpl::build_jacobian(int nt) {
+// Jacobian blocks and their inverses; the singular flag stays on the device
+int SystemImpl::build_jacobian_dev(int nt) {
   size_t nJ = (size_t)hjoff[nf];
   hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nJ + 255) / 256)),
                      dim3(256), 0, st, Jb.as<double>(), nJ, 0.0);
@@ -1344,6 +1419,11 @@ int SystemImpl::build_jacobian(int nt) {
                        Mb.as<double>(), Jinv_cm(), sing);
   }
   FS_LAUNCHED();
+  return FS_OK;
+}
+
+int SystemImpl::build_jacobian(int nt) {
+  FS_TRY(build_jacobian_dev(nt));
   // history reset (FilamentJacobianSolver.hpp:40-41)
   size_t nh = 6 * (size_t)np * p.broyden_maxiter;
   hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nh + 255) / 256)),
@@ -1351,6 +1431,7 @@ int SystemImpl::build_jacobian(int nt) {
   hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nh + 255) / 256)),
                      dim3(256), 0, st, Dm.as<double>(), nh, 0.0);
   FS_LAUNCHED();
+  int *sing = flags.as<int>() + 4;
   FS_HIP(hipMemcpyAsync(hflags + 4, sing, sizeof(int), hipMemcpyDeviceToHost, st));
   FS_HIP(hipStreamSynchronize(st));
   FS_CHECK(((int *)hflags)[4] == 0, FS_ERR_ARG,
@@ -1362,19 +1443,30 @@ int SystemImpl::jsolve(const double *b, double *x, int iter) {
   FS_CHECK(iter >= 0 && iter <= p.broyden_maxiter, FS_ERR_ARG, "bad iter %d",
            iter);
   FS_CHECK(b != x, FS_ERR_ARG, "jsolve needs distinct b and x");
+  return jsolve_impl(b, x, iter, nullptr, 0);
+}
+
+// x = blockdiag(J^-1) b + sum_{i < ncol} C_i (D_i . b). With ncolp the
+// history length is *ncolp + noff, read on the device (grids sized for
+// broyden_maxiter); the launches honour d.act.
+int SystemImpl::jsolve_impl(const double *b, double *x, int ncol,
+                            const int *ncolp, int noff) {
+  const int *act = d.act;
   hipLaunchKernelGGL(k_jsolve_block, dim3(nf), dim3(256), 0, st, d, Jinv_cm(),
                      b, x);
   FS_LAUNCHED();
-  if (iter > 0) {
+  const int gy = ncolp ? p.broyden_maxiter : ncol;
+  if (gy > 0) {
     size_t n6 = 6 * (size_t)np;
-    hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, iter), dim3(256), 0, st,
-                       Dm.as<double>(), n6, iter, b, n6, dot_chunk,
-                       partial.as<double>());
-    hipLaunchKernelGGL(k_dots_final, dim3(iter), dim3(64), 0, st,
-                       partial.as<double>(), dot_blocks, iter,
+    hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, gy), dim3(256), 0, st,
+                       act, Dm.as<double>(), n6, ncol, ncolp, noff, gy, b, n6,
+                       dot_chunk, partial.as<double>());
+    hipLaunchKernelGGL(k_dots_final, dim3(gy), dim3(64), 0, st, act,
+                       partial.as<double>(), dot_blocks, gy, ncol, ncolp, noff,
                        coefb.as<double>());
     hipLaunchKernelGGL(k_hist_axpy, dim3(ceil_div((long)n6, 256)), dim3(256), 0,
-                       st, Cm.as<double>(), n6, iter, coefb.as<double>(), n6, x);
+                       st, act, Cm.as<double>(), n6, ncol, ncolp, noff,
+                       coefb.as<double>(), n6, x);
     FS_LAUNCHED();
   }
   return FS_OK;
@@ -1388,11 +1480,13 @@ int SystemImpl::broyden_update(const double *Eo, const double *En,
   hipLaunchKernelGGL(k_sub, dim3(ceil_div((long)n6, 256)), dim3(256), 0, st, En,
                      Eo, dE, n6);
   // ||dE||^2 through the same deterministic dot reduction
-  hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, 1), dim3(256), 0, st, dE,
-                     n6, 1, dE, n6, dot_chunk, partial.as<double>());
+  hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, 1), dim3(256), 0, st,
+                     (const int *)nullptr, dE, n6, 1, (const int *)nullptr, 0,
+                     1, dE, n6, dot_chunk, partial.as<double>());
   double *nrm2 = coefb.as<double>() + p.broyden_maxiter;
   hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st,
-                     partial.as<double>(), dot_blocks, 1, nrm2);
+                     (const int *)nullptr, partial.as<double>(), dot_blocks, 1,
+                     1, (const int *)nullptr, 0, nrm2);
   FS_LAUNCHED();
   FS_TRY(jsolve(dE, tmp, iter));
   hipLaunchKernelGGL(k_broyden_cols, dim3(ceil_div((long)n6, 256)), dim3(256), 0,
@@ -1405,7 +1499,7 @@ int SystemImpl::broyden_update(const double *Eo, const double *En,
 int SystemImpl::residual(int nt, const double *V, const double *W, double *err,
                          int *check, double *maxabs) {
   unsigned long long *mb = flags.as<unsigned long long>();
-  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, mb, 0ull);
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, d.act, mb, 0ull);
   hipLaunchKernelGGL(k_residual, dim3(ceil_div(np, 256)), dim3(256), 0, st, d,
                      nt, V, W, err, mb);
   FS_LAUNCHED();
@@ -1478,21 +1572,109 @@ int SystemImpl::broyden_loop(int nt, int *iters) {
     if (!check || iter >= p.broyden_maxiter) break;
     // dE = NewE - E; |dE|^2 and dE.NewE; g = H_{iter-1} NewE
     double *dots = coefb.as<double>() + p.broyden_maxiter;  // 2 scalars
-    hipLaunchKernelGGL(k_diff_dots, dim3(dot_blocks), dim3(256), 0, st, NewE,
-                       E, dE, n6, dot_chunk, partial.as<double>());
+    hipLaunchKernelGGL(k_diff_dots, dim3(dot_blocks), dim3(256), 0, st,
+                       (const int *)nullptr, NewE, E, dE, n6, dot_chunk,
+                       partial.as<double>());
     hipLaunchKernelGGL(k_dots_final, dim3(2), dim3(64), 0, st,
-                       partial.as<double>(), dot_blocks, 2, dots);
+                       (const int *)nullptr, partial.as<double>(), dot_blocks, 2,
+                       2, (const int *)nullptr, 0, dots);
     FS_LAUNCHED();
     FS_TRY(jsolve(NewE, tmp, iter - 1));
     hipLaunchKernelGGL(k_broyden_fused, dim3(ceil_div((long)n6, 256)),
-                       dim3(256), 0, st, tmp, dE, dots, n6,
-                       Cm.as<double>() + n6 * (iter - 1),
-                       Dm.as<double>() + n6 * (iter - 1), Upd);
+                       dim3(256), 0, st, (const int *)nullptr, tmp, dE, dots,
+                       n6, Cm.as<double>(), Dm.as<double>(), iter - 1,
+                       (const int *)nullptr, Upd);
     FS_LAUNCHED();
     std::swap(E, NewE);
   }
   if (iters) *iters = iter;
   return FS_OK;
+}
+
+// ---------------------------------------------------------------------
+// batched sweep: host-sync-free launch sequences of one timestep. Kernel
+// arguments are fixed at capture; the step number and the loop state are
+// read on the device (d.ntp, ctl), and every kernel of a Broyden iteration
+// returns at once when the suspension has converged (d.act, fcm.act,
+// cb.act). The arithmetic, launch order and reductions are those of
+// timestep(): a batched suspension reproduces its unbatched run bit for bit.
+// Requires initial_friction_steps == 0 (the FCM path on every step).
+// ---------------------------------------------------------------------
+void SystemImpl::set_device_control(const int *act, const int *ntp) {
+  d.act = act;
+  d.ntp = ntp;
+  fcm.act = act;
+  cb.act = act;
+}
+
+int SystemImpl::begin_step_dev(int *status) {
+  const int nt = 1;  // read from d.ntp on the device
+  const int *ntp = d.ntp;
+  set_device_control(nullptr, ntp);
+  FS_TRY(initial_guess());
+  FS_TRY(forces(nt));
+  FS_TRY(build_jacobian_dev(nt));
+  FS_TRY(barrier(nt, 1, nullptr));
+  FS_TRY(mobility(nt));
+  unsigned long long *mb = flags.as<unsigned long long>();
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, (const int *)nullptr,
+                     mb, 0ull);
+  hipLaunchKernelGGL(k_residual, dim3(ceil_div(np, 256)), dim3(256), 0, st, d,
+                     nt, d.V, d.W, E, mb);
+  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(1), 0, st, ctl.as<int>(), mb,
+                     p.tol, p.broyden_maxiter);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, flags.as<int>() + 4, 1,
+                     ctl.as<int>() + 2);
+  FS_LAUNCHED();
+  set_device_control(ctl.as<int>() + 1, ntp);
+  FS_TRY(jsolve_impl(E, Upd, 0, nullptr, 0));
+  set_device_control(nullptr, ntp);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, ctl.as<int>(), 3,
+                     status);
+  FS_LAUNCHED();
+  return FS_OK;
+}
+
+int SystemImpl::iteration_dev(int *status) {
+  const int nt = 1;  // read from d.ntp on the device
+  const int *ntp = d.ntp;
+  const int *act = ctl.as<int>() + 1;
+  const size_t n6 = 6 * (size_t)np;
+  set_device_control(act, ntp);
+  FS_TRY(update(Upd));
+  FS_TRY(forces(nt));
+  FS_TRY(barrier(nt, 0, nullptr));
+  FS_TRY(mobility(nt));
+  unsigned long long *mb = flags.as<unsigned long long>();
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, act, mb, 0ull);
+  hipLaunchKernelGGL(k_residual, dim3(ceil_div(np, 256)), dim3(256), 0, st, d,
+                     nt, d.V, d.W, NewE, mb);
+  hipLaunchKernelGGL(k_ctl_after, dim3(1), dim3(1), 0, st, ctl.as<int>(), mb,
+                     p.tol, p.broyden_maxiter);
+  double *dots = coefb.as<double>() + p.broyden_maxiter;  // 2 scalars
+  hipLaunchKernelGGL(k_diff_dots, dim3(dot_blocks), dim3(256), 0, st, act, NewE,
+                     E, dE, n6, dot_chunk, partial.as<double>());
+  hipLaunchKernelGGL(k_dots_final, dim3(2), dim3(64), 0, st, act,
+                     partial.as<double>(), dot_blocks, 2, 2,
+                     (const int *)nullptr, 0, dots);
+  FS_LAUNCHED();
+  FS_TRY(jsolve_impl(NewE, tmp, 0, ctl.as<int>(), -1));
+  hipLaunchKernelGGL(k_broyden_fused, dim3(ceil_div((long)n6, 256)), dim3(256),
+                     0, st, act, tmp, dE, dots, n6, Cm.as<double>(),
+                     Dm.as<double>(), 0, ctl.as<int>(), Upd);
+  hipLaunchKernelGGL(k_copy_f64, dim3(ceil_div((long)n6, 256)), dim3(256), 0, st,
+                     act, NewE, E, n6);
+  set_device_control(nullptr, ntp);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, ctl.as<int>(), 3,
+                     status);
+  FS_LAUNCHED();
+  return FS_OK;
+}
+
+int SystemImpl::end_step_dev(int *status) {
+  (void)status;
+  set_device_control(nullptr, d.ntp);
+  return step(1);  // step number read from d.ntp
 }
 
 }  // namespace fs

@@ -135,6 +135,11 @@ extern "C" int fs_system_set_warmstart(fs_system *h, fs_mlp *m) {
   return h->ws.attach(h->impl, fs::mlp_impl(m));
 }
 
+extern "C" int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.fcm.autotune(h->impl.np, reps, best);
+}
+
 extern "C" int fs_system_warmstart_dims(fs_system *h, int *n_in, int *n_out) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
   for (int f = 1; f < h->impl.nf; ++f)
@@ -373,5 +378,153 @@ extern "C" int fs_make_filaments(const double box[3], int nfil, const int *nw,
     }
     g0 += n;
   }
+  return FS_OK;
+}
+
+// ---------------------------------------------------------------------
+// Batched sweep: many independent suspensions advanced together. No
+// reference counterpart (the reference runs one process per sweep point,
+// batch_run.py:60-229). Each suspension's timestep is captured once as
+// three graphs (begin, one Broyden iteration, end) with its loop state on
+// the device; a parent graph per phase holds every suspension as an
+// independent child node. A timestep launches begin, then the iteration
+// graph until no suspension is still iterating (one status read per
+// launch, not one per suspension and iteration), then end.
+// ---------------------------------------------------------------------
+struct fs_batch {
+  std::vector<fs_system *> sys;
+  hipStream_t st = nullptr;
+  hipGraph_t g[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t ge[3] = {nullptr, nullptr, nullptr};
+  int *dnt = nullptr, *dstat = nullptr;  // device: step number, 3 ints each
+  int *hnt = nullptr, *hstat = nullptr;  // pinned host mirrors
+  int maxiter = 0;
+};
+
+namespace {
+
+void batch_release(fs_batch *b) {
+  for (int k = 0; k < 3; ++k) {
+    if (b->ge[k]) (void)hipGraphExecDestroy(b->ge[k]);
+    if (b->g[k]) (void)hipGraphDestroy(b->g[k]);
+  }
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  if (b->dnt) (void)hipFree(b->dnt);
+  if (b->dstat) (void)hipFree(b->dstat);
+  if (b->hnt) (void)hipHostFree(b->hnt);
+  if (b->hstat) (void)hipHostFree(b->hstat);
+  for (fs_system *h : b->sys) h->impl.set_device_control(nullptr, nullptr);
+  delete b;
+}
+
+int batch_capture(fs_batch *b, int kind) {
+  FS_HIP(hipGraphCreate(&b->g[kind], 0));
+  for (size_t i = 0; i < b->sys.size(); ++i) {
+    SystemImpl &S = b->sys[i]->impl;
+    FS_HIP(hipStreamSynchronize(S.st));
+    S.set_device_control(nullptr, b->dnt);
+    FS_HIP(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal));
+    int *status = b->dstat + 3 * i;
+    int rc = kind == 0   ? S.begin_step_dev(status)
+             : kind == 1 ? S.iteration_dev(status)
+                         : S.end_step_dev(status);
+    hipGraph_t child = nullptr;
+    hipError_t e = hipStreamEndCapture(S.st, &child);
+    S.set_device_control(nullptr, nullptr);
+    if (rc != FS_OK) {
+      if (child) (void)hipGraphDestroy(child);
+      return rc;
+    }
+    FS_CHECK(e == hipSuccess && child, FS_ERR_HIP, "graph capture failed: %s",
+             hipGetErrorString(e));
+    hipGraphNode_t node;
+    hipError_t e2 = hipGraphAddChildGraphNode(&node, b->g[kind], nullptr, 0, child);
+    (void)hipGraphDestroy(child);
+    FS_CHECK(e2 == hipSuccess, FS_ERR_HIP, "adding a child graph failed: %s",
+             hipGetErrorString(e2));
+  }
+  FS_HIP(hipGraphInstantiate(&b->ge[kind], b->g[kind], nullptr, nullptr, 0));
+  return FS_OK;
+}
+
+int batch_status(fs_batch *b) {
+  FS_HIP(hipMemcpyAsync(b->hstat, b->dstat, 3 * sizeof(int) * b->sys.size(),
+                        hipMemcpyDeviceToHost, b->st));
+  FS_HIP(hipStreamSynchronize(b->st));
+  return FS_OK;
+}
+
+}  // namespace
+
+extern "C" int fs_batch_create(fs_system **sys, int n, fs_batch **out) {
+  FS_CHECK(sys && out && n >= 1, FS_ERR_ARG, "bad argument");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    FS_CHECK(sys[i], FS_ERR_ARG, "null system %d", i);
+    FS_CHECK(!sys[i]->ws.attached(), FS_ERR_STATE,
+             "system %d has a warm start attached or recording", i);
+    FS_CHECK(sys[i]->impl.p.initial_friction_steps == 0, FS_ERR_ARG,
+             "system %d: batched sweeps need initial_friction_steps == 0", i);
+    for (int j = 0; j < i; ++j)
+      FS_CHECK(sys[j] != sys[i], FS_ERR_ARG, "system %d listed twice", i);
+  }
+  fs_batch *b = new fs_batch;
+  b->sys.assign(sys, sys + n);
+  int rc = FS_OK;
+  auto fail = [&](int r) {
+    batch_release(b);
+    return r;
+  };
+  if (hipStreamCreate(&b->st) != hipSuccess ||
+      hipMalloc((void **)&b->dnt, sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&b->dstat, 3 * sizeof(int) * n) != hipSuccess ||
+      hipHostMalloc((void **)&b->hnt, sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&b->hstat, 3 * sizeof(int) * n,
+                    hipHostMallocDefault) != hipSuccess) {
+    fs::set_error("batch allocation failed");
+    return fail(FS_ERR_HIP);
+  }
+  for (int i = 0; i < n; ++i)
+    b->maxiter = std::max(b->maxiter, sys[i]->impl.p.broyden_maxiter);
+  for (int k = 0; k < 3 && rc == FS_OK; ++k) rc = batch_capture(b, k);
+  if (rc != FS_OK) return fail(rc);
+  *out = b;
+  return FS_OK;
+}
+
+extern "C" int fs_batch_destroy(fs_batch *b) {
+  if (b) batch_release(b);
+  return FS_OK;
+}
+
+extern "C" int fs_batch_timestep(fs_batch *b, int nt, int *iters) {
+  FS_CHECK(b, FS_ERR_ARG, "null batch");
+  const int n = (int)b->sys.size();
+  *b->hnt = nt;
+  FS_HIP(hipMemcpyAsync(b->dnt, b->hnt, sizeof(int), hipMemcpyHostToDevice,
+                        b->st));
+  FS_HIP(hipGraphLaunch(b->ge[0], b->st));
+  FS_TRY(batch_status(b));
+  for (int i = 0; i < n; ++i)
+    FS_CHECK(b->hstat[3 * i + 2] == 0, FS_ERR_ARG,
+             "system %d: approximate Jacobian is singular", i);
+  for (int it = 0; it < b->maxiter; ++it) {
+    bool any = false;
+    for (int i = 0; i < n; ++i) any = any || b->hstat[3 * i + 1] != 0;
+    if (!any) break;
+    FS_HIP(hipGraphLaunch(b->ge[1], b->st));
+    FS_TRY(batch_status(b));
+  }
+  FS_HIP(hipGraphLaunch(b->ge[2], b->st));
+  FS_HIP(hipStreamSynchronize(b->st));
+  if (iters)
+    for (int i = 0; i < n; ++i) iters[i] = b->hstat[3 * i];
+  return FS_OK;
+}
+
+extern "C" int fs_batch_size(fs_batch *b, int *n) {
+  FS_CHECK(b && n, FS_ERR_ARG, "bad argument");
+  *n = (int)b->sys.size();
   return FS_OK;
 }

@@ -23,6 +23,9 @@ struct SysDev {
   double dL, dt, mu, Lx, Ly, Lz, st0;
   double weight[3];
   int impl_euler_steps, sperm_decay;
+  // batched sweep only (nullptr otherwise): kernels of a Broyden iteration
+  // return at once while *act == 0; *ntp replaces the step number argument
+  const int *act, *ntp;
 };
 
 struct SystemImpl {
@@ -62,6 +65,18 @@ struct SystemImpl {
   int begin_step(int nt);
   int broyden_loop(int nt, int *iters);
   int check0 = 0;
+
+  // --- batched sweep: the timestep as three host-sync-free launch
+  // sequences, captured once into graphs and replayed. ctl = {iterations,
+  // still iterating, singular Jacobian}; ntp = device step number.
+  dbuf ctl;
+  int jsolve_impl(const double *b, double *x, int ncol, const int *ncolp,
+                  int noff);
+  int build_jacobian_dev(int nt);
+  void set_device_control(const int *act, const int *ntp);
+  int begin_step_dev(int *status);  // status: where to publish ctl
+  int iteration_dev(int *status);
+  int end_step_dev(int *status);
 };
 
 }  // namespace fs

@@ -106,3 +106,58 @@ def swimming_props(nfil, nw, dL=2.2, amplitude_times_L=0.5, period=1.0,
     return fil_props(nfil, kap=kap, T0=amplitude_times_L / L,
                      omega=2 * PI / period, wavenumber=2 * PI / L, phase=ph,
                      alpha=0.0, beta=1.0)
+
+
+SWEEP_GRIDS = (32, 48, 64, 96, 128)
+
+
+def sweep_point(seed, grids=SWEEP_GRIDS, nfil_range=(8, 200),
+                nw_range=(15, 40), kb_range=(0.1, 10.0), g_sigma=0.2,
+                precision=1, tol=1e-4, dt=0.01, maxiter=100,
+                max_volume_fraction=0.05, max_length_fraction=0.9):
+    """One suspension of a batched parameter sweep: grid drawn from `grids`,
+    filament count uniform in nfil_range with segment counts uniform in
+    nw_range, bending/twist modulus log-uniform in kb_range, gravity per
+    unit length ~ N(0, g_sigma^2) along z, barrier on, no initial friction
+    steps. Draws that cannot fit the box are capped and the caps returned:
+    segments per filament so a filament spans at most max_length_fraction
+    of the box, filament count so segments (spheres of radius a) fill at
+    most max_volume_fraction of it, and fewer filaments (10% at a time)
+    when the overlap-free placement still fails. Returns (fp, sp, nws,
+    props, state, info)."""
+    import numpy as np
+    from . import api
+    rng = np.random.default_rng(seed)
+    grid = int(rng.choice(grids))
+    fp = fcm_params(grid)
+    L = min(box(fp))
+    nfil = int(rng.integers(nfil_range[0], nfil_range[1] + 1))
+    nws = rng.integers(nw_range[0], nw_range[1] + 1, nfil)
+    nw_cap = max(2, int(max_length_fraction * L / 2.2))
+    nws = np.minimum(nws, nw_cap)
+    seg_vol = 4.0 / 3.0 * PI * fp.a ** 3
+    seg_cap = int(max_volume_fraction * float(np.prod(box(fp))) / seg_vol)
+    while nfil > 1 and int(nws[:nfil].sum()) > seg_cap:
+        nfil -= 1
+    nws = [int(n) for n in nws[:nfil]]
+    kb = float(np.exp(rng.uniform(np.log(kb_range[0]), np.log(kb_range[1]))))
+    g = float(rng.normal(0.0, g_sigma))
+    sp = sim_params(fp, dt=dt, tol=tol, maxiter=maxiter, precision=precision,
+                    weight=(0.0, 0.0, g), barrier_on=1,
+                    initial_friction_steps=0)
+    place_seed = int(rng.integers(1, 2 ** 31))
+    while True:
+        try:
+            X, q = api.make_filaments(box(fp), nws, dL=2.2, min_sep=2.2,
+                                      seed=place_seed)
+            break
+        except abi.FsError:
+            if nfil <= 1:
+                raise
+            nfil = max(1, int(nfil * 0.9))
+            nws = nws[:nfil]
+    props = fil_props(nfil, kap=kb)
+    state = dict(X=X, Xs=X, Xt=X, q=q, qs=q, qt=q)
+    info = dict(grid=grid, nfil=nfil, segments=int(sum(nws)), kb=kb,
+                gravity=g, nw_cap=nw_cap, segment_cap=seg_cap)
+    return fp, sp, nws, props, state, info

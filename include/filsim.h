@@ -87,6 +87,13 @@ typedef struct fs_fcm_tuning {
 } fs_fcm_tuning;
 int fs_fcm_get_tuning(fs_fcm *h, fs_fcm_tuning *t);
 int fs_fcm_set_tuning(fs_fcm *h, const fs_fcm_tuning *t);
+/* Autotuner (no reference counterpart; the reference has no launch
+ * shapes): times `reps` mobility solves of np synthetic segments (uniform
+ * in the box, forces/torques uniform in [-1, 1]) for each candidate of each
+ * tunable in turn (spread_tile, spread_zrun, fft_batch, fft_xlines), keeps
+ * the fastest, leaves it set and returns it in *best (may be NULL). Results
+ * of every candidate agree to rounding (the FP64 path to 1e-12). */
+int fs_fcm_autotune(fs_fcm *h, int np, int reps, fs_fcm_tuning *best);
 
 /* ------------------------------------------------------------------ */
 /* Collision barrier (replaces struct LinkedList,                      */
@@ -275,6 +282,24 @@ int fs_system_ws_count(fs_system *h, int *count);
  * infinity norms of the residuals of both candidate guesses. */
 int fs_system_last_step_info(fs_system *h, int *used_mlp, double *res_plain,
                              double *res_mlp);
+/* Autotune the system's FCM solver for its segment count (fs_fcm_autotune). */
+int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best);
+
+/* ---- Batched sweep (no reference counterpart: the reference runs one
+ * process per sweep point, batch_run.py:60-229). Independent systems of any
+ * grid and filament counts advance together: each system's timestep is
+ * captured once as hipGraphs (begin, one Broyden iteration, end) with the
+ * loop state on the device, and one parent graph per phase launches every
+ * system. A batched system reproduces its unbatched fs_system_timestep bit
+ * for bit. Systems need initial_friction_steps == 0 and no warm start; they
+ * must outlive the batch and must not be stepped on their own meanwhile. */
+typedef struct fs_batch fs_batch;
+int fs_batch_create(fs_system **systems, int n, fs_batch **out);
+int fs_batch_destroy(fs_batch *b);
+int fs_batch_size(fs_batch *b, int *n);
+/* One timestep of every system; iters[i] = Broyden iterations of system i
+ * (may be NULL). */
+int fs_batch_timestep(fs_batch *b, int nt, int *iters);
 
 #ifdef __cplusplus
 }
