@@ -136,6 +136,94 @@ def cpu_baseline(fp_iters):
     return t_solve, 1.0 / (t_solve * solves_per_step), solves_per_step
 
 
+def batch_bench(args, torch, n=64, seed=4000):
+    """Batched sweep: n suspensions (grid from {32..128}^3, 8-200
+    filaments of 15-40 segments capped by box size, KB log-uniform in
+    [0.1, 10], gravity N(0, 0.2^2)), FP32 FCM + FP64 Broyden, TOL 1e-4,
+    dt 0.01, each suspension's FCM autotuned. Timed: the batch (captured
+    graphs) against the same suspensions stepped one after another."""
+    from filsim_amd import abi, api, params
+    pts = [params.sweep_point(seed + k, precision=abi.FS_FP32, tol=1e-6)
+           for k in range(n)]
+    groups = {}
+    t0 = time.perf_counter()
+    for name in ("batched", "sequential"):
+        g = []
+        for fp, sp, nws, props, st, info in pts:
+            s = api.System(sp, nws, props)
+            s.set_state(**st)
+            g.append(s)
+        groups[name] = g
+    tunes = [s.autotune(reps=2) for s in groups["batched"]]
+    # the sequential group gets the same launch shapes
+    for s, t in zip(groups["sequential"], tunes):
+        s.set_tuning(t)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch = api.Batch(groups["batched"])
+    t_capture = time.perf_counter() - t0
+    res = {}
+    steps = max(2, args.steps // 2)
+    lane_modes = (0, 4, 8)
+    nts = {"batched": 1, "sequential": 1}
+    for name in ["batched_lanes%d" % k for k in lane_modes] + ["sequential"]:
+        if name.startswith("batched"):
+            batch.set_lanes(int(name[len("batched_lanes"):]))
+        nt = nts["batched" if name.startswith("batched") else name]
+        its_all = []
+        grp = groups["batched" if name.startswith("batched") else name]
+        for _ in range(args.warmup):
+            if name.startswith("batched"):
+                batch.timestep(nt)
+            else:
+                [s.timestep(nt) for s in grp]
+            nt += 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if name.startswith("batched"):
+                its_all.append(batch.timestep(nt))
+            else:
+                its_all.append([s.timestep(nt) for s in grp])
+            nt += 1
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        nts["batched" if name.startswith("batched") else name] = nt
+        res[name] = {"suspension_steps_per_s": n * steps / el,
+                     "s_per_batch_step": el / steps,
+                     "mean_iters": float(np.mean(its_all))}
+        log(name, res[name])
+    # the batched group ran len(lane_modes) times as many steps: replay the
+    # sequential group to the same step count before comparing
+    nt = nts["sequential"]
+    while nt < nts["batched"]:
+        [s.timestep(nt) for s in groups["sequential"]]
+        nt += 1
+    same = all(np.array_equal(a.get_state()["X"], b.get_state()["X"])
+               for a, b in zip(groups["batched"], groups["sequential"]))
+    batch.close()
+    best = max((k for k in res if k.startswith("batched")),
+               key=lambda k: res[k]["suspension_steps_per_s"])
+    res["batched"] = dict(res[best], mode=best)
+    info = [p[5] for p in pts]
+    return {"metric": "batched sweep: suspension-timesteps/s, %d suspensions"
+                      % n,
+            "value": res["batched"]["suspension_steps_per_s"],
+            "unit": "suspension-timesteps/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "higher_is_better": True,
+            "dtype": "f32 FCM + f64 Broyden", "data": "synthetic",
+            "config": {"workload": "%d suspensions, grids %s, %d segments "
+                                   "total, TOL 1e-6, dt 0.01" %
+                                   (n, sorted(set(i["grid"] for i in info)),
+                                    sum(i["segments"] for i in info))},
+            "batched": res["batched"], "sequential": res["sequential"],
+            "modes": {k: v for k, v in res.items() if k.startswith("batched_")},
+            "speedup": res["batched"]["suspension_steps_per_s"] /
+                       res["sequential"]["suspension_steps_per_s"],
+            "bitwise_equal_final_positions": bool(same),
+            "setup_s": t_setup, "capture_s": t_capture}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,7 +237,7 @@ def main():
     ap.add_argument("--skip", type=int, default=10,
                     help="start-up steps run before recording")
     ap.add_argument("--only", choices=["all", "mixed", "fp64", "mobility",
-                                       "copy"],
+                                       "copy", "batch"],
                     default="all")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -164,6 +252,9 @@ def main():
         sys.exit(2)
     from filsim_amd import abi
     out = {}
+    if args.only == "batch":
+        print(json.dumps(batch_bench(args, torch)), flush=True)
+        return
     if args.only == "copy":
         # calibration for HBM byte counters: a plain 1 GiB FP32 scale-by-one
         # copy (reads and writes exactly 2^30 bytes per dispatch)

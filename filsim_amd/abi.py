@@ -125,10 +125,12 @@ PROTOS = {
     "fs_system_ws_count": [P, IP],
     "fs_system_last_step_info": [P, IP, _DP, _DP],
     "fs_system_autotune": [P, C.c_int, C.POINTER(FcmTuning)],
+    "fs_system_set_tuning": [P, C.POINTER(FcmTuning)],
     "fs_batch_create": [C.POINTER(P), C.c_int, C.POINTER(P)],
     "fs_batch_destroy": [P],
     "fs_batch_size": [P, IP],
     "fs_batch_timestep": [P, C.c_int, IP],
+    "fs_batch_set_lanes": [P, C.c_int],
 }
 
 _lib = None

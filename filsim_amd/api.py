@@ -279,6 +279,10 @@ class System:
         abi.check(self.lib.fs_system_autotune(self.h, reps, C.byref(t)))
         return t
 
+    def set_tuning(self, t):
+        """Launch shapes of the system's FCM solver (fs_system_set_tuning)."""
+        abi.check(self.lib.fs_system_set_tuning(self.h, C.byref(t)))
+
 
 class Batch:
     """Independent systems advanced together through captured hipGraphs
@@ -304,6 +308,10 @@ class Batch:
         it = (C.c_int * len(self.systems))()
         abi.check(self.lib.fs_batch_timestep(self.h, nt, it))
         return list(it)
+
+    def set_lanes(self, lanes):
+        """0: one parent graph per phase; k: per-system graphs on k streams."""
+        abi.check(self.lib.fs_batch_set_lanes(self.h, lanes))
 
 
 def make_filaments(box, nw, dL=2.2, min_sep=2.2, seed=1, max_tries=100000):

@@ -284,6 +284,8 @@ int fs_system_last_step_info(fs_system *h, int *used_mlp, double *res_plain,
                              double *res_mlp);
 /* Autotune the system's FCM solver for its segment count (fs_fcm_autotune). */
 int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best);
+/* Set the launch shapes of the system's FCM solver (fs_fcm_set_tuning). */
+int fs_system_set_tuning(fs_system *h, const fs_fcm_tuning *t);
 
 /* ---- Batched sweep (no reference counterpart: the reference runs one
  * process per sweep point, batch_run.py:60-229). Independent systems of any
@@ -300,6 +302,11 @@ int fs_batch_size(fs_batch *b, int *n);
 /* One timestep of every system; iters[i] = Broyden iterations of system i
  * (may be NULL). */
 int fs_batch_timestep(fs_batch *b, int nt, int *iters);
+/* Launch mode: lanes in [1, 16] launches each system's own graphs
+ * round-robin on that many streams so independent systems overlap on the
+ * hardware queues (default min(n, 4)); 0 launches one parent graph per
+ * phase. Results do not depend on the mode. */
+int fs_batch_set_lanes(fs_batch *b, int lanes);
 
 #ifdef __cplusplus
 }

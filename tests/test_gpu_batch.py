@@ -38,6 +38,7 @@ def test_batched_equals_unbatched(cuda, precision):
             group.append(s)
     batch = api.Batch(bat)
     for nt in range(1, 5):
+        batch.set_lanes(0 if nt % 2 else 3)  # parent graph / 3 streams
         ib = batch.timestep(nt)
         isolo = [s.timestep(nt) for s in solo]
         print("step", nt, "batched iters", ib, "solo", isolo)
