@@ -224,6 +224,89 @@ def batch_bench(args, torch, n=64, seed=4000):
             "setup_s": t_setup, "capture_s": t_capture}
 
 
+def c5_bench(args, torch):
+    """Learned warm start at scale: 8,000 swimming filaments x 30 segments
+    (240,000) on a 384^3 grid (dx = a/1.86, ngd = 12), random phases,
+    KB = 1000, dt = 0.005, TOL 1e-6, barrier on, FP32 FCM + FP64 Broyden;
+    filaments on a jittered lattice (random placement saturates at this
+    density). The network is trained on 56,000 filament-steps (7 recorded
+    steps after 3 start-up steps) of the plain mixed path, then mixed +
+    learned and plain mixed are timed from the same state."""
+    from filsim_amd import abi, api, params, warmstart
+    fp = params.fcm_params(384)
+    nfil, nw = 8000, 30
+    rng = np.random.default_rng(9)
+    phases = list(rng.uniform(0, 2 * math.pi, nfil))
+    X0, q0 = params.lattice_filaments(params.box(fp), nfil, nw, seed=9)
+
+    def build():
+        sp = params.sim_params(fp, dt=0.005, tol=1e-6, maxiter=100,
+                               precision=abi.FS_FP32, weight=(0.0, 0.0, 0.0),
+                               barrier_on=1)
+        props = params.swimming_props(nfil, nw, kap=1000.0, phases=phases)
+        s = api.System(sp, [nw] * nfil, props)
+        return s
+
+    log("c5: building 8000 x 30 segments at 384^3")
+    s_l = build()
+    s_l.set_state(X=X0, Xs=X0, Xt=X0, q=q0, qs=q0, qt=q0)
+    log("c5: built")
+    t0 = time.perf_counter()
+    X, T, its_rec = warmstart.record(s_l, 7, nt0=1, torch=torch, skip=3,
+                                     log=log)
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mlp, hist = warmstart.train(X, T, torch, epochs=args.epochs, batch=1024,
+                                lr=1e-3, log=log)
+    t_train = time.perf_counter() - t0
+    log("c5: recorded %d samples in %.1fs, trained in %.1fs, holdout %.3f" %
+        (X.shape[0], t_rec, t_train, hist[-1][2]))
+    del X, T
+    state = s_l.get_state()
+    keep = {k: state[k] for k in ("X", "Xt", "U", "Ut", "q", "qt", "lam",
+                                  "lam1", "lam2")}
+    keep.update(Xs=state["X"], Us=state["U"], qs=state["q"])
+    api.system_set_warmstart(s_l, mlp)
+    nt0 = 11
+    res = {}
+    for name in ("learned", "mixed"):
+        if name == "learned":
+            s = s_l
+        else:
+            s = build()
+            s.set_state(**keep)
+        nt = nt0
+        for _ in range(args.warmup):
+            s.timestep(nt)
+            nt += 1
+        torch.cuda.synchronize()
+        its = []
+        steps = max(3, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            its.append(s.timestep(nt))
+            log("c5 %s step %d: %d iterations" % (name, nt, its[-1]))
+            nt += 1
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[name] = {"timesteps_per_s": steps / el, "iters": its,
+                     "mean_iters": float(np.mean(its))}
+        log("c5", name, res[name])
+        s.close()
+    mlp.close()
+    return {"metric": "timesteps/s at 384^3, 240k segs: mixed+learned vs "
+                      "mixed", "value": res["learned"]["timesteps_per_s"],
+            "unit": "timesteps/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32 FCM + f64 Broyden + bf16 MLP", "data": "synthetic",
+            "config": {"workload": "8000 swimming filaments x 30 segments, "
+                                   "384^3, KB=1000, dt=0.005, TOL=1e-6"},
+            "learned": res["learned"], "mixed": res["mixed"],
+            "speedup_vs_mixed": res["learned"]["timesteps_per_s"] /
+                                res["mixed"]["timesteps_per_s"],
+            "train": {"samples": 56000, "record_s": t_rec,
+                      "train_s": t_train, "holdout": hist[-1][2]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,7 +320,7 @@ def main():
     ap.add_argument("--skip", type=int, default=10,
                     help="start-up steps run before recording")
     ap.add_argument("--only", choices=["all", "mixed", "fp64", "mobility",
-                                       "copy", "batch"],
+                                       "copy", "batch", "c5"],
                     default="all")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -254,6 +337,9 @@ def main():
     out = {}
     if args.only == "batch":
         print(json.dumps(batch_bench(args, torch)), flush=True)
+        return
+    if args.only == "c5":
+        print(json.dumps(c5_bench(args, torch)), flush=True)
         return
     if args.only == "copy":
         # calibration for HBM byte counters: a plain 1 GiB FP32 scale-by-one

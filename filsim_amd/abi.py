@@ -126,6 +126,22 @@ PROTOS = {
     "fs_system_last_step_info": [P, IP, _DP, _DP],
     "fs_system_autotune": [P, C.c_int, C.POINTER(FcmTuning)],
     "fs_system_set_tuning": [P, C.POINTER(FcmTuning)],
+    "fs_io_write_dat": [C.c_char_p, C.c_int, C.c_int, IP, _DP, _DP],
+    "fs_io_write_bak": [C.c_char_p, C.c_int, C.c_int, IP, C.POINTER(StatePtrs)],
+    "fs_io_read_bak": [C.c_char_p, C.c_int, IP, C.POINTER(StatePtrs), IP],
+    "fs_io_write_par": [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, IP, _DP,
+                        C.POINTER(SimParams), C.c_double, C.c_double,
+                        C.c_int, C.c_int],
+    "fs_io_read_par": [C.c_char_p, IP, IP, C.c_int],
+    "fs_io_write_checkpoint": [C.c_char_p, C.c_int, C.c_int, IP,
+                               C.POINTER(StatePtrs)],
+    "fs_io_read_checkpoint": [C.c_char_p, C.c_int, IP, C.POINTER(StatePtrs),
+                              IP],
+    "fs_system_write_dat": [P, C.c_char_p, C.c_int],
+    "fs_system_write_bak": [P, C.c_char_p, C.c_int],
+    "fs_system_read_bak": [P, C.c_char_p, IP],
+    "fs_system_write_checkpoint": [P, C.c_char_p, C.c_int],
+    "fs_system_read_checkpoint": [P, C.c_char_p, IP],
     "fs_batch_create": [C.POINTER(P), C.c_int, C.POINTER(P)],
     "fs_batch_destroy": [P],
     "fs_batch_size": [P, IP],

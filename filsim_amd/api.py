@@ -428,3 +428,119 @@ def system_last_step_info(system):
     abi.check(system.lib.fs_system_last_step_info(system.h, C.byref(u),
                                                   C.byref(a), C.byref(b)))
     return u.value, a.value, b.value
+
+
+# ---- trajectory files (fs_io_*): state dicts of host arrays in the
+# get_state layout
+_IO_FIELDS = ("X", "Xs", "Xt", "U", "Us", "Ut", "q", "qs", "qt", "lam",
+              "lam1", "lam2")
+
+
+def _state_ptrs(state, nws, fill=False):
+    npt = int(sum(nws))
+    nlam = npt - len(nws)
+    s = abi.StatePtrs()
+    keep = {}
+    for k in _IO_FIELDS:
+        rows = nlam if k.startswith("lam") else npt
+        w = _width(k)
+        if fill:
+            a = np.zeros((max(rows, 1), w))
+        else:
+            a = _f64(state[k]).reshape(rows, w)
+        keep[k] = a
+        setattr(s, k, abi.dptr(a))
+    return s, keep
+
+
+def io_write_bak(path, nt, nws, state):
+    lib = abi.load()
+    nw = np.ascontiguousarray(nws, dtype=np.int32)
+    s, _keep = _state_ptrs(state, nws)
+    abi.check(lib.fs_io_write_bak(path.encode(), nt, len(nw), abi.iptr(nw),
+                                  C.byref(s)))
+
+
+def io_read_bak(path, nws):
+    lib = abi.load()
+    nw = np.ascontiguousarray(nws, dtype=np.int32)
+    s, keep = _state_ptrs(None, nws, fill=True)
+    nt = C.c_int()
+    abi.check(lib.fs_io_read_bak(path.encode(), len(nw), abi.iptr(nw),
+                                 C.byref(s), C.byref(nt)))
+    return nt.value, keep
+
+
+def io_write_dat(path, nt, nws, X, q):
+    lib = abi.load()
+    nw = np.ascontiguousarray(nws, dtype=np.int32)
+    X, q = _f64(X), _f64(q)
+    abi.check(lib.fs_io_write_dat(path.encode(), nt, len(nw), abi.iptr(nw),
+                                  abi.dptr(X), abi.dptr(q)))
+
+
+def io_write_par(path, nws, sp, kap=1.0, c=1.0, time_steps=0, plot_steps=1,
+                 bending_factors=None, name="filsim", tag="run"):
+    lib = abi.load()
+    nw = np.ascontiguousarray(nws, dtype=np.int32)
+    bf = _f64(bending_factors if bending_factors is not None
+              else np.ones(len(nw)))
+    abi.check(lib.fs_io_write_par(path.encode(), name.encode(), tag.encode(),
+                                  len(nw), abi.iptr(nw), abi.dptr(bf),
+                                  C.byref(sp), kap, c, time_steps,
+                                  plot_steps))
+
+
+def io_read_par(path):
+    lib = abi.load()
+    nf = C.c_int()
+    abi.check(lib.fs_io_read_par(path.encode(), C.byref(nf), None, 0))
+    nw = np.zeros(nf.value, dtype=np.int32)
+    abi.check(lib.fs_io_read_par(path.encode(), C.byref(nf), abi.iptr(nw),
+                                 len(nw)))
+    return [int(v) for v in nw]
+
+
+def io_write_checkpoint(path, nt, nws, state):
+    lib = abi.load()
+    nw = np.ascontiguousarray(nws, dtype=np.int32)
+    s, _keep = _state_ptrs(state, nws)
+    abi.check(lib.fs_io_write_checkpoint(path.encode(), nt, len(nw),
+                                         abi.iptr(nw), C.byref(s)))
+
+
+def io_read_checkpoint(path, nws):
+    lib = abi.load()
+    nw = np.ascontiguousarray(nws, dtype=np.int32)
+    s, keep = _state_ptrs(None, nws, fill=True)
+    nt = C.c_int()
+    abi.check(lib.fs_io_read_checkpoint(path.encode(), len(nw), abi.iptr(nw),
+                                        C.byref(s), C.byref(nt)))
+    return nt.value, keep
+
+
+def system_write_dat(system, path, nt):
+    abi.check(system.lib.fs_system_write_dat(system.h, path.encode(), nt))
+
+
+def system_write_bak(system, path, nt):
+    abi.check(system.lib.fs_system_write_bak(system.h, path.encode(), nt))
+
+
+def system_read_bak(system, path):
+    nt = C.c_int()
+    abi.check(system.lib.fs_system_read_bak(system.h, path.encode(),
+                                            C.byref(nt)))
+    return nt.value
+
+
+def system_write_checkpoint(system, path, nt):
+    abi.check(system.lib.fs_system_write_checkpoint(system.h, path.encode(),
+                                                    nt))
+
+
+def system_read_checkpoint(system, path):
+    nt = C.c_int()
+    abi.check(system.lib.fs_system_read_checkpoint(system.h, path.encode(),
+                                                   C.byref(nt)))
+    return nt.value

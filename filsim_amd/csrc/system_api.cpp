@@ -140,6 +140,73 @@ extern "C" int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best) {
   return h->impl.fcm.autotune(h->impl.np, reps, best);
 }
 
+// ---- trajectory files of a system (io.cpp holds the formats)
+namespace {
+
+struct HostState {
+  std::vector<double> a3[6], a4[3], al[3];
+  fs_state_ptrs p{};
+  explicit HostState(const SystemImpl &S) {
+    for (auto &v : a3) v.resize(3 * (size_t)S.np);
+    for (auto &v : a4) v.resize(4 * (size_t)S.np);
+    for (auto &v : al) v.resize(3 * (size_t)std::max(S.nlam, 1));
+    p.X = a3[0].data();
+    p.Xs = a3[1].data();
+    p.Xt = a3[2].data();
+    p.U = a3[3].data();
+    p.Us = a3[4].data();
+    p.Ut = a3[5].data();
+    p.q = a4[0].data();
+    p.qs = a4[1].data();
+    p.qt = a4[2].data();
+    p.lam = al[0].data();
+    p.lam1 = al[1].data();
+    p.lam2 = al[2].data();
+  }
+};
+
+}  // namespace
+
+extern "C" int fs_system_write_dat(fs_system *h, const char *path, int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  HostState hs(h->impl);
+  FS_TRY(fs_system_get_state(h, &hs.p));
+  return fs_io_write_dat(path, nt, h->impl.nf, h->impl.hnw.data(), hs.p.X,
+                         hs.p.q);
+}
+
+extern "C" int fs_system_write_bak(fs_system *h, const char *path, int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  HostState hs(h->impl);
+  FS_TRY(fs_system_get_state(h, &hs.p));
+  return fs_io_write_bak(path, nt, h->impl.nf, h->impl.hnw.data(), &hs.p);
+}
+
+extern "C" int fs_system_read_bak(fs_system *h, const char *path, int *nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  HostState hs(h->impl);
+  FS_TRY(fs_io_read_bak(path, h->impl.nf, h->impl.hnw.data(), &hs.p, nt));
+  return fs_system_set_state(h, &hs.p);
+}
+
+extern "C" int fs_system_write_checkpoint(fs_system *h, const char *path,
+                                          int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  HostState hs(h->impl);
+  FS_TRY(fs_system_get_state(h, &hs.p));
+  return fs_io_write_checkpoint(path, nt, h->impl.nf, h->impl.hnw.data(),
+                                &hs.p);
+}
+
+extern "C" int fs_system_read_checkpoint(fs_system *h, const char *path,
+                                         int *nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  HostState hs(h->impl);
+  FS_TRY(fs_io_read_checkpoint(path, h->impl.nf, h->impl.hnw.data(), &hs.p,
+                               nt));
+  return fs_system_set_state(h, &hs.p);
+}
+
 extern "C" int fs_system_set_tuning(fs_system *h, const fs_fcm_tuning *t) {
   FS_CHECK(h && t, FS_ERR_ARG, "null argument");
   return h->impl.fcm.set_tuning(*t);

@@ -161,3 +161,42 @@ def sweep_point(seed, grids=SWEEP_GRIDS, nfil_range=(8, 200),
     info = dict(grid=grid, nfil=nfil, segments=int(sum(nws)), kb=kb,
                 gravity=g, nw_cap=nw_cap, segment_cap=seg_cap)
     return fp, sp, nws, props, state, info
+
+
+def lattice_filaments(box_len, nfil, nw, dL=2.2, seed=1, jitter=0.3,
+                      min_sep=2.2):
+    """Dense straight filaments for large suspensions, where random
+    sequential placement saturates: rows of end-to-end filaments along +x or
+    -x (random per filament) on a square lattice in (y, z), each base point
+    jittered by up to `jitter` per axis. Returns (X, q) like
+    api.make_filaments; raises ValueError if nfil does not fit with
+    min_sep between segments of different filaments."""
+    import numpy as np
+    Lx, Ly, Lz = box_len
+    length = (nw - 1) * dL
+    per_row = int(Lx // (length + min_sep + 2 * jitter + dL))
+    if per_row < 1:
+        raise ValueError("filament longer than the box")
+    rows_needed = -(-nfil // per_row)
+    side = int(np.ceil(np.sqrt(rows_needed)))
+    pitch = min(Ly, Lz) / side
+    if pitch - 2 * jitter < min_sep:
+        raise ValueError("%d filaments do not fit at separation %g" %
+                         (nfil, min_sep))
+    rng = np.random.default_rng(seed)
+    slots = [(r, k) for r in range(side * side) for k in range(per_row)]
+    chosen = sorted(rng.choice(len(slots), nfil, replace=False))
+    X = np.zeros((nfil * nw, 3))
+    q = np.zeros((nfil * nw, 4))
+    xs = Lx / per_row
+    for f, si in enumerate(chosen):
+        r, k = slots[si]
+        y0, z0 = (r // side + 0.5) * pitch, (r % side + 0.5) * pitch
+        d = 1.0 if rng.random() < 0.5 else -1.0
+        base = np.array([k * xs + (0.0 if d > 0 else length),
+                         y0, z0]) + rng.uniform(-jitter, jitter, 3)
+        i = np.arange(nw)[:, None]
+        X[f * nw:(f + 1) * nw] = base + i * dL * np.array([d, 0.0, 0.0])
+        # +x: identity; -x: rotation by pi about z (tangent -e_x)
+        q[f * nw:(f + 1) * nw] = [1, 0, 0, 0] if d > 0 else [0, 0, 0, 1]
+    return X, q

@@ -12,20 +12,26 @@ import numpy as np
 from . import api
 
 
-def record(system, steps, nt0=1, torch=None, skip=0):
+def record(system, steps, nt0=1, torch=None, skip=0, log=None):
     """Runs `skip` plain timesteps from nt0 (the start-up transient, kept out
     of the data), then `steps` more while recording one sample per filament
     per step. Returns device tensors X (rows x n_in), T and the Broyden
-    iteration counts of the recorded steps."""
+    iteration counts of the recorded steps. `log` gets one line per step."""
     for nt in range(nt0, nt0 + skip):
-        system.timestep(nt)
+        it = system.timestep(nt)
+        if log:
+            log("start-up step %d: %d iterations" % (nt, it))
     nt0 += skip
     n_in, n_out = api.system_warmstart_dims(system)
     cap = system.nfil * steps
     X = torch.zeros((cap, n_in), dtype=torch.float32, device="cuda")
     T = torch.zeros((cap, n_out), dtype=torch.float32, device="cuda")
     api.system_record(system, X, T, cap)
-    its = [system.timestep(nt) for nt in range(nt0, nt0 + steps)]
+    its = []
+    for nt in range(nt0, nt0 + steps):
+        its.append(system.timestep(nt))
+        if log:
+            log("recorded step %d: %d iterations" % (nt, its[-1]))
     n = api.system_record_count(system)
     api.system_record(system, None, None, 0)
     return X[:n], T[:n], its

@@ -287,6 +287,39 @@ int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best);
 /* Set the launch shapes of the system's FCM solver (fs_fcm_set_tuning). */
 int fs_system_set_tuning(fs_system *h, const fs_fcm_tuning *t);
 
+/* ---- Trajectory output and restart (file_functions.cpp, Filament.hpp
+ * writeData/writeBackupData/continue_from_checkpoint). Text files keep the
+ * reference's layout (space separated, fixed notation, 16 decimals):
+ *   .dat  appends "nt " + per bead Xs(3) q(4) (file_functions.cpp:150-161)
+ *   .bak  "nt " + per filament: per bead X Xt q qt U Ut, per link lam lam1
+ *         lam2 (file_functions.cpp:211-222); reading sets Xs=X, qs=q, Us=U
+ *         (Filament.hpp:691-759)
+ *   .par  parameter file whose line 2 holds the filament count and line 5
+ *         the bead numbers (file_functions.cpp:6-81, 242-280)
+ * The binary checkpoint keeps every state array bit for bit. Host arrays in
+ * the fs_state_ptrs layout. */
+int fs_io_write_dat(const char *path, int nt, int nf, const int *nw,
+                    const double *X, const double *q);
+int fs_io_write_bak(const char *path, int nt, int nf, const int *nw,
+                    const fs_state_ptrs *s);
+int fs_io_read_bak(const char *path, int nf, const int *nw, fs_state_ptrs *s,
+                   int *nt);
+int fs_io_write_par(const char *path, const char *name, const char *tag, int nf,
+                    const int *nw, const double *bending_factors,
+                    const fs_sim_params *p, double kap, double c,
+                    int time_steps, int plot_steps);
+int fs_io_read_par(const char *path, int *nf, int *nw, int cap);
+int fs_io_write_checkpoint(const char *path, int nt, int nf, const int *nw,
+                           const fs_state_ptrs *s);
+int fs_io_read_checkpoint(const char *path, int nf, const int *nw,
+                          fs_state_ptrs *s, int *nt);
+/* The same for a system's state (copied to or from the device). */
+int fs_system_write_dat(fs_system *h, const char *path, int nt);
+int fs_system_write_bak(fs_system *h, const char *path, int nt);
+int fs_system_read_bak(fs_system *h, const char *path, int *nt);
+int fs_system_write_checkpoint(fs_system *h, const char *path, int nt);
+int fs_system_read_checkpoint(fs_system *h, const char *path, int *nt);
+
 /* ---- Batched sweep (no reference counterpart: the reference runs one
  * process per sweep point, batch_run.py:60-229). Independent systems of any
  * grid and filament counts advance together: each system's timestep is

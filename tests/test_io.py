@@ -1,0 +1,111 @@
+"""Trajectory and restart files (fs_io_*): the reference's text layouts
+(file_functions.cpp:6-81, 150-161, 211-319; Filament.hpp:547-562, 658-759)
+and the bit-faithful binary checkpoint. Host arrays only: no GPU."""
+import re
+
+import numpy as np
+import pytest
+
+from filsim_amd import abi, api, params
+
+NUM = re.compile(r"^-?\d+\.\d{16}$")
+
+
+def _state(nws, seed=0):
+    rng = np.random.default_rng(seed)
+    npt = sum(nws)
+    nlam = npt - len(nws)
+    st = {k: rng.normal(0, 50, (npt, 3)) for k in ("X", "Xt", "U", "Ut")}
+    for k in ("q", "qt"):
+        v = rng.normal(0, 1, (npt, 4))
+        st[k] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    for k in ("lam", "lam1", "lam2"):
+        st[k] = rng.normal(0, 3, (nlam, 3))
+    st["Xs"], st["Us"], st["qs"] = st["X"] + 1, st["U"] + 1, st["q"]
+    return st
+
+
+def test_bak_layout_and_round_trip(tmp_path):
+    nws = [5, 3, 7]
+    st = _state(nws)
+    path = str(tmp_path / "run.bak")
+    api.io_write_bak(path, 42, nws, st)
+    text = open(path).read()
+    assert not text.endswith("\n")
+    tok = text.split(" ")
+    assert tok[-1] == ""  # every value followed by one space
+    tok = tok[:-1]
+    npt, nlam = sum(nws), sum(nws) - len(nws)
+    assert tok[0] == "42" and len(tok) == 1 + 20 * npt + 9 * nlam
+    assert all(NUM.match(t) for t in tok[1:])
+    # per filament: beads (X Xt q qt U Ut) then links (lam lam1 lam2)
+    first = [float(t) for t in tok[1:21]]
+    assert np.allclose(first, np.concatenate(
+        [st["X"][0], st["Xt"][0], st["q"][0], st["qt"][0], st["U"][0],
+         st["Ut"][0]]), rtol=0, atol=1e-15 * 200)
+    link0 = [float(t) for t in tok[1 + 20 * 5:1 + 20 * 5 + 9]]
+    assert np.allclose(link0, np.concatenate(
+        [st["lam"][0], st["lam1"][0], st["lam2"][0]]), rtol=0, atol=1e-14)
+    nt, back = api.io_read_bak(path, nws)
+    assert nt == 42
+    for k in ("X", "Xt", "q", "qt", "U", "Ut", "lam", "lam1", "lam2"):
+        assert np.abs(back[k] - st[k]).max() <= 1e-15 * max(
+            1.0, np.abs(st[k]).max()), k
+    # continue_from_checkpoint: Xs = X, qs = q, Us = U
+    assert np.array_equal(back["Xs"], back["X"])
+    assert np.array_equal(back["qs"], back["q"])
+    assert np.array_equal(back["Us"], back["U"])
+
+
+def test_bak_truncated_rejected(tmp_path):
+    nws = [4, 4]
+    path = str(tmp_path / "t.bak")
+    api.io_write_bak(path, 3, nws, _state(nws))
+    text = open(path).read()
+    open(path, "w").write(text[: len(text) // 2])
+    with pytest.raises(abi.FsError):
+        api.io_read_bak(path, nws)
+
+
+def test_dat_appends_lines(tmp_path):
+    nws = [4, 6]
+    st = _state(nws, 1)
+    path = str(tmp_path / "run.dat")
+    for nt in (10, 20):
+        api.io_write_dat(path, nt, nws, st["X"], st["q"])
+    lines = open(path).read().split("\n")
+    assert lines[-1] == "" and len(lines) == 3
+    for nt, line in zip((10, 20), lines[:2]):
+        tok = line.split(" ")[:-1]
+        assert tok[0] == str(nt) and len(tok) == 1 + 7 * sum(nws)
+        v = np.array([float(t) for t in tok[1:]]).reshape(-1, 7)
+        assert np.abs(v[:, :3] - st["X"]).max() <= 1e-13
+        assert np.abs(v[:, 3:] - st["q"]).max() <= 1e-15
+
+
+def test_par_round_trip(tmp_path):
+    nws = [15, 20, 30, 2]
+    fp = params.fcm_params(64)
+    sp = params.sim_params(fp, dt=0.005, tol=1e-6, maxiter=100)
+    path = str(tmp_path / "run.par")
+    api.io_write_par(path, nws, sp, kap=1000.0, time_steps=2000)
+    lines = open(path).read().split("\n")
+    assert lines[0].split()[:4] == ["SimulationName", "SimulationTag", "Nsw",
+                                    "Nworm"]
+    assert lines[1].split()[2:4] == ["4", "30"] and lines[2] == ""
+    assert lines[3] == "BeadNumbers" and lines[4] == "15 20 30 2"
+    assert lines[6] == "BendingModuli are KAP*"
+    assert api.io_read_par(path) == nws
+
+
+def test_binary_checkpoint_bitwise(tmp_path):
+    nws = [7, 9]
+    st = _state(nws, 2)
+    path = str(tmp_path / "run.fsck")
+    api.io_write_checkpoint(path, 77, nws, st)
+    nt, back = api.io_read_checkpoint(path, nws)
+    assert nt == 77
+    for k in api._IO_FIELDS:
+        assert np.array_equal(back[k].reshape(st[k].shape), st[k]), k
+    with pytest.raises(abi.FsError):
+        api.io_read_checkpoint(path, [8, 8])
