@@ -126,6 +126,8 @@ PROTOS = {
     "fs_system_last_step_info": [P, IP, _DP, _DP],
     "fs_system_autotune": [P, C.c_int, C.POINTER(FcmTuning)],
     "fs_system_set_tuning": [P, C.POINTER(FcmTuning)],
+    "fs_system_set_profiling": [P, C.c_int],
+    "fs_system_profile": [P, _DP, C.c_int],
     "fs_io_write_dat": [C.c_char_p, C.c_int, C.c_int, IP, _DP, _DP],
     "fs_io_write_bak": [C.c_char_p, C.c_int, C.c_int, IP, C.POINTER(StatePtrs)],
     "fs_io_read_bak": [C.c_char_p, C.c_int, IP, C.POINTER(StatePtrs), IP],

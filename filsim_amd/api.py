@@ -283,6 +283,20 @@ class System:
         """Launch shapes of the system's FCM solver (fs_system_set_tuning)."""
         abi.check(self.lib.fs_system_set_tuning(self.h, C.byref(t)))
 
+    PROFILE_BUCKETS = ("FCM", "Assign F", "Bld Jinv", "ErrorChk", "Slv Jinv",
+                       "Total", "ColBarri", "FCMMakeZ", "FCMGauss",
+                       "FCMForcD", "FCMFFTFw", "FCMFFTBk", "FCMFlowS",
+                       "FCMParVR")
+
+    def set_profiling(self, on=True):
+        abi.check(self.lib.fs_system_set_profiling(self.h, 1 if on else 0))
+
+    def profile(self):
+        """ms per bucket of the last profiled timestep (fs_system_profile)."""
+        ms = np.zeros(len(self.PROFILE_BUCKETS))
+        abi.check(self.lib.fs_system_profile(self.h, abi.dptr(ms), len(ms)))
+        return dict(zip(self.PROFILE_BUCKETS, ms.tolist()))
+
 
 class Batch:
     """Independent systems advanced together through captured hipGraphs

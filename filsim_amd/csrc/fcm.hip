@@ -844,6 +844,9 @@ void FcmImpl::release() {
   scan_tmp.release();
   for (int i = 0; i < 8; ++i)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
+  for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
+  prof_ev.clear();
+  prof_used = 0;
 }
 
 template <typename T>
@@ -864,8 +867,19 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   g.ny = p.ny;
   g.nz = p.nz;
 
+  hipEvent_t *pset = nullptr;  // this solve's profiling events
+  if (prof) {
+    while (prof_ev.size() < prof_used + 7) {
+      hipEvent_t e = nullptr;
+      FS_HIP(hipEventCreate(&e));
+      prof_ev.push_back(e);
+    }
+    pset = prof_ev.data() + prof_used;
+    prof_used += 7;
+  }
   auto mark = [&](int i) {
     if (timing) (void)hipEventRecord(ev[i], st);
+    if (pset) (void)hipEventRecord(pset[i], st);
   };
   mark(0);
   // bin counting sort
@@ -964,6 +978,24 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   FS_CHECK(okn, FS_ERR_ARG, "interp dispatch failed");
   FS_LAUNCHED();
   mark(6);
+  return FS_OK;
+}
+
+// add the stage times of every solve since the last call into acc7
+// (buckets of stage_times); the caller has synchronised the stream
+int FcmImpl::prof_collect(double *acc7) {
+  for (size_t k = 0; k + 7 <= prof_used; k += 7) {
+    float t[6];
+    for (int i = 0; i < 6; ++i)
+      FS_HIP(hipEventElapsedTime(&t[i], prof_ev[k + i], prof_ev[k + i + 1]));
+    acc7[1] += t[0];
+    acc7[2] += t[1];
+    acc7[3] += t[2];
+    acc7[4] += t[3];
+    acc7[5] += t[4];
+    acc7[6] += t[5];
+  }
+  prof_used = 0;
   return FS_OK;
 }
 

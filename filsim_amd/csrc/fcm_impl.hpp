@@ -1,6 +1,8 @@
 // fcm_impl.hpp -- host-side state of one FCM solver (struct fs_fcm).
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 #include "fft.hpp"
 
@@ -22,6 +24,12 @@ struct FcmImpl {
   int tsz = 8;  // bytes per real
   int last_np = 0;
   int timing = 0;
+  // accumulate stage times over every solve (profiling): one set of 7
+  // events per solve from a pool, summed by prof_collect
+  int prof = 0;
+  std::vector<hipEvent_t> prof_ev;
+  size_t prof_used = 0;
+  int prof_collect(double *acc7);
   FftPlan plx{}, ply{}, plz{};
   fs_fcm_tuning tune{};
   // spread tile geometry (mirrors SpreadGeom in fcm.hip)

@@ -1373,19 +1373,85 @@ void SystemImpl::release() {
   st = nullptr;
 }
 
+// ---------------------------------------------------------------------
+// profiling: an event pair around each stage call, in the reference
+// profiler's buckets; collected after the timestep's stream sync
+// ---------------------------------------------------------------------
+hipEvent_t SystemImpl::prof_event() {
+  if (pev_used == pev.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    pev.push_back(e);
+  }
+  return pev[pev_used++];
+}
+
+void SystemImpl::prof_reset() {
+  pev_used = 0;
+  pmarks.clear();
+  for (double &v : pms) v = 0;
+  fcm.prof_used = 0;
+}
+
+int SystemImpl::prof_collect() {
+  for (const PMark &m : pmarks) {
+    float t = 0;
+    if (m.a && m.z) FS_HIP(hipEventElapsedTime(&t, m.a, m.z));
+    pms[m.b] += t;
+  }
+  double acc7[7] = {0, 0, 0, 0, 0, 0, 0};
+  FS_TRY(fcm.prof_collect(acc7));
+  pms[P_MAKEZ] += acc7[0];
+  pms[P_GAUSS] += acc7[1];
+  pms[P_FORCD] += acc7[2];
+  pms[P_FFTFW] += acc7[3];
+  pms[P_FLOWS] += acc7[4];
+  pms[P_FFTBK] += acc7[5];
+  pms[P_PARVR] += acc7[6];
+  pmarks.clear();
+  pev_used = 0;
+  return FS_OK;
+}
+
+namespace {
+struct ProfScope {
+  SystemImpl &S;
+  int b;
+  hipEvent_t a = nullptr;
+  ProfScope(SystemImpl &s, int bucket) : S(s), b(bucket) {
+    if (S.prof) {
+      a = S.prof_event();
+      if (a) (void)hipEventRecord(a, S.st);
+    }
+  }
+  ~ProfScope() {
+    if (S.prof && a) {
+      hipEvent_t z = S.prof_event();
+      if (z) {
+        (void)hipEventRecord(z, S.st);
+        S.pmarks.push_back({b, a, z});
+      }
+    }
+  }
+};
+}  // namespace
+
 int SystemImpl::initial_guess() {
+  ProfScope ps_(*this, P_ASSIGN);
   hipLaunchKernelGGL(k_guess, dim3(nf), dim3(64), 0, st, d);
   FS_LAUNCHED();
   return FS_OK;
 }
 
 int SystemImpl::forces(int nt) {
+  ProfScope ps_(*this, P_ASSIGN);
   hipLaunchKernelGGL(k_forces, dim3(nf), dim3(64), 0, st, d, nt);
   FS_LAUNCHED();
   return FS_OK;
 }
 
 int SystemImpl::update(const double *u) {
+  ProfScope ps_(*this, P_ASSIGN);
   hipLaunchKernelGGL(k_update, dim3(nf), dim3(64), 0, st, d, u);
   FS_LAUNCHED();
   return FS_OK;
@@ -1423,6 +1489,7 @@ int SystemImpl::build_jacobian_dev(int nt) {
 }
 
 int SystemImpl::build_jacobian(int nt) {
+  ProfScope ps_(*this, P_BLDJ);
   FS_TRY(build_jacobian_dev(nt));
   // history reset (FilamentJacobianSolver.hpp:40-41)
   size_t nh = 6 * (size_t)np * p.broyden_maxiter;
@@ -1451,6 +1518,7 @@ int SystemImpl::jsolve(const double *b, double *x, int iter) {
 // broyden_maxiter); the launches honour d.act.
 int SystemImpl::jsolve_impl(const double *b, double *x, int ncol,
                             const int *ncolp, int noff) {
+  ProfScope ps_(*this, P_SLVJ);
   const int *act = d.act;
   hipLaunchKernelGGL(k_jsolve_block, dim3(nf), dim3(256), 0, st, d, Jinv_cm(),
                      b, x);
@@ -1498,6 +1566,7 @@ int SystemImpl::broyden_update(const double *Eo, const double *En,
 
 int SystemImpl::residual(int nt, const double *V, const double *W, double *err,
                          int *check, double *maxabs) {
+  ProfScope ps_(*this, P_ERRCHK);
   unsigned long long *mb = flags.as<unsigned long long>();
   hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, d.act, mb, 0ull);
   hipLaunchKernelGGL(k_residual, dim3(ceil_div(np, 256)), dim3(256), 0, st, d,
@@ -1522,12 +1591,14 @@ int SystemImpl::step(int nt) {
 }
 
 int SystemImpl::mobility(int nt) {
+  ProfScope ps_(*this, P_FCM);
   if (nt > p.initial_friction_steps)
     return fcm.mobility(np, d.Ya, d.Fa, d.Ta, d.V, d.W, st);
   return fcm.friction(np, d.Fa, d.Ta, d.V, d.W, st);
 }
 
 int SystemImpl::barrier(int nt, int link, double *min_dist2) {
+  ProfScope ps_(*this, P_COLBAR);
   if (!p.barrier_on) {
     if (min_dist2) *min_dist2 = 100000.0;
     return FS_OK;
@@ -1572,19 +1643,25 @@ int SystemImpl::broyden_loop(int nt, int *iters) {
     if (!check || iter >= p.broyden_maxiter) break;
     // dE = NewE - E; |dE|^2 and dE.NewE; g = H_{iter-1} NewE
     double *dots = coefb.as<double>() + p.broyden_maxiter;  // 2 scalars
-    hipLaunchKernelGGL(k_diff_dots, dim3(dot_blocks), dim3(256), 0, st,
-                       (const int *)nullptr, NewE, E, dE, n6, dot_chunk,
-                       partial.as<double>());
-    hipLaunchKernelGGL(k_dots_final, dim3(2), dim3(64), 0, st,
-                       (const int *)nullptr, partial.as<double>(), dot_blocks, 2,
-                       2, (const int *)nullptr, 0, dots);
-    FS_LAUNCHED();
+    {
+      ProfScope ps_(*this, P_SLVJ);
+      hipLaunchKernelGGL(k_diff_dots, dim3(dot_blocks), dim3(256), 0, st,
+                         (const int *)nullptr, NewE, E, dE, n6, dot_chunk,
+                         partial.as<double>());
+      hipLaunchKernelGGL(k_dots_final, dim3(2), dim3(64), 0, st,
+                         (const int *)nullptr, partial.as<double>(), dot_blocks,
+                         2, 2, (const int *)nullptr, 0, dots);
+      FS_LAUNCHED();
+    }
     FS_TRY(jsolve(NewE, tmp, iter - 1));
-    hipLaunchKernelGGL(k_broyden_fused, dim3(ceil_div((long)n6, 256)),
-                       dim3(256), 0, st, (const int *)nullptr, tmp, dE, dots,
-                       n6, Cm.as<double>(), Dm.as<double>(), iter - 1,
-                       (const int *)nullptr, Upd);
-    FS_LAUNCHED();
+    {
+      ProfScope ps_(*this, P_SLVJ);
+      hipLaunchKernelGGL(k_broyden_fused, dim3(ceil_div((long)n6, 256)),
+                         dim3(256), 0, st, (const int *)nullptr, tmp, dE, dots,
+                         n6, Cm.as<double>(), Dm.as<double>(), iter - 1,
+                         (const int *)nullptr, Upd);
+      FS_LAUNCHED();
+    }
     std::swap(E, NewE);
   }
   if (iters) *iters = iter;

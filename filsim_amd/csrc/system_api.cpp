@@ -121,8 +121,40 @@ extern "C" int fs_system_fcm_arrays(fs_system *h, double **Y, double **F,
 
 extern "C" int fs_system_timestep(fs_system *h, int nt, int *iters) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
-  if (h->ws.attached()) return h->ws.timestep(h->impl, nt, iters);
-  return h->impl.timestep(nt, iters);
+  SystemImpl &S = h->impl;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (S.prof) {
+    S.prof_reset();
+    t0 = S.prof_event();
+    t1 = S.prof_event();
+    if (t0) FS_HIP(hipEventRecord(t0, S.st));
+  }
+  int rc = h->ws.attached() ? h->ws.timestep(S, nt, iters)
+                            : S.timestep(nt, iters);
+  if (rc != FS_OK || !S.prof) return rc;
+  if (t1) FS_HIP(hipEventRecord(t1, S.st));
+  FS_HIP(hipStreamSynchronize(S.st));
+  FS_TRY(S.prof_collect());
+  float total = 0;
+  if (t0 && t1) FS_HIP(hipEventElapsedTime(&total, t0, t1));
+  S.pms[SystemImpl::P_TOTAL] = total;
+  return FS_OK;
+}
+
+extern "C" int fs_system_set_profiling(fs_system *h, int on) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_HIP(hipStreamSynchronize(h->impl.st));
+  h->impl.prof = on ? 1 : 0;
+  h->impl.fcm.prof = on ? 1 : 0;
+  h->impl.prof_reset();
+  return FS_OK;
+}
+
+extern "C" int fs_system_profile(fs_system *h, double *ms, int n) {
+  FS_CHECK(h && ms && n >= SystemImpl::P_N, FS_ERR_ARG,
+           "need room for %d buckets", (int)SystemImpl::P_N);
+  for (int i = 0; i < SystemImpl::P_N; ++i) ms[i] = h->impl.pms[i];
+  return FS_OK;
 }
 
 namespace fs {
@@ -510,6 +542,7 @@ int batch_capture(fs_batch *b, int kind) {
   for (size_t i = 0; i < b->sys.size(); ++i) {
     SystemImpl &S = b->sys[i]->impl;
     FS_HIP(hipStreamSynchronize(S.st));
+    FS_CHECK(!S.prof, FS_ERR_STATE, "system %d: profiling is on", (int)i);
     S.set_device_control(nullptr, b->dnt);
     FS_HIP(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal));
     int *status = b->dstat + 3 * i;

@@ -66,6 +66,25 @@ struct SystemImpl {
   int broyden_loop(int nt, int *iters);
   int check0 = 0;
 
+  // --- profiling: the reference profiler's buckets (profilers.hpp:24-57),
+  // accumulated per timestep from HIP event pairs on the system's stream
+  enum {
+    P_FCM, P_ASSIGN, P_BLDJ, P_ERRCHK, P_SLVJ, P_TOTAL, P_COLBAR,
+    P_MAKEZ, P_GAUSS, P_FORCD, P_FFTFW, P_FFTBK, P_FLOWS, P_PARVR, P_N
+  };
+  int prof = 0;
+  std::vector<hipEvent_t> pev;
+  size_t pev_used = 0;
+  struct PMark {
+    int b;
+    hipEvent_t a, z;
+  };
+  std::vector<PMark> pmarks;
+  double pms[P_N] = {};
+  hipEvent_t prof_event();
+  void prof_reset();
+  int prof_collect();
+
   // --- batched sweep: the timestep as three host-sync-free launch
   // sequences, captured once into graphs and replayed. ctl = {iterations,
   // still iterating, singular Jacobian}; ntp = device step number.

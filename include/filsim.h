@@ -282,6 +282,19 @@ int fs_system_ws_count(fs_system *h, int *count);
  * infinity norms of the residuals of both candidate guesses. */
 int fs_system_last_step_info(fs_system *h, int *used_mlp, double *res_plain,
                              double *res_mlp);
+/* Per-timestep profile in the reference profiler's buckets
+ * (profilers.hpp:24-57), milliseconds of device time measured with HIP
+ * events on the system's stream, for the last fs_system_timestep with
+ * profiling on. Order (14): FCM, Assign F, Bld Jinv, ErrorChk, Slv Jinv,
+ * Total, ColBarri, FCMMakeZ, FCMGauss, FCMForcD, FCMFFTFw, FCMFFTBk,
+ * FCMFlowS, FCMParVR. Here FCMMakeZ is 0 (the spread writes every point
+ * once), FCMGauss holds the Gaussian tables and the bin sort, FCMFFTFw the
+ * z and y forward passes, FCMFlowS the fused x forward FFT + projection + x
+ * inverse FFT and FCMFFTBk the y and z inverse passes. Profiling adds event
+ * records (and one synchronisation per step), not kernels; not allowed in a
+ * batch. */
+int fs_system_set_profiling(fs_system *h, int on);
+int fs_system_profile(fs_system *h, double *ms, int n);
 /* Autotune the system's FCM solver for its segment count (fs_fcm_autotune). */
 int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best);
 /* Set the launch shapes of the system's FCM solver (fs_fcm_set_tuning). */
