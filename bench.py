@@ -428,6 +428,13 @@ def main():
             el = time.perf_counter() - t0
             finals[name] = s.get_state()["X"]
             res[name] = (el, its)
+            # one more step, profiled (outside the timed region), in the
+            # reference profiler's buckets
+            s.set_profiling(True)
+            s.timestep(nt)
+            info["profile_ms_" + name] = {
+                k: round(v, 4) for k, v in s.profile().items()}
+            s.set_profiling(False)
             log("%s: %d steps in %.3f s -> %.3f steps/s, iters %s%s" %
                 (name, args.steps, el, args.steps / el, its,
                  " (network guess used %d/%d)" % (used, args.steps)
