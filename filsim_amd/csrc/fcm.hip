@@ -49,6 +49,17 @@ struct GaussConst {
   int ngd, ngdh, nx, ny, nz;
 };
 
+// Grid cell holding coordinate y: truncation of y/dx, as
+// FCMfunctions.cpp:565-567. A coordinate that is not finite or lies more
+// than 2^30 cells out (a diverged iterate) gets cell s instead: no
+// saturating conversion or signed overflow downstream, and such segments
+// spread over many bins rather than piling into one (the per-bin sort is
+// quadratic in the bin's population). Its tables are then 0 or NaN.
+__device__ __forceinline__ int stencil_cell(double y, double dx, int s) {
+  const double cy = y / dx;
+  return fabs(cy) < 1073741824.0 ? (int)cy : s;
+}
+
 template <typename T>
 __global__ void k_prepare_tab(const int *__restrict__ act, int np, const double *__restrict__ Y,
                               GaussConst g, T *__restrict__ tab) {
@@ -58,7 +69,7 @@ __global__ void k_prepare_tab(const int *__restrict__ act, int np, const double 
   const int i = (int)(t % g.ngd), d = (int)((t / g.ngd) % 3),
             s = (int)(t / (3 * g.ngd));
   const double y = Y[3 * s + d];
-  const int c = (int)(y / g.dx);  // truncation, as FCMfunctions.cpp:565-567
+  const int c = stencil_cell(y, g.dx, s);
   const int xg = c - g.ngdh + 1 + i;
   const double xx = ((double)xg) * g.dx - y;
   T *ts = tab + (size_t)s * 9 * g.ngd;
@@ -81,7 +92,7 @@ __global__ void k_prepare(const int *__restrict__ act, int np, const double *__r
   const int nn[3] = {g.nx, g.ny, g.nz};
   int o[3];
   for (int d = 0; d < 3; ++d) {
-    int c = (int)(Y[3 * s + d] / g.dx);
+    int c = stencil_cell(Y[3 * s + d], g.dx, s);
     int w = (c - g.ngdh + 1) % nn[d];
     if (w < 0) w += nn[d];
     o[d] = w;

@@ -8,6 +8,7 @@
 namespace fs {
 
 constexpr int kScanBlock = 1024;
+constexpr int kScanMaxBlocks = 65536;  // up to 64Mi elements
 
 namespace {  // each including translation unit gets its own kernels
 
@@ -27,20 +28,27 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) bsum[b] = red[0];
 }
 
-// Exclusive scan in place of nb <= 1024 block sums; bsum[nb] = total.
+// Exclusive scan in place of nb block sums, 1024 at a time with a running
+// carry; bsum[nb] = total.
 __global__ void __launch_bounds__(1024) k_scan_top(int *__restrict__ bsum, int nb) {
   __shared__ int s[1024];
   const int t = threadIdx.x;
-  s[t] = (t < nb) ? bsum[t] : 0;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = (t >= off) ? s[t - off] : 0;
+  int carry = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    const int i = base + t;
+    s[t] = (i < nb) ? bsum[i] : 0;
     __syncthreads();
-    s[t] += v;
-    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int v = (t >= off) ? s[t - off] : 0;
+      __syncthreads();
+      s[t] += v;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + ((t == 0) ? 0 : s[t - 1]);
+    carry += s[1023];
+    __syncthreads();  // s is rewritten by the next chunk
   }
-  if (t < nb) bsum[t] = (t == 0) ? 0 : s[t - 1];
-  if (t == 0) bsum[nb] = s[1023];
+  if (t == 0) bsum[nb] = carry;
 }
 
 __global__ void __launch_bounds__(1024)
@@ -67,8 +75,8 @@ __global__ void __launch_bounds__(1024)
 inline int device_scan(const int *cnt, int n, int *start, int *scratch,
                        hipStream_t st) {
   const int nb = (n + kScanBlock - 1) / kScanBlock;
-  FS_CHECK(nb >= 1 && nb <= 1024, FS_ERR_ARG, "scan of %d elements too large",
-           n);
+  FS_CHECK(nb >= 1 && nb <= kScanMaxBlocks, FS_ERR_ARG,
+           "scan of %d elements too large", n);
   hipLaunchKernelGGL(k_scan_blocksum, dim3(nb), dim3(256), 0, st, cnt, n,
                      scratch);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, scratch, nb);

@@ -163,40 +163,82 @@ def sweep_point(seed, grids=SWEEP_GRIDS, nfil_range=(8, 200),
     return fp, sp, nws, props, state, info
 
 
-def lattice_filaments(box_len, nfil, nw, dL=2.2, seed=1, jitter=0.3,
-                      min_sep=2.2):
-    """Dense straight filaments for large suspensions, where random
-    sequential placement saturates: rows of end-to-end filaments along +x or
-    -x (random per filament) on a square lattice in (y, z), each base point
-    jittered by up to `jitter` per axis. Returns (X, q) like
-    api.make_filaments; raises ValueError if nfil does not fit with
-    min_sep between segments of different filaments."""
+def _qmul(a, b):
+    """Hamilton product of (w, x, y, z) quaternions (UnitQuaternion.hpp
+    convention: tangent = R(q) e_x)."""
+    aw, ax, ay, az = a
+    bw, bx, by, bz = b
+    return (aw * bw - ax * bx - ay * by - az * bz,
+            aw * bx + ax * bw + ay * bz - az * by,
+            aw * by - ax * bz + ay * bw + az * bx,
+            aw * bz + ax * by - ay * bx + az * bw)
+
+
+def rod_packing_filaments(box_len, nfil, nw, dL=2.2, seed=1, jitter=0.3,
+                          min_sep=2.2, roll=True):
+    """Dense straight filaments with no net alignment, for suspensions too
+    dense for random placement: a cubic rod packing with m x m lines along
+    each of x, y and z, per period cell (c_x, c_y, c_z) = box / m
+
+        x lines at (y, z) = (j c_y,         k c_z + c_z/2)
+        y lines at (x, z) = (j c_x + c_x/2, k c_z)
+        z lines at (x, y) = (j c_x,         k c_y + c_y/2)
+
+    (all offset by c/4), so lines of different axes are c/2 apart and
+    parallel lines c apart. Each line holds as many end-to-end filaments as
+    fit with a gap of min_sep + 2 jitter + dL between them; nfil of these
+    slots are drawn at random, each
+    filament pointing either way along its line, rolled about its own axis
+    by a random angle (roll=True) and its base jittered by up to `jitter`
+    per axis. Equal numbers of lines along the three axes cancel the mean
+    orientation order, whose coherent stresslets make an aligned lattice
+    of stiff swimmers at this density unstable. m is the smallest count
+    that holds nfil. Returns (X, q) like api.make_filaments; raises
+    ValueError when nfil does not fit with min_sep between filaments."""
     import numpy as np
-    Lx, Ly, Lz = box_len
+    L = np.asarray(box_len, dtype=float)
     length = (nw - 1) * dL
-    per_row = int(Lx // (length + min_sep + 2 * jitter + dL))
-    if per_row < 1:
+    per_line = [int(L[a] // (length + min_sep + 2 * jitter + dL))
+                for a in range(3)]
+    if min(per_line) < 1:
         raise ValueError("filament longer than the box")
-    rows_needed = -(-nfil // per_row)
-    side = int(np.ceil(np.sqrt(rows_needed)))
-    pitch = min(Ly, Lz) / side
-    if pitch - 2 * jitter < min_sep:
+    m = 1
+    while sum(per_line) * m * m < nfil:
+        m += 1
+    c = L / m
+    if 0.5 * c.min() - 2 * jitter < min_sep:
         raise ValueError("%d filaments do not fit at separation %g" %
                          (nfil, min_sep))
+    # line offsets in the two transverse axes, per line axis (in cells)
+    trans = {0: ((1, 0.0), (2, 0.5)), 1: ((0, 0.5), (2, 0.0)),
+             2: ((0, 0.0), (1, 0.5))}
+    h = np.sqrt(0.5)
+    q_axis = {0: (1.0, 0.0, 0.0, 0.0), 1: (h, 0.0, 0.0, h),
+              2: (h, 0.0, -h, 0.0)}
+    slots = [(a, j, k, n) for a in range(3) for j in range(m)
+             for k in range(m) for n in range(per_line[a])]
     rng = np.random.default_rng(seed)
-    slots = [(r, k) for r in range(side * side) for k in range(per_row)]
     chosen = sorted(rng.choice(len(slots), nfil, replace=False))
     X = np.zeros((nfil * nw, 3))
     q = np.zeros((nfil * nw, 4))
-    xs = Lx / per_row
+    i = np.arange(nw)[:, None]
     for f, si in enumerate(chosen):
-        r, k = slots[si]
-        y0, z0 = (r // side + 0.5) * pitch, (r % side + 0.5) * pitch
+        a, j, k, n = slots[si]
         d = 1.0 if rng.random() < 0.5 else -1.0
-        base = np.array([k * xs + (0.0 if d > 0 else length),
-                         y0, z0]) + rng.uniform(-jitter, jitter, 3)
-        i = np.arange(nw)[:, None]
-        X[f * nw:(f + 1) * nw] = base + i * dL * np.array([d, 0.0, 0.0])
-        # +x: identity; -x: rotation by pi about z (tangent -e_x)
-        q[f * nw:(f + 1) * nw] = [1, 0, 0, 0] if d > 0 else [0, 0, 0, 1]
+        theta = rng.uniform(0.0, 2.0 * np.pi) if roll else 0.0
+        base = np.zeros(3)
+        (t1, o1), (t2, o2) = trans[a]
+        base[t1] = (j + o1 + 0.25) * c[t1]
+        base[t2] = (k + o2 + 0.25) * c[t2]
+        base[a] = n * L[a] / per_line[a] + 0.25 * c[a] + \
+            (0.0 if d > 0 else length)
+        base += rng.uniform(-jitter, jitter, 3)
+        e = np.zeros(3)
+        e[a] = d
+        X[f * nw:(f + 1) * nw] = base + i * dL * e
+        qf = q_axis[a]
+        if d < 0:  # pi about the body z axis: tangent -> -tangent
+            qf = _qmul(qf, (0.0, 0.0, 0.0, 1.0))
+        qf = _qmul(qf, (np.cos(0.5 * theta), np.sin(0.5 * theta), 0.0, 0.0))
+        q[f * nw:(f + 1) * nw] = qf
     return X, q

@@ -55,6 +55,31 @@ def test_lists_pairs_forces(cuda, grid, nfil, nw, nt, ifs):
     gb.close()
 
 
+def test_more_than_a_million_cells(cuda):
+    """A 512^3 box has 110^3 barrier cells: the cell-count scan runs over
+    more than 1024 blocks of 1024. A dense cluster (a 64^3 box's worth of
+    filaments) sits inside it so pairs exist."""
+    torch = cuda
+    fp = params.fcm_params(512)
+    cp = params.colbar_params(fp)
+    X, sf, si = _system(params.fcm_params(64), 40, 15, seed=3)
+    rng = np.random.default_rng(2)
+    Xs = X + rng.normal(0, 0.6, X.shape) + 100.0
+    Y = np.fmod(Xs, np.array(params.box(fp)))
+    ob = O.Colbar(cp, sf, si)
+    ob.link(Xs)
+    h_o, l_o = ob.lists()
+    gb = api.Colbar(cp, sf, si)
+    gb.link(to_dev(torch, Xs))
+    h_g, l_g = gb.lists()
+    assert len(h_o) > 1024 * 1024
+    assert np.array_equal(h_g, h_o) and np.array_equal(l_g, l_o)
+    p_o = ob.pairs(Y, 1)
+    assert len(p_o) > 0
+    assert np.array_equal(gb.pairs(to_dev(torch, Y), 1), p_o)
+    gb.close()
+
+
 def test_stale_list(cuda):
     """Between links the list is reused even after segments move
     (main.cpp:394 vs :565)."""

@@ -19,6 +19,8 @@ CASES = [
     # stencils wrapping onto both ends of a spread tile
     ((16, 16, 16), 12, None),
     ((24, 24, 24), 20, "ref"),
+    # 512-point lines (the 512^3 suspension's FFT length)
+    ((512, 16, 24), 12, None),
 ]
 
 
