@@ -60,6 +60,16 @@ __device__ __forceinline__ int stencil_cell(double y, double dx, int s) {
   return fabs(cy) < 1073741824.0 ? (int)cy : s;
 }
 
+// Bin of tile (bx, by, bz): z fastest, so the bin-sorted segment list runs
+// along the grid's contiguous axis and segments next to each other in it
+// share cache lines of the real grids (interpolation reads them in list
+// order). The spread enumerates its candidate bins by tile offset, so its
+// summation order does not depend on this numbering.
+__device__ __forceinline__ int spread_bin(int bx, int by, int bz, int nbx,
+                                          int nby, int nbz) {
+  return bz + nbz * (by + nby * bx);
+}
+
 template <typename T>
 __global__ void k_prepare_tab(const int *__restrict__ act, int np, const double *__restrict__ Y,
                               GaussConst g, T *__restrict__ tab) {
@@ -82,7 +92,7 @@ template <typename T>
 __global__ void k_prepare(const int *__restrict__ act, int np, const double *__restrict__ Y,
                           const double *__restrict__ F,
                           const double *__restrict__ Tq, GaussConst g,
-                          int tx, int ty, int tz, int nbx, int nby,
+                          int tx, int ty, int tz, int nbx, int nby, int nbz,
                           int *__restrict__ origin, T *__restrict__ coef,
                           int *__restrict__ bin, int *__restrict__ bin_count,
                           int *__restrict__ slot) {
@@ -103,7 +113,7 @@ __global__ void k_prepare(const int *__restrict__ act, int np, const double *__r
     coef[6 * s + d] = (T)F[3 * s + d];
     coef[6 * s + 3 + d] = (T)(0.5 * Tq[3 * s + d]);
   }
-  int b = (o[0] / tx) + nbx * ((o[1] / ty) + nby * (o[2] / tz));
+  int b = spread_bin(o[0] / tx, o[1] / ty, o[2] / tz, nbx, nby, nbz);
   bin[s] = b;
   slot[s] = atomicAdd(&bin_count[b], 1);
 }
@@ -185,7 +195,7 @@ __global__ void __launch_bounds__(256)
     int ix = ((bx - G.hx + dx) % G.nbx + G.nbx) % G.nbx;
     int iy = ((by - G.hy + dy) % G.nby + G.nby) % G.nby;
     int iz = ((bz - G.hz + dz) % G.nbz + G.nbz) % G.nbz;
-    int b = ix + G.nbx * (iy + G.nby * iz);
+    int b = spread_bin(ix, iy, iz, G.nbx, G.nby, G.nbz);
     s_bfirst[t] = start[b];
     s_bstart[t + 1] = start[b + 1] - start[b];
   }
@@ -900,7 +910,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                      dim3(256), 0, st, act, np, Y, g, tab.as<T>());
   hipLaunchKernelGGL(k_prepare<T>, dim3(ceil_div(np, 64)), dim3(64), 0, st, act, np,
                      Y, F, Tq, g, geom.tx, geom.ty, geom.tz, geom.nbx, geom.nby,
-                     origin.as<int>(), coef.as<T>(), bin.as<int>(),
+                     geom.nbz, origin.as<int>(), coef.as<T>(), bin.as<int>(),
                      bin_count.as<int>(), slot.as<int>());
   FS_TRY(device_scan(bin_count.as<int>(), nbins, bin_start.as<int>(),
                      scan_tmp.as<int>(), st));
