@@ -317,16 +317,73 @@ __global__ void __launch_bounds__(256)
 // reads runs of consecutive x.
 // ---------------------------------------------------------------------
 
+// Kernels of the FFT passes are instantiated on LOGN: n = 2^LOGN known at
+// compile time (LOGN > 0) or taken from the plan (LOGN = 0).
+template <int LOGN>
+__device__ __forceinline__ int fft_len(const FftPlan &pl) {
+  if constexpr (LOGN > 0)
+    return 1 << LOGN;
+  else
+    return pl.n;
+}
+template <int LOGN>
+__device__ __forceinline__ int div_len(int e, const FftPlan &pl) {  // e / n
+  if constexpr (LOGN > 0)
+    return e >> LOGN;
+  else
+    return pl.dn.q(e);
+}
+
+// 16-byte global accesses of reals (float4 / double2); p must be 16-byte
+// aligned.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4, logn = 2;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2, logn = 1;
+};
+template <typename T>
+__device__ __forceinline__ void ld16(const T *p, T *out) {
+  using V = typename Vec16<T>::type;
+  const V v = *reinterpret_cast<const V *>(p);
+  __builtin_memcpy(out, &v, 16);
+}
+template <typename T>
+__device__ __forceinline__ void st16(T *p, const T *in) {
+  using V = typename Vec16<T>::type;
+  V v;
+  __builtin_memcpy(&v, in, 16);
+  *reinterpret_cast<V *>(p) = v;
+}
+
+// Complex values per 16-byte access: 2 (FP32) or 1 (FP64).
+template <typename T>
+constexpr int kCx16 = 16 / (int)sizeof(cx<T>);
+template <typename T>
+__device__ __forceinline__ void ld16c(const cx<T> *p, cx<T> *out) {
+  ld16(reinterpret_cast<const T *>(p), reinterpret_cast<T *>(out));
+}
+template <typename T>
+__device__ __forceinline__ void st16c(cx<T> *p, const cx<T> *in) {
+  st16(reinterpret_cast<T *>(p), reinterpret_cast<const T *>(in));
+}
+
 // z forward: XB = 2P consecutive x rows at one y, as P complex sequences
 // (row 2p in the real part, row 2p+1 in the imaginary part).
-template <typename T, int E>
+template <typename T, int E, int LOGN>
 __global__ void __launch_bounds__(1024)
     k_fft_z_r2c(const int *__restrict__ act, const T *__restrict__ R, cx<T> *__restrict__ C, FftPlan pl,
                 const cx<T> *__restrict__ twg, int nx, int ny, int P,
-                FastDiv dXB) {
+                FastDiv dXB, FastDiv dP) {
   if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
-  const int n = pl.n, nh = n / 2 + 1, XB = 2 * P;
+  const int n = fft_len<LOGN>(pl), nh = n / 2 + 1, XB = 2 * P;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *a = tw + n;
   const int comp = blockIdx.y;
@@ -336,37 +393,67 @@ __global__ void __launch_bounds__(1024)
   const T *Rc = R + (size_t)comp * nx * ny * n;
   cx<T> *Cc = C + (size_t)comp * nh * ny * nx;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
-  for (int e = threadIdx.x; e < XB * n; e += blockDim.x) {
-    int i = pl.dn.q(e), z = e - i * n;
-    T v = Rc[((size_t)(x0 + i) * ny + y) * n + z];
-    if (i & 1)
-      a[(i >> 1) * pn + fft_pad(z)].y = v;
-    else
-      a[(i >> 1) * pn + fft_pad(z)].x = v;
-  }
-  __syncthreads();
-  lds_fft<T, E, true>(a, pl, P, 1, pn, tw, false);
   const T half = (T)0.5;
-  for (int e = threadIdx.x; e < nh * XB; e += blockDim.x) {
-    int kz = dXB.q(e), i = e - kz * XB;
-    int p = i >> 1;
-    cx<T> z = a[p * pn + fft_pad(kz)];
-    cx<T> w = a[p * pn + fft_pad(kz == 0 ? 0 : n - kz)];
-    cx<T> o = (i & 1) ? cx<T>{half * (z.y + w.y), half * (w.x - z.x)}
-                      : cx<T>{half * (z.x + w.x), half * (z.y - w.y)};
-    Cc[((size_t)kz * ny + y) * nx + x0 + i] = o;
+  if constexpr (LOGN > 0) {
+    // 16-byte loads of both rows of a sequence (VW consecutive z each);
+    // z is a multiple of VW <= 4, so the VW values share one pad group
+    constexpr int VW = Vec16<T>::n, LV = LOGN - Vec16<T>::logn;
+    for (int e = threadIdx.x; e < (P << LV); e += blockDim.x) {
+      const int p = e >> LV, z = (e & ((1 << LV) - 1)) * VW;
+      const T *r0 = Rc + ((size_t)(x0 + 2 * p) * ny + y) * n + z;
+      T u0[VW], u1[VW];
+      ld16(r0, u0);
+      ld16(r0 + (size_t)ny * n, u1);
+      cx<T> *dst = a + p * pn + fft_pad(z);
+#pragma unroll
+      for (int j = 0; j < VW; ++j) dst[j] = cx<T>{u0[j], u1[j]};
+    }
+    __syncthreads();
+    lds_fft<T, E, true, false, LOGN>(a, pl, P, 1, pn, tw);
+    // both rows of a sequence from the same two spectrum values, stored
+    // as one 16-byte pair (FP32) or two (FP64) of consecutive x
+    for (int e = threadIdx.x; e < nh * P; e += blockDim.x) {
+      const int kz = dP.q(e), p = e - kz * P;
+      const cx<T> z = a[p * pn + fft_pad(kz)];
+      const cx<T> w = a[p * pn + fft_pad((n - kz) & (n - 1))];
+      const T o[4] = {half * (z.x + w.x), half * (z.y - w.y),
+                      half * (z.y + w.y), half * (w.x - z.x)};
+      T *dst = reinterpret_cast<T *>(Cc + ((size_t)kz * ny + y) * nx + x0 + 2 * p);
+      st16(dst, o);
+      if constexpr (VW == 2) st16(dst + 2, o + 2);
+    }
+  } else {
+    for (int e = threadIdx.x; e < XB * n; e += blockDim.x) {
+      int i = div_len<LOGN>(e, pl), z = e - i * n;
+      T v = Rc[((size_t)(x0 + i) * ny + y) * n + z];
+      if (i & 1)
+        a[(i >> 1) * pn + fft_pad(z)].y = v;
+      else
+        a[(i >> 1) * pn + fft_pad(z)].x = v;
+    }
+    __syncthreads();
+    lds_fft<T, E, true, false, LOGN>(a, pl, P, 1, pn, tw);
+    for (int e = threadIdx.x; e < nh * XB; e += blockDim.x) {
+      int kz = dXB.q(e), i = e - kz * XB;
+      int p = i >> 1;
+      cx<T> z = a[p * pn + fft_pad(kz)];
+      cx<T> w = a[p * pn + fft_pad(kz == 0 ? 0 : n - kz)];
+      cx<T> o = (i & 1) ? cx<T>{half * (z.y + w.y), half * (w.x - z.x)}
+                        : cx<T>{half * (z.x + w.x), half * (z.y - w.y)};
+      Cc[((size_t)kz * ny + y) * nx + x0 + i] = o;
+    }
   }
 }
 
 // z inverse: the mirror of k_fft_z_r2c.
-template <typename T, int E>
+template <typename T, int E, int LOGN>
 __global__ void __launch_bounds__(1024)
     k_fft_z_c2r(const int *__restrict__ act, const cx<T> *__restrict__ C, T *__restrict__ R, FftPlan pl,
                 const cx<T> *__restrict__ twg, int nx, int ny, int P,
                 FastDiv dP) {
   if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
-  const int n = pl.n, nh = n / 2 + 1, XB = 2 * P;
+  const int n = fft_len<LOGN>(pl), nh = n / 2 + 1, XB = 2 * P;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *a = tw + n;
   const int comp = blockIdx.y;
@@ -379,7 +466,17 @@ __global__ void __launch_bounds__(1024)
   for (int e = threadIdx.x; e < nh * P; e += blockDim.x) {
     int k = dP.q(e), p = e - k * P;
     const cx<T> *src = Cc + ((size_t)k * ny + y) * nx + x0 + 2 * p;
-    cx<T> A = src[0], B = src[1];
+    cx<T> A, B;
+    if constexpr (LOGN > 0) {  // x0 + 2p even: the pair is 16-byte aligned
+      T u[4];
+      ld16(reinterpret_cast<const T *>(src), u);
+      if constexpr (Vec16<T>::n == 2) ld16(reinterpret_cast<const T *>(src) + 2, u + 2);
+      A = {u[0], u[1]};
+      B = {u[2], u[3]};
+    } else {
+      A = src[0];
+      B = src[1];
+    }
     if (k == 0 || 2 * k == n) {  // DC / Nyquist of a real line are real
       A.y = 0;
       B.y = 0;
@@ -388,22 +485,39 @@ __global__ void __launch_bounds__(1024)
     if (k > 0 && 2 * k < n) a[p * pn + fft_pad(n - k)] = {A.x + B.y, B.x - A.y};
   }
   __syncthreads();
-  lds_fft<T, E, true>(a, pl, P, 1, pn, tw, true);
-  for (int e = threadIdx.x; e < XB * n; e += blockDim.x) {
-    int i = pl.dn.q(e), z = e - i * n;
-    cx<T> v = a[(i >> 1) * pn + fft_pad(z)];
-    Rc[((size_t)(x0 + i) * ny + y) * n + z] = (i & 1) ? v.y : v.x;
+  lds_fft<T, E, true, true, LOGN>(a, pl, P, 1, pn, tw);
+  if constexpr (LOGN > 0) {  // 16-byte stores of both rows of a sequence
+    constexpr int VW = Vec16<T>::n, LV = LOGN - Vec16<T>::logn;
+    for (int e = threadIdx.x; e < (P << LV); e += blockDim.x) {
+      const int p = e >> LV, z = (e & ((1 << LV) - 1)) * VW;
+      const cx<T> *src = a + p * pn + fft_pad(z);
+      T u0[VW], u1[VW];
+#pragma unroll
+      for (int j = 0; j < VW; ++j) {
+        u0[j] = src[j].x;
+        u1[j] = src[j].y;
+      }
+      T *r0 = Rc + ((size_t)(x0 + 2 * p) * ny + y) * n + z;
+      st16(r0, u0);
+      st16(r0 + (size_t)ny * n, u1);
+    }
+  } else {
+    for (int e = threadIdx.x; e < XB * n; e += blockDim.x) {
+      int i = div_len<LOGN>(e, pl), z = e - i * n;
+      cx<T> v = a[(i >> 1) * pn + fft_pad(z)];
+      Rc[((size_t)(x0 + i) * ny + y) * n + z] = (i & 1) ? v.y : v.x;
+    }
   }
 }
 
 // y pass: one workgroup per (comp, kz, run of B consecutive x).
-template <typename T, int E>
+template <typename T, int E, int LOGN>
 __global__ void __launch_bounds__(1024)
     k_fft_y(const int *__restrict__ act, cx<T> *__restrict__ C, FftPlan pl, const cx<T> *__restrict__ twg,
             int nx, int nh, int B, int inverse, FastDiv dB) {
   if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
-  const int n = pl.n;  // ny
+  const int n = fft_len<LOGN>(pl);  // ny
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *a = tw + n;
   const int nxb = nx / B;
@@ -411,15 +525,36 @@ __global__ void __launch_bounds__(1024)
   const int comp = blockIdx.y;
   cx<T> *base = C + ((size_t)comp * nh + kz) * n * nx + x0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
-  for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    int m = dB.q(e), col = e - m * B;
-    a[e] = base[(size_t)m * nx + col];
+  // 16-byte accesses of VC consecutive x (x0 and B multiples of VC: the
+  // run starts 16-byte aligned)
+  constexpr int VC = kCx16<T>;
+  const bool vec = LOGN > 0 && B % VC == 0;
+  if (vec) {
+    for (int e = threadIdx.x * VC; e < B * n; e += blockDim.x * VC) {
+      int m = dB.q(e), col = e - m * B;
+      ld16c(base + (size_t)m * nx + col, a + e);
+    }
+  } else {
+    for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
+      int m = dB.q(e), col = e - m * B;
+      a[e] = base[(size_t)m * nx + col];
+    }
   }
   __syncthreads();
-  lds_fft<T, E>(a, pl, B, B, 1, tw, inverse != 0, dB);
-  for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    int m = dB.q(e), col = e - m * B;
-    base[(size_t)m * nx + col] = a[e];
+  if (inverse)
+    lds_fft<T, E, false, true, LOGN>(a, pl, B, B, 1, tw, dB);
+  else
+    lds_fft<T, E, false, false, LOGN>(a, pl, B, B, 1, tw, dB);
+  if (vec) {
+    for (int e = threadIdx.x * VC; e < B * n; e += blockDim.x * VC) {
+      int m = dB.q(e), col = e - m * B;
+      st16c(base + (size_t)m * nx + col, a + e);
+    }
+  } else {
+    for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
+      int m = dB.q(e), col = e - m * B;
+      base[(size_t)m * nx + col] = a[e];
+    }
   }
 }
 
@@ -431,14 +566,14 @@ struct ProjConst {
 
 // x pass + Stokes projection + inverse x pass: one workgroup per (kz, Q
 // consecutive y), the three components' contiguous x lines in LDS.
-template <typename T, int E>
+template <typename T, int E, int LOGN>
 __global__ void __launch_bounds__(1024)
     k_fft_x_project(const int *__restrict__ act, cx<T> *__restrict__ C, FftPlan pl,
                     const cx<T> *__restrict__ twg, ProjConst pc, int Q,
                     FastDiv dQ) {
   if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
-  const int n = pl.n;  // nx
+  const int n = fft_len<LOGN>(pl);  // nx
   const int nh = pc.nh;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *a = tw + n;  // [3][Q][padded n]
@@ -449,18 +584,27 @@ __global__ void __launch_bounds__(1024)
   cx<T> *base = C + ((size_t)kz * pc.ny + y0) * n;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
   const int nseq = 3 * Q;
-  for (int e = threadIdx.x; e < nseq * n; e += blockDim.x) {
-    int sq = pl.dn.q(e), x = e - sq * n;
-    int c = dQ.q(sq), ql = sq - c * Q;
-    a[sq * pn + fft_pad(x)] = base[c * comp_stride + (size_t)ql * n + x];
+  constexpr int VC = kCx16<T>;  // x even: VC values in one pad group
+  if constexpr (LOGN > 0) {
+    for (int e = threadIdx.x * VC; e < nseq * n; e += blockDim.x * VC) {
+      int sq = e >> LOGN, x = e & (n - 1);
+      int c = dQ.q(sq), ql = sq - c * Q;
+      ld16c(base + c * comp_stride + (size_t)ql * n + x, a + sq * pn + fft_pad(x));
+    }
+  } else {
+    for (int e = threadIdx.x; e < nseq * n; e += blockDim.x) {
+      int sq = div_len<LOGN>(e, pl), x = e - sq * n;
+      int c = dQ.q(sq), ql = sq - c * Q;
+      a[sq * pn + fft_pad(x)] = base[c * comp_stride + (size_t)ql * n + x];
+    }
   }
   __syncthreads();
-  lds_fft<T, E, true>(a, pl, nseq, 1, pn, tw, false);
+  lds_fft<T, E, true, false, LOGN>(a, pl, nseq, 1, pn, tw);
   // flow_solve (FCMfunctions.cpp:648-736)
   const T q3 = (T)(kz * pc.kfz);
   const T sc = (T)pc.scale;
   for (int e = threadIdx.x; e < Q * n; e += blockDim.x) {
-    int ql = pl.dn.q(e), x = e - ql * n;
+    int ql = div_len<LOGN>(e, pl), x = e - ql * n;
     int y = y0 + ql;
     T q1 = (T)(((x <= pc.nx / 2) ? x : x - pc.nx) * pc.kfx);
     T q2 = (T)(((y <= pc.ny / 2) ? y : y - pc.ny) * pc.kfy);
@@ -484,11 +628,19 @@ __global__ void __launch_bounds__(1024)
     a[(2 * Q + ql) * pn + px] = u3;
   }
   __syncthreads();
-  lds_fft<T, E, true>(a, pl, nseq, 1, pn, tw, true);
-  for (int e = threadIdx.x; e < nseq * n; e += blockDim.x) {
-    int sq = pl.dn.q(e), x = e - sq * n;
-    int c = dQ.q(sq), ql = sq - c * Q;
-    base[c * comp_stride + (size_t)ql * n + x] = a[sq * pn + fft_pad(x)];
+  lds_fft<T, E, true, true, LOGN>(a, pl, nseq, 1, pn, tw);
+  if constexpr (LOGN > 0) {
+    for (int e = threadIdx.x * VC; e < nseq * n; e += blockDim.x * VC) {
+      int sq = e >> LOGN, x = e & (n - 1);
+      int c = dQ.q(sq), ql = sq - c * Q;
+      st16c(base + c * comp_stride + (size_t)ql * n + x, a + sq * pn + fft_pad(x));
+    }
+  } else {
+    for (int e = threadIdx.x; e < nseq * n; e += blockDim.x) {
+      int sq = div_len<LOGN>(e, pl), x = e - sq * n;
+      int c = dQ.q(sq), ql = sq - c * Q;
+      base[c * comp_stride + (size_t)ql * n + x] = a[sq * pn + fft_pad(x)];
+    }
   }
 }
 
@@ -635,6 +787,48 @@ static bool with_E(int E, F f) {
 
 static int largest_divisor_le(int n, int t);
 
+// Register tile E and compile-time length of an FFT plan's kernels: n = 2^LOGN
+// for n in {128, 256, 512} with E = 8 or 16, provided the plan's radices are
+// those of the compile-time rule (p2_radix); otherwise LOGN = 0 (run-time
+// plan).
+static bool p2_plan_matches(const FftPlan &pl, int logn) {
+  int ls = 0, s = 0;
+  while (ls < logn) {
+    int rem = logn - ls;
+    int r = (rem >= 4 && pl.E >= 16) ? 16 : (rem >= 3 && pl.E >= 8) ? 8
+            : (rem >= 2)                ? 4
+                                        : 2;
+    if (s >= pl.nst || pl.rad[s] != r) return false;
+    ++s;
+    ls += (r == 16 ? 4 : r == 8 ? 3 : r == 4 ? 2 : 1);
+  }
+  return s == pl.nst;
+}
+
+template <typename F>
+static bool with_EL(const FftPlan &pl, F f) {
+  using L0 = std::integral_constant<int, 0>;
+  int logn = 0;
+  if (pl.n == 128) logn = 7;
+  if (pl.n == 256) logn = 8;
+  if (pl.n == 512) logn = 9;
+  if (logn && (pl.E == 8 || pl.E == 16) && p2_plan_matches(pl, logn)) {
+    auto go = [&](auto e) {
+      switch (logn) {
+        case 7: f(e, std::integral_constant<int, 7>{}); break;
+        case 8: f(e, std::integral_constant<int, 8>{}); break;
+        default: f(e, std::integral_constant<int, 9>{}); break;
+      }
+    };
+    if (pl.E == 8)
+      go(std::integral_constant<int, 8>{});
+    else
+      go(std::integral_constant<int, 16>{});
+    return true;
+  }
+  return with_E(pl.E, [&](auto e) { f(e, L0{}); });
+}
+
 template <typename F>
 static bool with_TZ(int tz, F f) {
   switch (tz) {
@@ -668,21 +862,21 @@ static int set_fft_lds_attr(const FftPlan &plx, const FftPlan &ply,
                             const FftPlan &plz) {
   const int lds_max = 160 * 1024;
   bool ok = true;
-  ok &= with_E(plz.E, [&](auto e) {
-    constexpr int EE = decltype(e)::value;
-    (void)hipFuncSetAttribute((const void *)k_fft_z_r2c<T, EE>,
+  ok &= with_EL(plz, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    (void)hipFuncSetAttribute((const void *)k_fft_z_r2c<T, EE, LL>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-    (void)hipFuncSetAttribute((const void *)k_fft_z_c2r<T, EE>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-  });
-  ok &= with_E(ply.E, [&](auto e) {
-    constexpr int EE = decltype(e)::value;
-    (void)hipFuncSetAttribute((const void *)k_fft_y<T, EE>,
+    (void)hipFuncSetAttribute((const void *)k_fft_z_c2r<T, EE, LL>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
   });
-  ok &= with_E(plx.E, [&](auto e) {
-    constexpr int EE = decltype(e)::value;
-    (void)hipFuncSetAttribute((const void *)k_fft_x_project<T, EE>,
+  ok &= with_EL(ply, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    (void)hipFuncSetAttribute((const void *)k_fft_y<T, EE, LL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  });
+  ok &= with_EL(plx, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    (void)hipFuncSetAttribute((const void *)k_fft_x_project<T, EE, LL>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
   });
   FS_CHECK(ok, FS_ERR_ARG, "no FFT register tile for these sizes");
@@ -932,17 +1126,17 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   mark(2);
   const int nh = p.nz / 2 + 1;
   const size_t csz = sizeof(cx<T>);
-  bool ok = with_E(plz.E, [&](auto e) {
-    constexpr int EE = decltype(e)::value;
-    hipLaunchKernelGGL((k_fft_z_r2c<T, EE>), dim3(p.ny * (p.nx / (2 * zP)), 3),
+  bool ok = with_EL(plz, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    hipLaunchKernelGGL((k_fft_z_r2c<T, EE, LL>), dim3(p.ny * (p.nx / (2 * zP)), 3),
                        dim3(p.nz / EE * zP),
                        csz * (p.nz + (size_t)zP * fft_padded_len(p.nz)), st, act,
                        R.as<T>(), C.as<cx<T>>(), plz, twz.as<cx<T>>(), p.nx,
-                       p.ny, zP, make_div(2 * zP));
+                       p.ny, zP, make_div(2 * zP), make_div(zP));
   });
-  ok &= with_E(ply.E, [&](auto e) {
-    constexpr int EE = decltype(e)::value;
-    hipLaunchKernelGGL((k_fft_y<T, EE>), dim3(nh * (p.nx / yB), 3),
+  ok &= with_EL(ply, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    hipLaunchKernelGGL((k_fft_y<T, EE, LL>), dim3(nh * (p.nx / yB), 3),
                        dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st, act,
                        C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 0,
                        make_div(yB));
@@ -959,9 +1153,9 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   pc.kfy = 2.0 * kPi / (p.ny * p.dx);
   pc.kfz = 2.0 * kPi / (p.nz * p.dx);
   pc.scale = 1.0 / ((double)p.nx * p.ny * p.nz * p.mu);
-  ok = with_E(plx.E, [&](auto e) {
-    constexpr int EE = decltype(e)::value;
-    hipLaunchKernelGGL((k_fft_x_project<T, EE>), dim3(nh * (p.ny / xQ)),
+  ok = with_EL(plx, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    hipLaunchKernelGGL((k_fft_x_project<T, EE, LL>), dim3(nh * (p.ny / xQ)),
                        dim3(p.nx / EE * 3 * xQ),
                        csz * (p.nx + 3 * (size_t)xQ * fft_padded_len(p.nx)), st, act,
                        C.as<cx<T>>(),
@@ -970,16 +1164,16 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
   mark(4);
-  ok = with_E(ply.E, [&](auto e) {
-    constexpr int EE = decltype(e)::value;
-    hipLaunchKernelGGL((k_fft_y<T, EE>), dim3(nh * (p.nx / yB), 3),
+  ok = with_EL(ply, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    hipLaunchKernelGGL((k_fft_y<T, EE, LL>), dim3(nh * (p.nx / yB), 3),
                        dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st, act,
                        C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 1,
                        make_div(yB));
   });
-  ok &= with_E(plz.E, [&](auto e) {
-    constexpr int EE = decltype(e)::value;
-    hipLaunchKernelGGL((k_fft_z_c2r<T, EE>), dim3(p.ny * (p.nx / (2 * zP)), 3),
+  ok &= with_EL(plz, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    hipLaunchKernelGGL((k_fft_z_c2r<T, EE, LL>), dim3(p.ny * (p.nx / (2 * zP)), 3),
                        dim3(p.nz / EE * zP),
                        csz * (p.nz + (size_t)zP * fft_padded_len(p.nz)), st, act,
                        C.as<cx<T>>(), R.as<T>(), plz, twz.as<cx<T>>(), p.nx,

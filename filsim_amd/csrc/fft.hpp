@@ -226,8 +226,10 @@ __device__ __forceinline__ void dftR(cx<T> *v, bool inv) {
 // (d = j*R + r); unpadded, 16 (FP32) or 32 (FP64) lanes of a wave would hit
 // one bank. One pad element per 8 spreads those strided writes over the
 // banks, and contiguous reads stay conflict-free.
-__host__ __device__ __forceinline__ int fft_pad(int i) { return i + (i >> 3); }
-__host__ __device__ __forceinline__ int fft_padded_len(int n) {
+__host__ __device__ __forceinline__ constexpr int fft_pad(int i) {
+  return i + (i >> 3);
+}
+__host__ __device__ __forceinline__ constexpr int fft_padded_len(int n) {
   return fft_pad(n - 1) + 1;
 }
 
@@ -239,11 +241,12 @@ __device__ __forceinline__ int lds_idx(int i, int col, int sm, int sc) {
 }
 
 // One in-place stage; blockDim.x * (E/R) == (n/R) * ncols exactly.
-template <typename T, int R, int E, bool PAD>
+template <typename T, int R, int E, bool PAD, bool INV>
 __device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int ns, int ncols,
                                           int sm, int sc, const cx<T> *tw,
-                                          bool inv, FastDiv dnb, FastDiv dns,
+                                          FastDiv dnb, FastDiv dns,
                                           FastDiv dcols) {
+  constexpr bool inv = INV;
   constexpr int NB = E / R;
   const int nb = n / R;
   const int twstep = n / (ns * R);
@@ -292,34 +295,152 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int ns, int ncols,
 
 // All stages of the plan, in place in buf. Needs blockDim.x = n*ncols/E.
 // PAD: contiguous padded sequences (sm = 1, sc = fft_padded_len(n)).
-template <typename T, int E, bool PAD = false>
-__device__ void lds_fft(cx<T> *buf, const FftPlan &pl, int ncols, int sm,
-                        int sc, const cx<T> *tw, bool inv,
-                        FastDiv dcols = FastDiv{0, 0}) {
+template <typename T, int E, bool PAD, bool INV>
+__device__ void lds_fft_plan(cx<T> *buf, const FftPlan &pl, int ncols, int sm,
+                             int sc, const cx<T> *tw, FastDiv dcols) {
   int ns = 1;
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.rad[s];
     const FastDiv a = pl.dnb[s], b = pl.dns[s];
     if constexpr (E % 16 == 0)
       if (R == 16)
-        fft_stage<T, 16, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
+        fft_stage<T, 16, E, PAD, INV>(buf, pl.n, ns, ncols, sm, sc, tw, a, b, dcols);
     if constexpr (E % 8 == 0)
       if (R == 8)
-        fft_stage<T, 8, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
+        fft_stage<T, 8, E, PAD, INV>(buf, pl.n, ns, ncols, sm, sc, tw, a, b, dcols);
     if constexpr (E % 4 == 0)
       if (R == 4)
-        fft_stage<T, 4, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
+        fft_stage<T, 4, E, PAD, INV>(buf, pl.n, ns, ncols, sm, sc, tw, a, b, dcols);
     if constexpr (E % 2 == 0)
       if (R == 2)
-        fft_stage<T, 2, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
+        fft_stage<T, 2, E, PAD, INV>(buf, pl.n, ns, ncols, sm, sc, tw, a, b, dcols);
     if constexpr (E % 3 == 0)
       if (R == 3)
-        fft_stage<T, 3, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
+        fft_stage<T, 3, E, PAD, INV>(buf, pl.n, ns, ncols, sm, sc, tw, a, b, dcols);
     if constexpr (E % 5 == 0)
       if (R == 5)
-        fft_stage<T, 5, E, PAD>(buf, pl.n, ns, ncols, sm, sc, tw, inv, a, b, dcols);
+        fft_stage<T, 5, E, PAD, INV>(buf, pl.n, ns, ncols, sm, sc, tw, a, b, dcols);
     ns *= R;
   }
+}
+
+// --- n = 2^LOGN with the plan fixed at compile time ------------------------
+// Same stages as make_plan (largest radix first, at most E points per
+// butterfly), so the same arithmetic; spans, butterfly counts, twiddle steps
+// and padded offsets are constants: divisions become shifts and masks, and
+// the R addresses of a butterfly are one base plus immediate offsets.
+
+constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v / 2); }
+
+template <int LOGN, int LOGNS, int E>
+constexpr int p2_radix() {
+  return (LOGN - LOGNS >= 4 && E >= 16)  ? 16
+         : (LOGN - LOGNS >= 3 && E >= 8) ? 8
+         : (LOGN - LOGNS >= 2)           ? 4
+                                         : 2;
+}
+
+template <typename T, int E, bool PAD, bool INV, int LOGN, int LOGNS>
+__device__ __forceinline__ void p2_stage(cx<T> *buf, int ncols, int sm,
+                                         const cx<T> *tw, FastDiv dcols) {
+  constexpr int R = p2_radix<LOGN, LOGNS, E>();
+  constexpr int LOGR = ilog2_c(R);
+  constexpr int NB = E / R;
+  constexpr int LOGNB = LOGN - LOGR;
+  constexpr int nb = 1 << LOGNB;
+  constexpr int ns = 1 << LOGNS;
+  constexpr int LOGTW = LOGN - LOGNS - LOGR;  // twiddle step n / (ns R)
+  constexpr int PN = fft_padded_len(1 << LOGN);
+  static_assert(E % R == 0, "register tile must hold whole butterflies");
+  cx<T> v[E];
+  int jj[NB], cc[NB];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    const int t = threadIdx.x + m * blockDim.x;
+    int j, col;
+    if constexpr (PAD) {  // sequences contiguous: butterflies fastest
+      col = t >> LOGNB;
+      j = t & (nb - 1);
+    } else {  // sequences interleaved (element m of column c at m*sm + c)
+      j = dcols.q(t);
+      col = t - j * ncols;
+    }
+    jj[m] = j;
+    cc[m] = col;
+    const int k = j & (ns - 1);
+    if constexpr (PAD) {
+      const cx<T> *src = buf + col * PN;
+      const int pj = fft_pad(j);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        int idx;
+        if constexpr (nb % 8 == 0)
+          idx = pj + fft_pad(r * nb);
+        else
+          idx = fft_pad(j + r * nb);
+        v[m * R + r] = src[idx];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[m * R + r] = buf[(j + r * nb) * sm + col];
+    }
+    if constexpr (LOGNS > 0) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cx<T> w = tw[(r * k) << LOGTW];
+        if constexpr (INV) w.y = -w.y;
+        v[m * R + r] = cmul(v[m * R + r], w);
+      }
+    }
+    dftR<T, R>(&v[m * R], INV);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    const int j = jj[m], col = cc[m];
+    const int d = ((j >> LOGNS) << (LOGNS + LOGR)) + (j & (ns - 1));
+    if constexpr (PAD) {
+      cx<T> *dst = buf + col * PN;
+      const int pd = fft_pad(d);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        int idx;
+        if constexpr (ns % 8 == 0)
+          idx = pd + fft_pad(r * ns);
+        else
+          idx = fft_pad(d + r * ns);
+        dst[idx] = v[m * R + r];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(d + r * ns) * sm + col] = v[m * R + r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int E, bool PAD, bool INV, int LOGN, int LOGNS>
+__device__ __forceinline__ void p2_stages(cx<T> *buf, int ncols, int sm,
+                                          const cx<T> *tw, FastDiv dcols) {
+  if constexpr (LOGNS < LOGN) {
+    p2_stage<T, E, PAD, INV, LOGN, LOGNS>(buf, ncols, sm, tw, dcols);
+    constexpr int LOGR = ilog2_c(p2_radix<LOGN, LOGNS, E>());
+    p2_stages<T, E, PAD, INV, LOGN, LOGNS + LOGR>(buf, ncols, sm, tw, dcols);
+  }
+}
+
+// The transform for kernels instantiated on LOGN: LOGN > 0 takes the
+// compile-time plan (n = 2^LOGN, PAD: sc = fft_padded_len(n)), LOGN = 0 the
+// run-time plan pl.
+template <typename T, int E, bool PAD, bool INV, int LOGN>
+__device__ __forceinline__ void lds_fft(cx<T> *buf, const FftPlan &pl,
+                                        int ncols, int sm, int sc,
+                                        const cx<T> *tw,
+                                        FastDiv dcols = FastDiv{0, 0}) {
+  if constexpr (LOGN > 0)
+    p2_stages<T, E, PAD, INV, LOGN, 0>(buf, ncols, sm, tw, dcols);
+  else
+    lds_fft_plan<T, E, PAD, INV>(buf, pl, ncols, sm, sc, tw, dcols);
 }
 
 }  // namespace fs

@@ -21,6 +21,11 @@ CASES = [
     ((24, 24, 24), 20, "ref"),
     # 512-point lines (the 512^3 suspension's FFT length)
     ((512, 16, 24), 12, None),
+    # 128/256/512-point lines take compile-time power-of-two plans
+    ((256, 128, 32), 12, None),
+    ((32, 256, 128), 12, None),
+    ((16, 24, 256), 12, None),
+    ((24, 16, 512), 12, None),
 ]
 
 
@@ -42,8 +47,9 @@ def test_mobility_fp64_matches_oracle(cuda, grid, ngd, dx):
     assert rel(W, Wo) <= 1e-12
 
 
-def test_mobility_fp32_error_bound(cuda):
-    fp = params.fcm_params(64)
+@pytest.mark.parametrize("grid", [64, (256, 128, 64), (512, 32, 48)])
+def test_mobility_fp32_error_bound(cuda, grid):
+    fp = params.fcm_params(grid)
     Y, F, T = random_segments(fp, 400, seed=5)
     Vo, Wo = O.mobility(fp, Y, F, T)
     with api.Fcm(fp, 400, precision=abi.FS_FP32) as fcm:
