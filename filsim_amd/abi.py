@@ -135,6 +135,9 @@ PROTOS = {
                         C.POINTER(SimParams), C.c_double, C.c_double,
                         C.c_int, C.c_int],
     "fs_io_read_par": [C.c_char_p, IP, IP, C.c_int],
+    "fs_system_set_spring_links": [P, C.c_int, IP, C.c_double, C.c_double],
+    "fs_io_write_spring_links": [C.c_char_p, C.c_int, IP],
+    "fs_io_read_spring_links": [C.c_char_p, IP, IP, C.c_int],
     "fs_io_write_checkpoint": [C.c_char_p, C.c_int, C.c_int, IP,
                                C.POINTER(StatePtrs)],
     "fs_io_read_checkpoint": [C.c_char_p, C.c_int, IP, C.POINTER(StatePtrs),

@@ -291,6 +291,15 @@ class System:
     def set_profiling(self, on=True):
         abi.check(self.lib.fs_system_set_profiling(self.h, 1 if on else 0))
 
+    def set_spring_links(self, links, k, natural_length):
+        """Spring links as (n, 4) rows (filament A, bead B, filament C,
+        bead D) with spring constant k and natural length
+        (fs_system_set_spring_links); an empty array removes them."""
+        a = np.ascontiguousarray(links, dtype=np.int32).reshape(-1, 4)
+        abi.check(self.lib.fs_system_set_spring_links(
+            self.h, len(a), abi.iptr(a) if len(a) else None, float(k),
+            float(natural_length)))
+
     def profile(self):
         """ms per bucket of the last profiled timestep (fs_system_profile)."""
         ms = np.zeros(len(self.PROFILE_BUCKETS))
@@ -513,6 +522,25 @@ def io_read_par(path):
     abi.check(lib.fs_io_read_par(path.encode(), C.byref(nf), abi.iptr(nw),
                                  len(nw)))
     return [int(v) for v in nw]
+
+
+def io_write_spring_links(path, links):
+    lib = abi.load()
+    a = np.ascontiguousarray(links, dtype=np.int32).reshape(-1, 4)
+    abi.check(lib.fs_io_write_spring_links(path.encode(), len(a),
+                                           abi.iptr(a) if len(a) else None))
+
+
+def io_read_spring_links(path):
+    """(n, 4) int array of (filament A, bead B, filament C, bead D)."""
+    lib = abi.load()
+    n = C.c_int()
+    abi.check(lib.fs_io_read_spring_links(path.encode(), C.byref(n), None, 0))
+    a = np.zeros((n.value, 4), dtype=np.int32)
+    if n.value:
+        abi.check(lib.fs_io_read_spring_links(path.encode(), C.byref(n),
+                                              abi.iptr(a), n.value))
+    return a
 
 
 def io_write_checkpoint(path, nt, nws, state):

@@ -339,6 +339,45 @@ __global__ void k_forces(SysDev S, int nt) {
   }
 }
 
+// Spring links (spring_links::apply_spring_link_forces,
+// spring_link_functions.cpp:24-45), called after the filament forces
+// (main.cpp:354-357, 540-543) and before assign_filament_data. One thread
+// per linked bead adds its links' forces in link order, the order in which
+// the reference accumulates them: +f on the first bead of a link, -f on the
+// second, f = -k (|d| - L) d / |d| with d = VecPeriodicX(Xs_a, Xs_b)
+// (spring_link_functions.cpp:131-138). ent[e] = 2 * link + (second bead).
+__global__ void k_springs(SysDev S, int nb, const int *__restrict__ bead,
+                          const int *__restrict__ start,
+                          const int *__restrict__ ent,
+                          const int *__restrict__ pair, double k, double len) {
+#pragma clang fp contract(off)
+  if (S.act && !*S.act) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int g = bead[i];
+  const double Lb[3] = {S.Lx, S.Ly, S.Lz};
+  double F[3] = {S.F[3 * g], S.F[3 * g + 1], S.F[3 * g + 2]};
+  for (int e = start[i]; e < start[i + 1]; ++e) {
+    const int l = ent[e] >> 1, second = ent[e] & 1;
+    const int a = pair[2 * l], b = pair[2 * l + 1];
+    double dv[3];
+    for (int c = 0; c < 3; ++c) {
+      dv[c] = S.Xs[3 * a + c] - S.Xs[3 * b + c];
+      dv[c] = dv[c] - Lb[c] * floor(dv[c] / Lb[c] + 0.5);
+    }
+    const double dist = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    const double mag = k * (dist - len);
+    for (int c = 0; c < 3; ++c) {
+      const double f = -mag * dv[c] / dist;
+      F[c] = second ? F[c] - f : F[c] + f;
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    S.F[3 * g + c] = F[c];
+    S.Fa[3 * g + c] = F[c];
+  }
+}
+
 // ---------------------------------------------------------------------
 // residual (ErrorCheckFCM, MobilityConstraints.cpp:447-770)
 // ---------------------------------------------------------------------
@@ -1352,6 +1391,7 @@ int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
 
 void SystemImpl::release() {
   if (st) (void)hipStreamSynchronize(st);
+  springb.release();
   arena.release();
   ints.release();
   joffb.release();
@@ -1446,7 +1486,73 @@ int SystemImpl::initial_guess() {
 int SystemImpl::forces(int nt) {
   ProfScope ps_(*this, P_ASSIGN);
   hipLaunchKernelGGL(k_forces, dim3(nf), dim3(64), 0, st, d, nt);
+  if (spring_nb > 0) {
+    const int *sb = springb.as<int>();
+    hipLaunchKernelGGL(k_springs, dim3(ceil_div(spring_nb, 64)), dim3(64), 0,
+                       st, d, spring_nb, sb, sb + spring_nb,
+                       sb + 2 * spring_nb + 1, sb + 2 * spring_nb + 1 + 2 * nspring,
+                       spring_k, spring_len);
+  }
   FS_LAUNCHED();
+  return FS_OK;
+}
+
+// Spring links as (filament, bead, filament, bead) rows; per linked bead
+// the list of its links in link order, uploaded as one int buffer:
+// bead[nb] | start[nb + 1] | ent[2 n] | pair[2 n].
+int SystemImpl::set_spring_links(int n, const int *links, double k,
+                                 double len) {
+  FS_CHECK(n >= 0 && (n == 0 || links), FS_ERR_ARG, "bad spring links");
+  std::vector<int> pair(2 * (size_t)n);
+  for (int l = 0; l < n; ++l)
+    for (int e = 0; e < 2; ++e) {
+      const int f = links[4 * l + 2 * e], b = links[4 * l + 2 * e + 1];
+      FS_CHECK(f >= 0 && f < nf && b >= 0 && b < hnw[f], FS_ERR_ARG,
+               "spring link %d: bead (%d, %d) is not in the system", l, f, b);
+      pair[2 * l + e] = hboff[f] + b;
+    }
+  for (int l = 0; l < n; ++l)
+    FS_CHECK(pair[2 * l] != pair[2 * l + 1], FS_ERR_ARG,
+             "spring link %d joins a bead to itself", l);
+  // entries sorted by bead, in link order within a bead
+  std::vector<std::pair<int, int>> es;  // (bead, 2 link + second)
+  es.reserve(2 * (size_t)n);
+  for (int l = 0; l < n; ++l) {
+    es.push_back({pair[2 * l], 2 * l});
+    es.push_back({pair[2 * l + 1], 2 * l + 1});
+  }
+  std::stable_sort(es.begin(), es.end(),
+                   [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                     return x.first < y.first;
+                   });
+  std::vector<int> bead, start, ent;
+  for (size_t e = 0; e < es.size(); ++e) {
+    if (e == 0 || es[e].first != es[e - 1].first) {
+      bead.push_back(es[e].first);
+      start.push_back((int)e);
+    }
+    ent.push_back(es[e].second);
+  }
+  start.push_back((int)es.size());
+  const int nb = (int)bead.size();
+  std::vector<int> buf;
+  buf.insert(buf.end(), bead.begin(), bead.end());
+  buf.insert(buf.end(), start.begin(), start.end());
+  buf.insert(buf.end(), ent.begin(), ent.end());
+  buf.insert(buf.end(), pair.begin(), pair.end());
+  FS_HIP(hipStreamSynchronize(st));
+  springb.release();
+  spring_nb = 0;
+  nspring = 0;
+  if (n > 0) {
+    FS_TRY(springb.alloc(sizeof(int) * buf.size()));
+    FS_HIP(hipMemcpy(springb.p, buf.data(), sizeof(int) * buf.size(),
+                     hipMemcpyHostToDevice));
+  }
+  spring_nb = nb;
+  nspring = n;
+  spring_k = k;
+  spring_len = len;
   return FS_OK;
 }
 

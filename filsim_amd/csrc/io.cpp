@@ -229,6 +229,50 @@ extern "C" int fs_io_read_par(const char *path, int *nf, int *nw, int cap) {
   return FS_OK;
 }
 
+// ---- spring links (spring_links::save_spring_links_to_file and
+// read_spring_links_from_file, spring_link_functions.cpp:50-101): a header
+// line, the link count, then one "A B C D" row per link (filament A bead B
+// to filament C bead D).
+extern "C" int fs_io_write_spring_links(const char *path, int n,
+                                        const int *links) {
+  FS_CHECK(path && n >= 0 && (n == 0 || links), FS_ERR_ARG, "bad argument");
+  File o(path, "w");
+  FS_CHECK(o.f, FS_ERR_ARG, "cannot open %s", path);
+  std::fprintf(o.f,
+               "Spring link data file. Number of links written first, then "
+               "springs between {filament A, bead B} and {filament C, bead D} "
+               "in the form 'A B C D'.\n%d\n",
+               n);
+  for (int l = 0; l < n; ++l)
+    std::fprintf(o.f, "%d %d %d %d\n", links[4 * l], links[4 * l + 1],
+                 links[4 * l + 2], links[4 * l + 3]);
+  FS_CHECK(!std::ferror(o.f), FS_ERR_ARG, "write to %s failed", path);
+  return FS_OK;
+}
+
+// links may be NULL to query the count.
+extern "C" int fs_io_read_spring_links(const char *path, int *n, int *links,
+                                       int cap) {
+  FS_CHECK(path && n, FS_ERR_ARG, "null argument");
+  File in(path, "r");
+  FS_CHECK(in.f, FS_ERR_ARG, "cannot open %s", path);
+  int c;
+  while ((c = std::fgetc(in.f)) != EOF && c != '\n') {
+  }
+  int count = -1;
+  FS_CHECK(c == '\n' && std::fscanf(in.f, "%d", &count) == 1 && count >= 0,
+           FS_ERR_ARG, "%s: bad link count", path);
+  *n = count;
+  if (!links) return FS_OK;
+  FS_CHECK(cap >= count, FS_ERR_SIZE, "%s: %d links, room for %d", path, count,
+           cap);
+  for (int l = 0; l < count; ++l)
+    FS_CHECK(std::fscanf(in.f, "%d %d %d %d", links + 4 * l, links + 4 * l + 1,
+                         links + 4 * l + 2, links + 4 * l + 3) == 4,
+             FS_ERR_ARG, "%s: bad link %d", path, l);
+  return FS_OK;
+}
+
 // ---- binary checkpoint: "FSCK", version, nt, nf, np, nlam, nw[nf], then
 // X Xs Xt U Us Ut (np*3 each), q qs qt (np*4 each), lam lam1 lam2 (nlam*3
 // each) as raw little-endian doubles

@@ -141,6 +141,13 @@ extern "C" int fs_system_timestep(fs_system *h, int nt, int *iters) {
   return FS_OK;
 }
 
+extern "C" int fs_system_set_spring_links(fs_system *h, int n,
+                                          const int *links, double k,
+                                          double natural_length) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.set_spring_links(n, links, k, natural_length);
+}
+
 extern "C" int fs_system_set_profiling(fs_system *h, int on) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
   FS_HIP(hipStreamSynchronize(h->impl.st));

@@ -66,6 +66,12 @@ struct SystemImpl {
   int broyden_loop(int nt, int *iters);
   int check0 = 0;
 
+  // spring links between beads (spring_link_functions.cpp:24-45)
+  dbuf springb;
+  int nspring = 0, spring_nb = 0;
+  double spring_k = 0, spring_len = 0;
+  int set_spring_links(int n, const int *links, double k, double len);
+
   // --- profiling: the reference profiler's buckets (profilers.hpp:24-57),
   // accumulated per timestep from HIP event pairs on the system's stream
   enum {

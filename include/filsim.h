@@ -300,6 +300,21 @@ int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best);
 /* Set the launch shapes of the system's FCM solver (fs_fcm_set_tuning). */
 int fs_system_set_tuning(fs_system *h, const fs_fcm_tuning *t);
 
+/* ---- Spring-link networks (struct spring_links, spring_link_functions.hpp:
+ * 18-39). n links as (filament A, bead B, filament C, bead D) rows; each
+ * timestep's forces (main.cpp:354-357, 540-543) add, in link order,
+ * f = -k (|d| - L) d / |d| to bead (A, B) and -f to bead (C, D), with d the
+ * minimum-image difference of their previous positions Xs
+ * (apply_spring_link_forces, spring_link_functions.cpp:24-45). n = 0
+ * removes them. Set them before a batch is created from the system. The
+ * file functions keep the reference's "-springlinks.dat" layout
+ * (spring_link_functions.cpp:50-101); read with links = NULL for the count.
+ */
+int fs_system_set_spring_links(fs_system *h, int n, const int *links, double k,
+                               double natural_length);
+int fs_io_write_spring_links(const char *path, int n, const int *links);
+int fs_io_read_spring_links(const char *path, int *n, int *links, int cap);
+
 /* ---- Trajectory output and restart (file_functions.cpp, Filament.hpp
  * writeData/writeBackupData/continue_from_checkpoint). Text files keep the
  * reference's layout (space separated, fixed notation, 16 decimals):

@@ -9,6 +9,7 @@
 //                        640-663, 704-707 (FilamentJacobianSolver.hpp,
 //                        Filament.hpp), batched over every filament
 //   filsim::ErrorCheckFCM (MobilityConstraints.cpp:447)
+//   filsim::spring_links struct spring_links (spring_link_functions.hpp:18-39)
 //
 // Differences from the reference, all forced by running on one GPU:
 //   - sizes are runtime parameters (fs_fcm_params / fs_sim_params) instead
@@ -189,6 +190,55 @@ struct Filaments {
   void free_memory() {
     if (h) fs_system_destroy(h);
     h = nullptr;
+  }
+};
+
+// struct spring_links: the link table {A, B, C, D} (filament A bead B to
+// filament C bead D) and its "-springlinks.dat" file. The reference calls
+// apply_spring_link_forces after every force assembly (main.cpp:354-357,
+// 540-543); here attach() hands the table to the device once and
+// Filaments::applyForcesTorques / timestep apply it from then on.
+struct spring_links {
+  std::vector<std::vector<int>> links;
+
+  int size() const { return (int)links.size(); }
+  void read_spring_links_from_file(const std::string &path) {
+    int n = 0;
+    check(fs_io_read_spring_links(path.c_str(), &n, nullptr, 0),
+          "read_spring_links_from_file");
+    std::vector<int> flat(4 * (size_t)n);
+    if (n > 0)
+      check(fs_io_read_spring_links(path.c_str(), &n, flat.data(), n),
+            "read_spring_links_from_file");
+    links.assign(n, std::vector<int>(4));
+    for (int l = 0; l < n; ++l)
+      for (int c = 0; c < 4; ++c) links[l][c] = flat[4 * l + c];
+  }
+  void save_spring_links_to_file(const std::string &path) const {
+    std::vector<int> flat = this->flat();
+    check(fs_io_write_spring_links(path.c_str(), size(), flat.data()),
+          "save_spring_links_to_file");
+  }
+  // NetworkSpringConstant, NetworkSpringNaturalLength (config_template.hpp:
+  // 117-118)
+  void attach(Filaments &f, double k, double natural_length) const {
+    std::vector<int> flat = this->flat();
+    check(fs_system_set_spring_links(f.h, size(), flat.data(), k,
+                                     natural_length),
+          "spring_links::attach");
+  }
+  // Same as attach: the forces are then added in every force assembly.
+  void apply_spring_link_forces(Filaments &f, double k,
+                                double natural_length) const {
+    attach(f, k, natural_length);
+  }
+
+ private:
+  std::vector<int> flat() const {
+    std::vector<int> v;
+    v.reserve(4 * links.size());
+    for (const auto &r : links) v.insert(v.end(), r.begin(), r.begin() + 4);
+    return v;
   }
 };
 

@@ -15,6 +15,7 @@
  *                           main.cpp:494-687
  *   timestep                main.cpp:284-754
  *   assign_filament_data    FCMfunctions.cpp:882-918
+ *   spring links            spring_link_functions.cpp:24-45, 131-138
  * arma::inv (Filament.hpp:862, LAPACK getrf/getri) is replaced by an LU
  * with partial pivoting restated from the textbook algorithm.
  *
@@ -26,7 +27,9 @@
  * differences); the residual vanishes on exact friction-mobility
  * translation; J * J^-1 = I; the Broyden secant condition; the analytic
  * Jacobian blocks against finite differences of the residual (the parts
- * where the approximation is exact); whole steps converge to TOL.
+ * where the approximation is exact); whole steps converge to TOL; spring
+ * forces against Hooke's law on hand-placed beads, across the periodic
+ * boundary, equal and opposite.
  * Parity unpinned: the exact numbers of the approximate BlockE terms (no
  * reference output exists to compare against; pinned only by re-derivation
  * from the source text and by Broyden convergence).
@@ -270,6 +273,10 @@ struct or_sim {
   double *E, *NewE, *Upd, *tmp, *dE;
   or_colbar *cb;
   int *sfil, *sseg;
+  /* spring links: global bead pairs, in link order */
+  int nspring;
+  int *spring; /* 2 per link */
+  double spring_k, spring_len;
 };
 
 static double *dz(size_t n) { return (double *)calloc(n ? n : 1, 8); }
@@ -519,6 +526,48 @@ static void constraint_forces(or_sim *h, int f) {
   }
 }
 
+/* spring_links (spring_link_functions.cpp:24-45): links between bead
+ * (filament A, bead B) and (filament C, bead D); the global bead pairs are
+ * kept. */
+int or_sim_set_spring_links(or_sim *h, int n, const int *links, double k,
+                            double len) {
+  free(h->spring);
+  h->spring = (int *)malloc(sizeof(int) * 2 * (n > 0 ? n : 1));
+  for (int l = 0; l < n; ++l)
+    for (int e = 0; e < 2; ++e) {
+      int f = links[4 * l + 2 * e], b = links[4 * l + 2 * e + 1];
+      if (f < 0 || f >= h->nf || b < 0 || b >= h->nw[f]) return -1;
+      h->spring[2 * l + e] = h->boff[f] + b;
+    }
+  h->nspring = n;
+  h->spring_k = k;
+  h->spring_len = len;
+  return 0;
+}
+
+/* apply_spring_link_forces (spring_link_functions.cpp:24-45): in link
+ * order, F_a += f and F_b -= f with d = VecPeriodicX(Xs_a, Xs_b)
+ * (spring_link_functions.cpp:131-138), f = -k (|d| - L) d / |d|. */
+static void spring_forces(or_sim *h) {
+  const double Lb[3] = {h->p.fcm.nx * h->p.fcm.dx, h->p.fcm.ny * h->p.fcm.dx,
+                        h->p.fcm.nz * h->p.fcm.dx};
+  for (int l = 0; l < h->nspring; ++l) {
+    const int a = h->spring[2 * l], b = h->spring[2 * l + 1];
+    double d[3];
+    for (int c = 0; c < 3; ++c) {
+      d[c] = h->Xs[3 * a + c] - h->Xs[3 * b + c];
+      d[c] = d[c] - Lb[c] * floor(d[c] / Lb[c] + 0.5);
+    }
+    const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double mag = h->spring_k * (dist - h->spring_len);
+    for (int c = 0; c < 3; ++c) {
+      const double f = -mag * d[c] / dist;
+      h->F[3 * a + c] += f;
+      h->F[3 * b + c] -= f;
+    }
+  }
+}
+
 /* forces + torques + assign_filament_data (main.cpp:337-361) */
 void or_sim_forces(or_sim *h, int nt) {
   memset(h->F, 0, sizeof(double) * 3 * h->np);
@@ -531,6 +580,7 @@ void or_sim_forces(or_sim *h, int nt) {
       for (int d = 0; d < 3; ++d)
         h->F[3 * (h->boff[f] + i) + d] += h->p.weight_per_length[d] * h->p.dL;
   }
+  spring_forces(h); /* main.cpp:354-357, 540-543 */
   /* Fcm::assign_filament_data (FCMfunctions.cpp:882-918) */
   double L[3] = {h->p.fcm.nx * h->p.fcm.dx, h->p.fcm.ny * h->p.fcm.dx,
                  h->p.fcm.nz * h->p.fcm.dx};

@@ -79,6 +79,8 @@ int or_sim_timestep(or_sim *h, int nt);
 /* Pieces. */
 void or_sim_initial_guess(or_sim *h);
 void or_sim_forces(or_sim *h, int nt);
+int or_sim_set_spring_links(or_sim *h, int n, const int *links, double k,
+                            double len);
 void or_sim_fcm_arrays(or_sim *h, double **Y, double **F, double **T,
                        double **V, double **W);
 void or_sim_build_jacobian(or_sim *h, int nt);

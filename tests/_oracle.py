@@ -63,6 +63,8 @@ def lib():
         "or_sim_timestep": (C.c_int, [P, C.c_int]),
         "or_sim_initial_guess": (None, [P]),
         "or_sim_forces": (None, [P, C.c_int]),
+        "or_sim_set_spring_links": (C.c_int, [P, C.c_int, _IP, C.c_double,
+                                              C.c_double]),
         "or_sim_fcm_arrays": (None, [P] + [C.POINTER(_DP)] * 5),
         "or_sim_build_jacobian": (None, [P, C.c_int]),
         "or_sim_get_jacobian": (None, [P, C.c_int, _DP]),
@@ -290,6 +292,12 @@ class Sim:
 
     def forces(self, nt):
         lib().or_sim_forces(self.h, nt)
+
+    def set_spring_links(self, links, k, length):
+        """links: (n, 4) rows (filament A, bead B, filament C, bead D)."""
+        a = np.ascontiguousarray(links, dtype=np.int32).reshape(-1, 4)
+        rc = lib().or_sim_set_spring_links(self.h, len(a), ip(a), k, length)
+        assert rc == 0, "spring link outside the system"
 
     def build_jacobian(self, nt):
         lib().or_sim_build_jacobian(self.h, nt)

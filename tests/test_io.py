@@ -109,3 +109,25 @@ def test_binary_checkpoint_bitwise(tmp_path):
         assert np.array_equal(back[k].reshape(st[k].shape), st[k]), k
     with pytest.raises(abi.FsError):
         api.io_read_checkpoint(path, [8, 8])
+
+
+def test_spring_link_file(tmp_path):
+    """The reference's -springlinks.dat layout (spring_link_functions.cpp:
+    50-101): header line, count, 'A B C D' rows."""
+    p = tmp_path / "Test-springlinks.dat"
+    p.write_text("Spring link data file. Number of links written first, then "
+                 "springs between {filament A, bead B} and {filament C, bead "
+                 "D} in the form 'A B C D'.\n3\n0 14 1 0\n1 14 2 0\n"
+                 "2 14 0 0\n")
+    links = api.io_read_spring_links(str(p))
+    assert links.tolist() == [[0, 14, 1, 0], [1, 14, 2, 0], [2, 14, 0, 0]]
+    q = tmp_path / "out-springlinks.dat"
+    api.io_write_spring_links(str(q), links)
+    assert q.read_text() == p.read_text()
+    assert api.io_read_spring_links(str(q)).tolist() == links.tolist()
+    api.io_write_spring_links(str(q), np.zeros((0, 4), np.int32))
+    assert api.io_read_spring_links(str(q)).shape == (0, 4)
+    bad = tmp_path / "bad-springlinks.dat"
+    bad.write_text("header\n2\n0 1 2 3\n")
+    with pytest.raises(abi.FsError):
+        api.io_read_spring_links(str(bad))

@@ -202,3 +202,44 @@ def test_timestep_converges_to_tol():
     its = [s.timestep(nt) for nt in (1, 2)]
     assert all(0 < i < sp.broyden_maxiter for i in its)
     s.forces(2)
+
+
+def test_spring_links_hookes_law():
+    """Spring links (spring_link_functions.cpp:24-45) on hand-placed beads:
+    k = 5, natural length 2. Stretched along z (|d| = 3): the beads attract
+    with 5 each. Compressed across the periodic boundary (|d| = 1 via the
+    minimum image): they repel through the boundary with 5 each. A bead in
+    two links gets both. Forces come from the previous positions Xs."""
+    sp, nws, props, st = config(grid=64, nfil=3, nw=8, seed=2)
+    Lz = sp.fcm.nz * sp.fcm.dx
+    st = dict(st)
+    Xs = st["Xs"].copy()
+    Xs[0] = [1.0, 1.0, 1.0]          # filament 0, bead 0
+    Xs[8] = [1.0, 1.0, 4.0]          # filament 1, bead 0: 3 above
+    Xs[16] = [5.0, 5.0, 0.5]         # filament 2, bead 0
+    Xs[23] = [5.0, 5.0, Lz - 0.5]    # filament 2, bead 7: 1 below, wrapped
+    st["Xs"] = Xs
+    s0 = O.Sim(sp, nws, props)
+    s0.set_state(**st)
+    s0.forces(1)
+    F0 = s0.get_state()["F"]
+    s1 = O.Sim(sp, nws, props)
+    s1.set_state(**st)
+    s1.set_spring_links([[0, 0, 1, 0], [2, 0, 2, 7], [1, 0, 2, 0]], 5.0, 2.0)
+    s1.forces(1)
+    dF = s1.get_state()["F"] - F0
+    # the third link (8 -> 16): d = (-4, -4, 3.5), |d| = sqrt(44.25)
+    d3 = np.array([-4.0, -4.0, 3.5])
+    f3 = -5.0 * (np.sqrt(44.25) - 2.0) * d3 / np.sqrt(44.25)
+    want = np.zeros_like(dF)
+    want[0] = [0, 0, 5.0]
+    want[8] = [0, 0, -5.0]
+    want[8] += f3
+    want[16] = [0, 0, 5.0]
+    want[23] = [0, 0, -5.0]
+    want[16] -= f3
+    assert np.abs(dF - want).max() < 1e-12
+    assert np.abs(dF.sum(0)).max() < 1e-12
+    # the FCM forces carry them (assign_filament_data after the springs)
+    Fa = s1.fcm_arrays()[1]
+    assert np.array_equal(Fa, s1.get_state()["F"])
