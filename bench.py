@@ -229,21 +229,21 @@ def c5_bench(args, torch):
     (240,000) at the 1,000-filament 256^3 workload's density, i.e. on a
     512^3 grid (dx = a/1.86, ngd = 12), random non-overlapping placement,
     random phases, KB = 1000, dt = 0.005, TOL 1e-6, barrier on, FP32 FCM +
-    FP64 Broyden. (On a 384^3 grid the same filaments fill 11% of the box
-    and the per-filament approximate Jacobian no longer makes Broyden
-    converge: the iterates diverge in the first steps, on the CPU oracle
-    as on the GPU.) The network is trained on 56,000 filament-steps (7
-    recorded steps after 40 start-up steps, in which the swimming gait
-    develops from straight filaments and the iteration count settles) of
-    the plain mixed path, then
-    mixed + learned and plain mixed are timed from the same state."""
+    FP64 Broyden, placed by the GPU generator. (On a 384^3 grid the same
+    filaments fill 11% of the box and the per-filament approximate Jacobian
+    no longer makes Broyden converge: the iterates diverge in the first
+    steps, on the CPU oracle as on the GPU.) The network is trained on
+    56,000 filament-steps (7 recorded steps after 40 start-up steps, in
+    which the swimming gait develops from straight filaments and the
+    iteration count settles) of the plain mixed path, then mixed + learned
+    and plain mixed are timed from the same state."""
     from filsim_amd import abi, api, params, warmstart
     fp = params.fcm_params(512)
     nfil, nw = 8000, 30
     rng = np.random.default_rng(9)
     phases = list(rng.uniform(0, 2 * math.pi, nfil))
     X0, q0 = api.make_filaments(params.box(fp), [nw] * nfil, dL=2.2,
-                                min_sep=2.2, seed=20261202)
+                                min_sep=2.2, seed=20261202, gpu=True)
 
     def build():
         sp = params.sim_params(fp, dt=0.005, tol=1e-6, maxiter=100,

@@ -110,6 +110,7 @@ PROTOS = {
     "fs_system_mobility": [P, I],
     "fs_system_barrier": [P, I, I, _DP],
     "fs_make_filaments": [_DP, I, IP, D, D, C.c_ulonglong, I, _DP, _DP],
+    "fs_make_filaments_gpu": [_DP, I, IP, D, D, C.c_ulonglong, I, _DP, _DP],
     "fs_mlp_create": [I, I, I, C.c_ulonglong, C.POINTER(P)],
     "fs_mlp_destroy": [P],
     "fs_mlp_num_params": [P, IP],

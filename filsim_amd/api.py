@@ -337,18 +337,21 @@ class Batch:
         abi.check(self.lib.fs_batch_set_lanes(self.h, lanes))
 
 
-def make_filaments(box, nw, dL=2.2, min_sep=2.2, seed=1, max_tries=100000):
+def make_filaments(box, nw, dL=2.2, min_sep=2.2, seed=1, max_tries=100000,
+                   gpu=False):
     """Seeded straight filaments (heads uniform, orientations uniform),
-    overlap-rejected at min_sep. Returns X (np,3), q (np,4)."""
+    overlap-rejected at min_sep. Returns X (np,3), q (np,4). gpu=True tests
+    the trials on the GPU (fs_make_filaments_gpu); the result is the same
+    bit for bit."""
     lib = abi.load()
     nw = np.ascontiguousarray(nw, dtype=np.int32)
     npt = int(nw.sum())
     X = np.zeros((npt, 3))
     q = np.zeros((npt, 4))
     b = np.ascontiguousarray(box, dtype=np.float64)
-    abi.check(lib.fs_make_filaments(abi.dptr(b), len(nw), abi.iptr(nw), dL,
-                                    min_sep, seed, max_tries, abi.dptr(X),
-                                    abi.dptr(q)))
+    fn = lib.fs_make_filaments_gpu if gpu else lib.fs_make_filaments
+    abi.check(fn(abi.dptr(b), len(nw), abi.iptr(nw), dL, min_sep, seed,
+                 max_tries, abi.dptr(X), abi.dptr(q)))
     return X, q
 
 

@@ -5,7 +5,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <unordered_map>
 #include <vector>
 
 #include "common.hpp"
@@ -363,133 +362,6 @@ extern "C" int fs_system_barrier(fs_system *h, int nt, int link,
                                  double *min_dist2) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
   return h->impl.barrier(nt, link, min_dist2);
-}
-
-// ---------------------------------------------------------------------
-// Initial configuration: straight filaments, heads uniform in the box,
-// orientation uniform on the sphere (random unit quaternion), rejected
-// while any two segments of different filaments are closer than min_sep
-// (periodic). Deterministic: splitmix64 stream from the seed.
-// ---------------------------------------------------------------------
-namespace {
-
-struct Rng {
-  uint64_t s;
-  uint64_t next() {
-    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-  }
-  double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
-  double normal() {  // Box-Muller
-    double u1 = uniform(), u2 = uniform();
-    if (u1 < 1e-300) u1 = 1e-300;
-    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-  }
-};
-
-struct Grid {
-  double L[3];
-  int M[3];
-  double cw[3];
-  std::unordered_map<long, std::vector<int>> cells;
-  long key(int a, int b, int c) const {
-    return ((long)a * M[1] + b) * M[2] + c;
-  }
-  void coords(const double *x, int *c) const {
-    for (int d = 0; d < 3; ++d) {
-      double w = x[d] - L[d] * std::floor(x[d] / L[d]);
-      int ci = (int)(w / cw[d]);
-      c[d] = std::min(std::max(ci, 0), M[d] - 1);
-    }
-  }
-};
-
-}  // namespace
-
-extern "C" int fs_make_filaments(const double box[3], int nfil, const int *nw,
-                                 double dL, double min_sep,
-                                 unsigned long long seed, int max_tries,
-                                 double *X, double *q) {
-  FS_CHECK(box && nw && X && q && nfil >= 1 && dL > 0 && min_sep >= 0,
-           FS_ERR_ARG, "bad argument");
-  Grid G;
-  for (int d = 0; d < 3; ++d) {
-    G.L[d] = box[d];
-    G.M[d] = std::max(1, (int)(box[d] / std::max(min_sep, 1e-12)));
-    G.M[d] = std::min(G.M[d], 4096);
-    G.cw[d] = box[d] / G.M[d];
-  }
-  Rng rng{seed};
-  std::vector<double> pts;  // accepted segment positions
-  int g0 = 0;
-  const double ms2 = min_sep * min_sep;
-  for (int f = 0; f < nfil; ++f) {
-    int n = nw[f];
-    bool ok = false;
-    std::vector<double> cand(3 * n);
-    double qq[4];
-    for (int tr = 0; tr < max_tries && !ok; ++tr) {
-      double head[3];
-      for (int d = 0; d < 3; ++d) head[d] = rng.uniform() * box[d];
-      double nn = 0;
-      for (int c = 0; c < 4; ++c) {
-        qq[c] = rng.normal();
-        nn += qq[c] * qq[c];
-      }
-      nn = std::sqrt(nn);
-      for (int c = 0; c < 4; ++c) qq[c] /= nn;
-      // tangent = R(q) e_x (UnitQuaternion.hpp:341-352)
-      double t[3] = {1 - 2 * qq[2] * qq[2] - 2 * qq[3] * qq[3],
-                     2 * (qq[1] * qq[2] + qq[0] * qq[3]),
-                     2 * (qq[1] * qq[3] - qq[0] * qq[2])};
-      for (int i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) cand[3 * i + d] = head[d] + (i * dL) * t[d];
-      ok = true;
-      for (int i = 0; i < n && ok; ++i) {
-        int c[3];
-        G.coords(&cand[3 * i], c);
-        for (int a = -1; a <= 1 && ok; ++a)
-          for (int b = -1; b <= 1 && ok; ++b)
-            for (int e = -1; e <= 1 && ok; ++e) {
-              int ca = ((c[0] + a) % G.M[0] + G.M[0]) % G.M[0];
-              int cb = ((c[1] + b) % G.M[1] + G.M[1]) % G.M[1];
-              int cc = ((c[2] + e) % G.M[2] + G.M[2]) % G.M[2];
-              auto it = G.cells.find(G.key(ca, cb, cc));
-              if (it == G.cells.end()) continue;
-              for (int j : it->second) {
-                double r2 = 0;
-                for (int d = 0; d < 3; ++d) {
-                  double x = cand[3 * i + d] - pts[3 * j + d];
-                  x -= G.L[d] * std::floor(x / G.L[d] + 0.5);
-                  r2 += x * x;
-                }
-                if (r2 < ms2) {
-                  ok = false;
-                  break;
-                }
-              }
-            }
-      }
-    }
-    FS_CHECK(ok, FS_ERR_ARG,
-             "could not place filament %d without overlap in %d tries", f,
-             max_tries);
-    for (int i = 0; i < n; ++i) {
-      int g = g0 + i;
-      for (int d = 0; d < 3; ++d) {
-        X[3 * g + d] = cand[3 * i + d];
-        pts.push_back(cand[3 * i + d]);
-      }
-      for (int c = 0; c < 4; ++c) q[4 * g + c] = qq[c];
-      int cc[3];
-      G.coords(&cand[3 * i], cc);
-      G.cells[G.key(cc[0], cc[1], cc[2])].push_back(g);
-    }
-    g0 += n;
-  }
-  return FS_OK;
 }
 
 // ---------------------------------------------------------------------

@@ -231,13 +231,20 @@ int fs_system_mobility(fs_system *h, int nt);
 /* Collision barrier on the system's Y/F (link = 1 rebuilds the list). */
 int fs_system_barrier(fs_system *h, int nt, int link, double *min_dist2);
 
-/* Deterministic initial configuration: nfil straight filaments of nw[f]
- * segments, heads uniform in the box, orientation uniform on the sphere,
- * rejected while any two segments of different filaments are closer than
- * min_sep (periodic). Writes X (np x 3) and q (np x 4). seed: 64-bit. */
+/* Deterministic initial configuration (the reference's overlap-rejected
+ * placement, Filament.cpp:8-153, filament_initialisation_functions.cpp:
+ * 631-650): nfil straight filaments of nw[f] segments, heads uniform in the
+ * box, orientation uniform on the sphere, each retried (up to max_tries
+ * times) while any of its segments is closer than min_sep to a segment
+ * already placed (periodic). Writes host X (np x 3) and q (np x 4); seed:
+ * 64-bit. fs_make_filaments_gpu tests windows of trials on the current GPU
+ * and returns the same bits as fs_make_filaments. */
 int fs_make_filaments(const double box[3], int nfil, const int *nw,
                       double dL, double min_sep, unsigned long long seed,
                       int max_tries, double *X, double *q);
+int fs_make_filaments_gpu(const double box[3], int nfil, const int *nw,
+                          double dL, double min_sep, unsigned long long seed,
+                          int max_tries, double *X, double *q);
 
 /* ------------------------------------------------------------------ */
 /* Learned warm start (no reference counterpart: replaces the linear   */
