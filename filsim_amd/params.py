@@ -6,6 +6,8 @@ benchmarks and tests run on (grid spacing a/1.86 with 12 support points).
 """
 import math
 
+import numpy as np
+
 from . import abi
 
 PI = 3.14159265358979323846264338327950288
@@ -242,3 +244,125 @@ def rod_packing_filaments(box_len, nfil, nw, dL=2.2, seed=1, jitter=0.3,
         qf = _qmul(qf, (np.cos(0.5 * theta), np.sin(0.5 * theta), 0.0, 0.0))
         q[f * nw:(f + 1) * nw] = qf
     return X, q
+
+
+def _qmul4(a, b):
+    return np.array(_qmul(tuple(a), tuple(b)))
+
+
+def _qsqrt(p):
+    """quaSquareRootInPlace (UnitQuaternion.hpp:388-409)."""
+    s, v = p[0], np.asarray(p[1:], dtype=float)
+    if s < -.9999999999999:
+        return np.array([0.0, 0.0, 0.0, 1.0])
+    if s > .9999999999999:
+        return np.array([1.0, 0.0, 0.0, 0.0])
+    return np.concatenate([[np.sqrt(.5 * (1 + s))], v / np.sqrt(2. * (1 + s))])
+
+
+def _qrotate(q, v):
+    """RotateVector (UnitQuaternion.hpp:323-338)."""
+    w, x, y, z = q
+    R = np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    return R @ np.asarray(v, dtype=float)
+
+
+def frame_quaternion(tangent, normal):
+    """Bead orientation of Bead::initialBeadSetup (Bead.hpp:151-166): the
+    half-angle rotation taking e_x to the tangent, then the one taking the
+    rotated e_y to the normal."""
+    t = np.asarray(tangent, dtype=float)
+    qt = _qsqrt(np.array([t[0], 0.0, -t[2], t[1]]))
+    v = _qrotate(qt, [0.0, 1.0, 0.0])
+    p = _qsqrt(np.concatenate([[np.dot(v, normal)], np.cross(v, normal)]))
+    return _qmul4(p, qt)
+
+
+def _periodic(d, L):
+    """VecPeriodicX (spring_link_functions.cpp:131-138)."""
+    return d - L * np.floor(d / L + 0.5)
+
+
+def network_filaments(box_len, n_nodes, min_node_spacing, connection_radius,
+                      connection_probability, spacing_away_from_node,
+                      dL=2.2, a=1.0, allow_overlap=False, seed=1,
+                      node_box=None, max_node_tries=100000):
+    """Connected-nodes network (filament_initialisation_functions.cpp:
+    259-410, spring_link_functions.cpp:8-22, 145-177), with numpy's seeded
+    generator in place of Armadillo's:
+      * nodes uniform in node_box (default: the periodic box), each redrawn
+        while closer than min_node_spacing to an earlier node;
+      * every node pair nn < mm draws u ~ U(0,1) and is joined when
+        u < connection_probability and the pair is closer than
+        connection_radius (minimum image);
+      * a joining filament has int((d - 2 spacing_away_from_node) / dL)
+        beads, centred between the nodes along the unit vector node1 ->
+        node2 with bead n at CoM + (n - nbeads/2) dL t, and the frame of
+        Bead::initialBeadSetup with normal t x e_x; it is skipped when any
+        bead is closer than sep (2a) to a bead of a placed filament, with
+        sep = 0.01 when allow_overlap else 1.05 (the reference also skips
+        pairs whose centres are 2.5 filament lengths apart; here every pair
+        of beads is checked). Pairs giving fewer than 2 beads are skipped;
+      * every two filament ends at one node are joined by a spring link
+        (spring_links::decide_spring_links).
+    Returns (nws, X, q, links, nodes): segment counts, positions (np, 3),
+    orientations (np, 4), spring links (n, 4) as (filament A, bead B,
+    filament C, bead D) rows and the node positions (n_nodes, 3)."""
+    rng = np.random.default_rng(seed)
+    L = np.asarray(box_len, dtype=float)
+    NB = L if node_box is None else np.asarray(node_box, dtype=float)
+    nodes = []
+    for _ in range(n_nodes):
+        for _t in range(max_node_tries):
+            p = rng.uniform(size=3) * NB
+            if not nodes or np.sqrt(((_periodic(np.array(nodes) - p, L)) ** 2)
+                                    .sum(1)).min() >= min_node_spacing:
+                break
+        else:
+            raise ValueError("could not place node %d at spacing %g" %
+                             (len(nodes), min_node_spacing))
+        nodes.append(p)
+    nodes = np.array(nodes).reshape(-1, 3)
+    pairs = []
+    for nn in range(n_nodes):
+        for mm in range(nn + 1, n_nodes):
+            if rng.uniform() < connection_probability:
+                d = np.linalg.norm(_periodic(nodes[nn] - nodes[mm], L))
+                if d < connection_radius:
+                    pairs.append((nn, mm))
+    sep = (0.01 if allow_overlap else 1.05) * 2 * a
+    nws, Xs, qs = [], [], []
+    at_nodes = [[] for _ in range(n_nodes)]
+    placed = np.zeros((0, 3))
+    ex = np.array([1.0, 0.0, 0.0])
+    for n1, n2 in pairs:
+        v = _periodic(nodes[n2] - nodes[n1], L)
+        d = np.linalg.norm(v)
+        nb = int((d - 2 * spacing_away_from_node) / dL)
+        if nb < 2:
+            continue
+        t = v / d
+        normal = np.cross(t, ex)
+        normal = normal / np.linalg.norm(normal)
+        com = nodes[n1] + v / 2
+        X = np.array([com + (n * dL) * t - nb * dL / 2 * t for n in range(nb)])
+        if len(placed):
+            dd = _periodic(X[:, None, :] - placed[None, :, :], L)
+            if np.sqrt((dd ** 2).sum(-1)).min() < sep:
+                continue
+        f = len(nws)
+        nws.append(nb)
+        Xs.append(X)
+        qs.append(np.tile(frame_quaternion(t, normal), (nb, 1)))
+        placed = np.concatenate([placed, X])
+        at_nodes[n1].append((f, 0))
+        at_nodes[n2].append((f, nb - 1))
+    links = [[e1[0], e1[1], e2[0], e2[1]] for ends in at_nodes
+             for i, e1 in enumerate(ends) for e2 in ends[i + 1:]]
+    X = np.concatenate(Xs) if Xs else np.zeros((0, 3))
+    q = np.concatenate(qs) if qs else np.zeros((0, 4))
+    return (nws, X, q, np.array(links, dtype=np.int32).reshape(-1, 4),
+            nodes)

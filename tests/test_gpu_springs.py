@@ -104,3 +104,36 @@ def test_batch_with_springs_is_bitwise(cuda):
     batch.close()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("ifs", [100, 0])
+def test_network_timesteps_match_oracle(cuda, ifs):
+    """A connected-nodes network (filaments of different lengths between
+    random nodes, spring links at the nodes) stepped under the friction
+    (ifs=100) and FCM (ifs=0) mobilities."""
+    from filsim_amd import params
+    import _oracle as O
+    fp = params.fcm_params(48)
+    box = params.box(fp)
+    nws, X, q, links, _ = params.network_filaments(
+        box, n_nodes=14, min_node_spacing=7.0, connection_radius=16.0,
+        connection_probability=0.6, spacing_away_from_node=1.5, seed=4)
+    assert len(nws) >= 4 and len(links) >= 2
+    sp = params.sim_params(fp, dt=0.01, tol=1e-9, maxiter=100,
+                           weight=(0.0, 0.0, -0.5), barrier_on=1,
+                           initial_friction_steps=ifs)
+    props = params.fil_props(len(nws), kap=5.0)
+    st = dict(X=X, Xs=X, Xt=X, q=q, qs=q, qt=q)
+    o = O.Sim(sp, nws, props)
+    g = api.System(sp, nws, props)
+    for s in (o, g):
+        s.set_state(**st)
+        s.set_spring_links(links, 4.0, 2.2)
+    for nt in range(1, 4):
+        io, ig = o.timestep(nt), g.timestep(nt)
+        assert abs(io - ig) <= 1, (nt, io, ig)
+    so, sg = o.get_state(), g.get_state()
+    Lb = np.abs(so["X"]).max()
+    assert np.abs(sg["X"] - so["X"]).max() <= 1e-8 * Lb
+    assert np.abs(sg["q"] - so["q"]).max() <= 1e-8
+    g.close()
