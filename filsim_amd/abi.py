@@ -23,7 +23,8 @@ class FcmParams(C.Structure):
 
 class FcmTuning(C.Structure):
     _fields_ = [("fft_batch", C.c_int), ("spread_tile", C.c_int),
-                ("spread_zrun", C.c_int), ("fft_xlines", C.c_int)]
+                ("spread_zrun", C.c_int), ("fft_xlines", C.c_int),
+                ("spread_mfma", C.c_int)]
 
 
 class ColbarParams(C.Structure):

@@ -92,8 +92,9 @@ class Fcm:
         return t
 
     def set_tuning(self, fft_batch=-1, spread_tile=-1, spread_zrun=-1,
-                   fft_xlines=-1):
-        t = abi.FcmTuning(fft_batch, spread_tile, spread_zrun, fft_xlines)
+                   fft_xlines=-1, spread_mfma=-1):
+        t = abi.FcmTuning(fft_batch, spread_tile, spread_zrun, fft_xlines,
+                          spread_mfma)
         abi.check(self.lib.fs_fcm_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, np_, reps=3):

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -308,6 +309,151 @@ __global__ void __launch_bounds__(256)
       R[(size_t)c * N + ((size_t)rx * G.ny + ry) * G.nz + pz0 + k] = stage[e];
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------
+// spread on the matrix cores (FP32): one workgroup per 8 x 8 x 16 tile,
+// four wavefronts of 16 (x, y) points each. Per component the tile is
+//   R[xy][z] = sum over candidates c and terms t of A_t[xy][c] * B_t[c][z]
+// with the force term   A_0 = F * gx gy,            B_0 = gz,
+// and the torque terms  A_1 = (ax, ay, az) gdx gdy,  B_1 = gdz,
+//                       A_2 = (-G_y, G_x, 0) gdx gdy, B_2 = gdz * grad_z
+// (ax = G_z grad_y, ay = -G_z grad_x, az = G_y grad_x - G_x grad_y, G the
+// half torque: the same sum as k_spread). Each MFMA 16x16x4 step takes four
+// candidates of one term; lane l supplies A[xy = l % 16][c = l / 16] and
+// B[c = l / 16][z = l % 16] and holds R[xy = 4 (l / 16) + r][z = l % 16].
+// Candidates outside a point's stencil contribute exact zeros (selects,
+// never products with stale LDS).
+// ---------------------------------------------------------------------
+constexpr int kMfmaChunk = 32;  // candidates staged per round (LDS for 8 workgroups per CU)
+
+template <int NGD>
+__global__ void __launch_bounds__(256)
+    k_spread_mfma(const int *__restrict__ act, SpreadGeom G,
+                  const float *__restrict__ tab, const int *__restrict__ origin,
+                  const float *__restrict__ coef, const int *__restrict__ start,
+                  const int *__restrict__ ids, float *__restrict__ R) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  if (act && !*act) return;  // batched sweep: suspension converged
+  __shared__ int s_bstart[kMaxBinsPerTile + 1];
+  __shared__ int s_bfirst[kMaxBinsPerTile];
+  __shared__ int s_ntot;
+  __shared__ float s_xy[kMfmaChunk][6][NGD];  // gx gy gdx gdy grad_x grad_y
+  __shared__ float s_zt[kMfmaChunk][3][16];       // B_0..B_2 at the tile's z
+  __shared__ float s_coef[kMfmaChunk][6];
+  __shared__ int s_org[kMfmaChunk][3];
+  __shared__ int s_cid[kMfmaChunk];
+
+  const int ntiles = G.nbx * G.nby * G.nbz;
+  int bid = blockIdx.x;  // XCD-aware order, as k_spread
+  if ((ntiles & 7) == 0) bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
+  const int bx = bid % G.nbx;
+  const int by = (bid / G.nbx) % G.nby;
+  const int bz = bid / (G.nbx * G.nby);
+  constexpr int ngd = NGD;
+  const int cx = min(G.hx + 1, G.nbx), cy = min(G.hy + 1, G.nby),
+            cz = min(G.hz + 1, G.nbz);
+  const int nbins = cx * cy * cz;
+  for (int t = threadIdx.x; t < nbins; t += blockDim.x) {
+    int dz = t / (cx * cy), dy = (t / cx) % cy, dx = t % cx;
+    int ix = ((bx - G.hx + dx) % G.nbx + G.nbx) % G.nbx;
+    int iy = ((by - G.hy + dy) % G.nby + G.nby) % G.nby;
+    int iz = ((bz - G.hz + dz) % G.nbz + G.nbz) % G.nbz;
+    int b = spread_bin(ix, iy, iz, G.nbx, G.nby, G.nbz);
+    s_bfirst[t] = start[b];
+    s_bstart[t + 1] = start[b + 1] - start[b];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_bstart[0] = 0;
+    for (int t = 0; t < nbins; ++t) s_bstart[t + 1] += s_bstart[t];
+    s_ntot = s_bstart[nbins];
+  }
+  __syncthreads();
+  const int ntot = s_ntot;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = lane >> 4, i16 = lane & 15;
+  const int xy = wave * 16 + i16;  // this lane's A row
+  const int px = bx * 8 + (xy >> 3), py = by * 8 + (xy & 7), pz0 = bz * 16;
+  v4f acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
+
+  for (int c0 = 0; c0 < ntot; c0 += kMfmaChunk) {
+    const int nc = min(kMfmaChunk, ntot - c0);
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const int gc = c0 + c;
+      int t = 0;
+      while (gc >= s_bstart[t + 1]) ++t;
+      const int s = ids[s_bfirst[t] + gc - s_bstart[t]];
+      s_cid[c] = s;
+      for (int d = 0; d < 3; ++d) s_org[c][d] = origin[3 * s + d];
+      for (int r = 0; r < 6; ++r) s_coef[c][r] = coef[6 * s + r];
+    }
+    __syncthreads();
+    // x/y rows of the tables (rows 0, 1, 3, 4, 6, 7 of [9][ngd])
+    for (int e = threadIdx.x; e < nc * 6 * ngd; e += blockDim.x) {
+      const int c = e / (6 * ngd), r = (e / ngd) % 6, i = e % ngd;
+      const int row = r + (r >> 1);  // 0 1 3 4 6 7
+      s_xy[c][r][i] = tab[(size_t)s_cid[c] * 9 * ngd + row * ngd + i];
+    }
+    // z functions at the tile's 16 z (nz >= 16 + ngd: one overlap interval)
+    for (int e = threadIdx.x; e < nc * 16; e += blockDim.x) {
+      const int c = e >> 4, z = e & 15;
+      int o = pz0 + z - s_org[c][2];
+      if (o < 0) o += G.nz;
+      if (o >= G.nz) o -= G.nz;
+      float g = 0.f, gd = 0.f, gg = 0.f;
+      if (o < ngd) {
+        const float *tb = tab + (size_t)s_cid[c] * 9 * ngd;
+        g = tb[2 * ngd + o];
+        gd = tb[5 * ngd + o];
+        gg = gd * tb[8 * ngd + o];
+      }
+      s_zt[c][0][z] = g;
+      s_zt[c][1][z] = gd;
+      s_zt[c][2][z] = gg;
+    }
+    __syncthreads();
+    for (int c4 = 0; c4 < nc; c4 += 4) {
+      const int c = c4 + q;
+      const bool live = c < nc;
+      const int cs = live ? c : 0;
+      int ix = px - s_org[cs][0];
+      if (ix < 0) ix += G.nx;
+      int iy = py - s_org[cs][1];
+      if (iy < 0) iy += G.ny;
+      const bool in = live && ix < ngd && iy < ngd;
+      const int ixc = in ? ix : 0, iyc = in ? iy : 0;
+      const float *cf = s_coef[cs];
+      const float gxy = s_xy[cs][0][ixc] * s_xy[cs][1][iyc];
+      const float dxy = s_xy[cs][2][ixc] * s_xy[cs][3][iyc];
+      const float xx = s_xy[cs][4][ixc], yy = s_xy[cs][5][iyc];
+      const float ax = cf[5] * yy, ay = -cf[5] * xx, az = cf[4] * xx - cf[3] * yy;
+      const float b0 = live ? s_zt[cs][0][i16] : 0.f;
+      const float b1 = live ? s_zt[cs][1][i16] : 0.f;
+      const float b2 = live ? s_zt[cs][2][i16] : 0.f;
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? cf[0] * gxy : 0.f, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? cf[1] * gxy : 0.f, b0, acc1, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? cf[2] * gxy : 0.f, b0, acc2, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? ax * dxy : 0.f, b1, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? ay * dxy : 0.f, b1, acc1, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? az * dxy : 0.f, b1, acc2, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? -cf[4] * dxy : 0.f, b2, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? cf[3] * dxy : 0.f, b2, acc1, 0, 0, 0);
+    }
+    __syncthreads();  // the next round overwrites the staged candidates
+  }
+  // lane holds R[xy = wave*16 + 4q + r][z = i16]: 16 lanes write 16
+  // consecutive z of one (x, y) column
+  const size_t N = (size_t)G.nx * G.ny * G.nz;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int oxy = wave * 16 + 4 * q + r;
+    const size_t o = ((size_t)(bx * 8 + (oxy >> 3)) * G.ny + by * 8 + (oxy & 7)) *
+                         G.nz + pz0 + i16;
+    R[o] = acc0[r];
+    R[N + o] = acc1[r];
+    R[2 * N + o] = acc2[r];
   }
 }
 
@@ -956,6 +1102,7 @@ int FcmImpl::init(const fs_fcm_params &pp, int maxnp, int prec) {
   tune.fft_batch = (prec == FS_FP64) ? 8 : 16;
   tune.spread_tile = 16;
   tune.spread_zrun = 8;
+  tune.spread_mfma = 1;
   tune.fft_xlines = (prec == FS_FP32) ? 4 : 2;
   FS_TRY(set_geometry());
   for (int i = 0; i < 8; ++i) FS_HIP(hipEventCreate(&ev[i]));
@@ -989,13 +1136,21 @@ int FcmImpl::set_geometry() {
     }
   FS_CHECK(tz > 0, FS_ERR_ARG, "no spread z run divides nz=%d", p.nz);
   int tx = largest_divisor_le(p.nx, t), ty = largest_divisor_le(p.ny, t);
+  // FP32 matrix-core spread: fixed 8 x 8 x 16 tiles (four 16-point MFMA rows)
+  mfma_spread = precision == FS_FP32 && tune.spread_mfma == 1 &&
+                p.nx % 8 == 0 && p.ny % 8 == 0 && p.nz % 16 == 0 &&
+                p.nz >= 16 + p.ngd;
+  if (mfma_spread) {
+    tx = ty = 8;
+    tz = 16;
+  }
   int bins = 0;
   for (int attempt = 0; attempt < 8; ++attempt) {
     int hx = (p.ngd - 1 + tx - 1) / tx, hy = (p.ngd - 1 + ty - 1) / ty,
         hz = (p.ngd - 1 + tz - 1) / tz;
     bins = std::min(hx + 1, p.nx / tx) * std::min(hy + 1, p.ny / ty) *
            std::min(hz + 1, p.nz / tz);
-    if (bins <= kMaxBinsPerTile) break;
+    if (bins <= kMaxBinsPerTile || mfma_spread) break;
     int nx2 = largest_divisor_le(p.nx, 2 * tx), ny2 = largest_divisor_le(p.ny, 2 * ty);
     if (nx2 * ny2 > 256) break;
     tx = nx2;
@@ -1115,12 +1270,24 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                      nbins, bin_start.as<int>(), ids.as<int>());
   mark(1);
   const int ntiles = nbins;
-  bool okz = with_TZ(geom.tz, [&](auto z) {
-    constexpr int TZ = decltype(z)::value;
-    hipLaunchKernelGGL((k_spread<T, TZ>), dim3(ntiles), dim3(geom.tx * geom.ty),
-                       0, st, act, geom, tab.as<T>(), origin.as<int>(), coef.as<T>(),
-                       bin_start.as<int>(), ids.as<int>(), R.as<T>());
-  });
+  bool okz = true;
+  if constexpr (std::is_same<T, float>::value) {
+    if (mfma_spread)
+      okz = with_NGD(p.ngd, [&](auto g) {
+        constexpr int NG = decltype(g)::value;
+        hipLaunchKernelGGL(k_spread_mfma<NG>, dim3(ntiles), dim3(256), 0, st,
+                           act, geom, tab.as<float>(), origin.as<int>(),
+                           coef.as<float>(), bin_start.as<int>(),
+                           ids.as<int>(), R.as<float>());
+      });
+  }
+  if (!mfma_spread)
+    okz = with_TZ(geom.tz, [&](auto z) {
+      constexpr int TZ = decltype(z)::value;
+      hipLaunchKernelGGL((k_spread<T, TZ>), dim3(ntiles), dim3(geom.tx * geom.ty),
+                         0, st, act, geom, tab.as<T>(), origin.as<int>(), coef.as<T>(),
+                         bin_start.as<int>(), ids.as<int>(), R.as<T>());
+    });
   FS_CHECK(okz, FS_ERR_ARG, "spread dispatch failed");
   FS_LAUNCHED();
   mark(2);
@@ -1268,6 +1435,7 @@ int FcmImpl::set_tuning(const fs_fcm_tuning &t) {
   if (t.spread_tile > 0) nt.spread_tile = t.spread_tile;
   if (t.spread_zrun > 0) nt.spread_zrun = t.spread_zrun;
   if (t.fft_xlines > 0) nt.fft_xlines = t.fft_xlines;
+  if (t.spread_mfma >= 0) nt.spread_mfma = t.spread_mfma != 0;
   FS_CHECK(nt.fft_batch >= 1 && nt.fft_batch <= 64 && nt.fft_xlines >= 1 &&
                nt.fft_xlines <= 16,
            FS_ERR_ARG, "fft_batch must be in [1,64], fft_xlines in [1,16]");
@@ -1343,13 +1511,14 @@ int FcmImpl::autotune(int np, int reps, fs_fcm_tuning *best) {
   fs_fcm_tuning cur = tune;
   float tbest = 0;
   rc = time_it(&tbest);
-  static const int tiles[] = {4, 8, 16}, zruns[] = {4, 8, 16},
+  static const int mfma[] = {0, 1}, tiles[] = {4, 8, 16}, zruns[] = {4, 8, 16},
                    batches[] = {4, 8, 16, 32}, xlines[] = {1, 2, 4, 8};
   struct Field {
     int fs_fcm_tuning::*m;
     const int *v;
     int n;
-  } fields[] = {{&fs_fcm_tuning::spread_tile, tiles, 3},
+  } fields[] = {{&fs_fcm_tuning::spread_mfma, mfma, precision == FS_FP32 ? 2 : 0},
+                {&fs_fcm_tuning::spread_tile, tiles, 3},
                 {&fs_fcm_tuning::spread_zrun, zruns, 3},
                 {&fs_fcm_tuning::fft_batch, batches, 4},
                 {&fs_fcm_tuning::fft_xlines, xlines, 4}};

@@ -84,13 +84,20 @@ typedef struct fs_fcm_tuning {
   int spread_tile;  /* spread tile edge in x and y (4, 8 or 16)        */
   int spread_zrun;  /* largest spread tile extent in z (4..16)         */
   int fft_xlines;   /* x lines (per component) per workgroup, x pass  */
+  int spread_mfma;  /* FP32: 1 spreads with matrix-core products on
+                       8 x 8 x 16 tiles (grids with nx, ny multiples of 8,
+                       nz of 16, nz >= 16 + ngd), 0 with the vector gather;
+                       ignored in FP64. In fs_fcm_set_tuning a negative
+                       value keeps the current setting, as do values <= 0
+                       of the other fields. */
 } fs_fcm_tuning;
 int fs_fcm_get_tuning(fs_fcm *h, fs_fcm_tuning *t);
 int fs_fcm_set_tuning(fs_fcm *h, const fs_fcm_tuning *t);
 /* Autotuner (no reference counterpart; the reference has no launch
  * shapes): times `reps` mobility solves of np synthetic segments (uniform
  * in the box, forces/torques uniform in [-1, 1]) for each candidate of each
- * tunable in turn (spread_tile, spread_zrun, fft_batch, fft_xlines), keeps
+ * tunable in turn (spread_mfma, spread_tile, spread_zrun, fft_batch,
+ * fft_xlines), keeps
  * the fastest, leaves it set and returns it in *best (may be NULL). Results
  * of every candidate agree to rounding (the FP64 path to 1e-12). */
 int fs_fcm_autotune(fs_fcm *h, int np, int reps, fs_fcm_tuning *best);

@@ -47,12 +47,16 @@ def test_mobility_fp64_matches_oracle(cuda, grid, ngd, dx):
     assert rel(W, Wo) <= 1e-12
 
 
+@pytest.mark.parametrize("mfma", [1, 0])
 @pytest.mark.parametrize("grid", [64, (256, 128, 64), (512, 32, 48)])
-def test_mobility_fp32_error_bound(cuda, grid):
+def test_mobility_fp32_error_bound(cuda, grid, mfma):
+    """FP32 spread on the matrix cores (mfma=1, the default) and with the
+    vector gather (mfma=0)."""
     fp = params.fcm_params(grid)
     Y, F, T = random_segments(fp, 400, seed=5)
     Vo, Wo = O.mobility(fp, Y, F, T)
     with api.Fcm(fp, 400, precision=abi.FS_FP32) as fcm:
+        fcm.set_tuning(spread_mfma=mfma)
         V, W = fcm.mobility_host(Y, F, T)
     ev, ew = rel(V, Vo), rel(W, Wo)
     print("fp32 relV %.3e relW %.3e" % (ev, ew))
