@@ -33,11 +33,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per dispatch of k_fft_x_project<float> at 256^3, measured with
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in separate runs, each scaled by
-# the same counter's reading for a 1 GiB copy (profiles/
-# fcm_mobility_256cubed_30k_hbm_bytes.txt): 203.1 MB read + 202.9 MB written
-X_PASS_FP32_TRAFFIC_BYTES = 406.0e6
+# HBM bytes per dispatch of the FP32 spread (k_spread_mfma<12>) at 256^3 /
+# 30,000 segments, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in
+# separate runs, each scaled by the same counter's reading for a 1 GiB copy
+# (profiles/fcm_mobility_256cubed_30k_hbm_bytes.txt): 51.1 MB read (ids,
+# origins, coefficients, tables) + 201.3 MB written
+SPREAD_FP32_TRAFFIC_BYTES = 252.4e6
 
 
 def log(*a):
@@ -375,12 +376,12 @@ def main():
         "fp32_algorithmic_GBps": 24 * N * 4 / t32 / 1e9,
         "fp64_stage_ms": [float(x) for x in st64],
         "fp32_stage_ms": [float(x) for x in st32]}
-    # dominant kernel: the fused x pass (forward x FFT + Stokes projection +
-    # inverse x FFT over the 3 half-spectrum components), stage bucket 4.
-    # Algorithmic bytes: read + write 3 components of 256*256*129 complex.
-    nc = 256 * 256 * 129
-    xk_bytes = 2 * 3 * nc * 2 * 4  # fp32 path (the headline precision)
-    xk_ms = float(st32[4])
+    # dominant kernel of the timestep: the FP32 spread (stage bucket 2, HIP
+    # events on the solver's stream around its launch). Algorithmic bytes:
+    # one write of the 3 real grids (3 * 256^3 floats); the candidate
+    # tables it reads are ~2% of that and not counted.
+    xk_bytes = 3 * N * 4
+    xk_ms = float(st32[2])
 
     res = {}
     info = {}
@@ -492,11 +493,11 @@ def main():
                                "KB=1000, dt=0.005, TOL=1e-6, barrier on",
                    "global_batch": 30000, "seq_len": 256,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_fft_x_project<float>",
+        "roofline": {"bound": "hbm", "kernel": "k_spread_mfma<12> (FP32)",
                      "achieved": xk_bytes / (xk_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": xk_bytes / (xk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": X_PASS_FP32_TRAFFIC_BYTES,
+                     "traffic": SPREAD_FP32_TRAFFIC_BYTES,
                      "algorithmic_bytes": xk_bytes},
         "cpu_baseline": cpu,
         "extra": out,
