@@ -140,71 +140,73 @@ def cpu_baseline(fp_iters):
 def batch_bench(args, torch, n=64, seed=4000):
     """Batched sweep: n suspensions (grid from {32..128}^3, 8-200
     filaments of 15-40 segments capped by box size, KB log-uniform in
-    [0.1, 10], gravity N(0, 0.2^2)), FP32 FCM + FP64 Broyden, TOL 1e-4,
-    dt 0.01, each suspension's FCM autotuned. Timed: the batch (captured
-    graphs) against the same suspensions stepped one after another."""
+    [0.1, 10], gravity N(0, 0.2^2)), FP32 FCM + FP64 Broyden, TOL 1e-6,
+    dt 0.01, each suspension's FCM autotuned. Every mode (the batch in each
+    launch mode, and the same suspensions stepped one after another) runs a
+    fresh copy of the suspensions from the same initial state over the same
+    steps, so the modes do the same work."""
     from filsim_amd import abi, api, params
     pts = [params.sweep_point(seed + k, precision=abi.FS_FP32, tol=1e-6)
            for k in range(n)]
-    groups = {}
     t0 = time.perf_counter()
-    for name in ("batched", "sequential"):
+
+    def fresh(tunes=None):
         g = []
-        for fp, sp, nws, props, st, info in pts:
+        for k, (fp, sp, nws, props, st, info) in enumerate(pts):
             s = api.System(sp, nws, props)
             s.set_state(**st)
+            if tunes is not None:
+                s.set_tuning(tunes[k])
             g.append(s)
-        groups[name] = g
-    tunes = [s.autotune(reps=2) for s in groups["batched"]]
-    # the sequential group gets the same launch shapes
-    for s, t in zip(groups["sequential"], tunes):
-        s.set_tuning(t)
+        return g
+
+    tuner = fresh()
+    tunes = [s.autotune(reps=2) for s in tuner]
+    for s in tuner:
+        s.close()
     t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    batch = api.Batch(groups["batched"])
-    t_capture = time.perf_counter() - t0
-    res = {}
+    res, finals, t_capture = {}, {}, 0.0
     steps = max(2, args.steps // 2)
-    lane_modes = (0, 4, 8)
-    nts = {"batched": 1, "sequential": 1}
-    for name in ["batched_lanes%d" % k for k in lane_modes] + ["sequential"]:
+    for name in ["batched_lanes%d" % k for k in (0, 4, 8)] + ["sequential"]:
+        grp = fresh(tunes)
+        batch = None
         if name.startswith("batched"):
+            t0 = time.perf_counter()
+            batch = api.Batch(grp)
+            t_capture = time.perf_counter() - t0
             batch.set_lanes(int(name[len("batched_lanes"):]))
-        nt = nts["batched" if name.startswith("batched") else name]
-        its_all = []
-        grp = groups["batched" if name.startswith("batched") else name]
+
+        def step(nt):
+            if batch is not None:
+                return batch.timestep(nt)
+            return [s.timestep(nt) for s in grp]
+
+        nt = 1
         for _ in range(args.warmup):
-            if name.startswith("batched"):
-                batch.timestep(nt)
-            else:
-                [s.timestep(nt) for s in grp]
+            step(nt)
             nt += 1
         torch.cuda.synchronize()
+        its_all = []
         t0 = time.perf_counter()
         for _ in range(steps):
-            if name.startswith("batched"):
-                its_all.append(batch.timestep(nt))
-            else:
-                its_all.append([s.timestep(nt) for s in grp])
+            its_all.append(step(nt))
             nt += 1
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        nts["batched" if name.startswith("batched") else name] = nt
         res[name] = {"suspension_steps_per_s": n * steps / el,
                      "s_per_batch_step": el / steps,
                      "mean_iters": float(np.mean(its_all))}
         log(name, res[name])
-    # the batched group ran len(lane_modes) times as many steps: replay the
-    # sequential group to the same step count before comparing
-    nt = nts["sequential"]
-    while nt < nts["batched"]:
-        [s.timestep(nt) for s in groups["sequential"]]
-        nt += 1
-    same = all(np.array_equal(a.get_state()["X"], b.get_state()["X"])
-               for a, b in zip(groups["batched"], groups["sequential"]))
-    batch.close()
+        finals[name] = [s.get_state()["X"] for s in grp]
+        if batch is not None:
+            batch.close()
+        for s in grp:
+            s.close()
     best = max((k for k in res if k.startswith("batched")),
                key=lambda k: res[k]["suspension_steps_per_s"])
+    same = all(np.array_equal(a, b)
+               for k in res if k.startswith("batched")
+               for a, b in zip(finals[k], finals["sequential"]))
     res["batched"] = dict(res[best], mode=best)
     info = [p[5] for p in pts]
     return {"metric": "batched sweep: suspension-timesteps/s, %d suspensions"
