@@ -792,9 +792,8 @@ __global__ void __launch_bounds__(1024)
 
 // ---------------------------------------------------------------------
 // interpolation: one 64-lane wavefront per segment (segments in bin order
-// so neighbours share L2 lines). Lane l owns the (x, y) stencil columns
-// l, l + 64, ... and loads each column's z run at once (NGD independent
-// loads per component in flight), then the wavefront reduces.
+// so neighbours share L2 lines); each lane owns one z of the stencil and a
+// share of its (x, y) columns, then the wavefront reduces.
 // ---------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -844,47 +843,74 @@ __global__ void __launch_bounds__(256)
             oz = valid ? origin[3 * s + 2] : 0;
   __syncthreads();
   const size_t N = (size_t)G.nx * G.ny * G.nz;
-  // chunks of CH points per lane: all CH loads of a chunk are issued before
-  // they are used (3 CH independent loads in flight), consecutive lanes on
-  // consecutive z
-  constexpr int CH = PER < 9 ? PER : 9;
-  T v0 = 0, v1 = 0, v2 = 0, w0 = 0, w1 = 0, w2 = 0;
-  for (int t0 = 0; t0 < PER; t0 += CH) {
+  // NGD lanes per (x, y) column, KL columns per pass: lane (cl, k) keeps one
+  // z (its weights and wrapped z are constants) and walks the columns
+  // cl, cl + KL, ...; consecutive lanes still read consecutive z. The
+  // stencil sums factor per lane into 9 running sums scaled by the lane's
+  // z weights at the end:
+  //   V   = g_z   sum gx gy u
+  //   W_x = gd_z (sum d y u_z - z sum d u_y), d = gdx gdy, x, y, z the
+  //   W_y = gd_z (z sum d u_x - sum d x u_z)   dipole gradient factors
+  //   W_z = gd_z (sum d x u_y - sum d y u_x)
+  constexpr int KL = 64 / NGD;
+  constexpr int NPASS = (NC + KL - 1) / KL;
+  constexpr int KLI = KL / NGD, KLJ = KL % NGD;
+  constexpr int CH = 6;  // passes whose loads are in flight together
+  const int cl = lane / NGD, kz = lane - cl * NGD;
+  const bool on = valid && cl < KL;
+  int zw = oz + (on ? kz : 0);
+  if (zw >= G.nz) zw -= G.nz;
+  const T gz = on ? s_z[wave][kz] : (T)0, gdz = on ? s_z[wave][NGD + kz] : (T)0,
+          zk = on ? s_z[wave][2 * NGD + kz] : (T)0;
+  int ci = cl / NGD, cj = cl % NGD;  // the next pass's column (i, j)
+  T sg0 = 0, sg1 = 0, sg2 = 0, sd0 = 0, sd1 = 0;
+  T sx1 = 0, sx2 = 0, sy0 = 0, sy2 = 0;
+  for (int t0 = 0; t0 < NPASS; t0 += CH) {
     T ux[CH], uy[CH], uz[CH];
+    int ic[CH], jc[CH];
+    bool okc[CH];
 #pragma unroll
     for (int t = 0; t < CH; ++t) {
-      const int p = lane + 64 * (t0 + t);
-      const bool ok = valid && p < NP;
-      const int c = ok ? p / NGD : 0, k = ok ? p - (p / NGD) * NGD : 0;
-      const int i = c / NGD, j = c - (c / NGD) * NGD;
-      int ii = ox + i;
+      const bool ok = on && t0 + t < NPASS && ci < NGD;
+      int ii = ox + (ok ? ci : 0);
       if (ii >= G.nx) ii -= G.nx;
-      int jj = oy + j;
+      int jj = oy + (ok ? cj : 0);
       if (jj >= G.ny) jj -= G.ny;
-      int z = oz + k;
-      if (z >= G.nz) z -= G.nz;
-      const size_t o = ((size_t)ii * G.ny + jj) * G.nz + z;
+      const size_t o = ((size_t)ii * G.ny + jj) * G.nz + zw;
       ux[t] = ok ? R[o] : (T)0;
       uy[t] = ok ? R[N + o] : (T)0;
       uz[t] = ok ? R[2 * N + o] : (T)0;
+      ic[t] = ok ? ci : 0;
+      jc[t] = ok ? cj : 0;
+      okc[t] = ok;
+      ci += KLI;
+      cj += KLJ;
+      if (cj >= NGD) {
+        cj -= NGD;
+        ++ci;
+      }
     }
 #pragma unroll
     for (int t = 0; t < CH; ++t) {
-      const int p = lane + 64 * (t0 + t);
-      const bool ok = valid && p < NP;
-      const int c = ok ? p / NGD : 0, k = ok ? p - (p / NGD) * NGD : 0;
-      const int i = c / NGD, j = c - (c / NGD) * NGD;
-      const T temp = s_gxy[wave][c] * s_z[wave][k];
-      const T temp2 = s_dxy[wave][c] * s_z[wave][NGD + k];
-      const T xx = s_xx[wave][i], yy = s_yy[wave][j], zz = s_z[wave][2 * NGD + k];
-      v0 += ux[t] * temp;
-      v1 += uy[t] * temp;
-      v2 += uz[t] * temp;
-      w0 += (uz[t] * yy - uy[t] * zz) * temp2;
-      w1 += (ux[t] * zz - uz[t] * xx) * temp2;
-      w2 += (uy[t] * xx - ux[t] * yy) * temp2;
+      const int c = ic[t] * NGD + jc[t];
+      const T gxy = okc[t] ? s_gxy[wave][c] : (T)0;
+      const T dxy = okc[t] ? s_dxy[wave][c] : (T)0;
+      const T tx = dxy * s_xx[wave][ic[t]], ty = dxy * s_yy[wave][jc[t]];
+      sg0 += gxy * ux[t];
+      sg1 += gxy * uy[t];
+      sg2 += gxy * uz[t];
+      sd0 += dxy * ux[t];
+      sd1 += dxy * uy[t];
+      sx1 += tx * uy[t];
+      sx2 += tx * uz[t];
+      sy0 += ty * ux[t];
+      sy2 += ty * uz[t];
     }
   }
+  T v0 = gz * sg0, v1 = gz * sg1, v2 = gz * sg2;
+  T w0 = gdz * (sy2 - zk * sd1);
+  T w1 = gdz * (zk * sd0 - sx2);
+  T w2 = gdz * (sx1 - sy0);
   v0 = wave_sum(v0);
   v1 = wave_sum(v1);
   v2 = wave_sum(v2);
