@@ -131,7 +131,14 @@ __device__ __forceinline__ int cb_wrap(int v, int m) {
 
 // mode 0: F[i] += sum of pair forces, min distance; mode 1: count pairs
 // (i<j) inside the cutoff into *npairs and write them when room.
-__global__ void __launch_bounds__(64)
+// kCbLanes lanes per segment split its 27 cells (lane q: cells q,
+// q + kCbLanes, ..., in the reference's dz, dy, dx order) and combine their
+// partial forces pairwise by xor distance 1, 2 (deterministic); more
+// wavefronts in flight for the dependent cell/id/position loads than one
+// thread per segment.
+constexpr int kCbLanes = 4;
+
+__global__ void __launch_bounds__(256)
     k_cb_apply(const int *__restrict__ act, int np,
                const double *__restrict__ Y, CbGeom g, CbConst c,
                const int *__restrict__ cell, const int *__restrict__ start,
@@ -141,52 +148,59 @@ __global__ void __launch_bounds__(64)
                int *__restrict__ npairs, int max_pairs,
                int *__restrict__ pairs) {
   if (act && !*act) return;  // batched sweep: suspension converged
-  __shared__ double s_min[64];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double s_min[256];
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt / kCbLanes, q = gt % kCbLanes;
   double mymin = 100000.0;
+  double fx = 0, fy = 0, fz = 0;
   if (i < np) {
     const double yi[3] = {Y[3 * i], Y[3 * i + 1], Y[3 * i + 2]};
     const int ifil = sfil[i], iseg = sseg[i];
     const int ci = cell[i];
     const int cx = ci % g.MX, cy = (ci / g.MX) % g.MY, cz = ci / (g.MX * g.MY);
-    double fx = 0, fy = 0, fz = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          int nc = cb_wrap(cx + dx, g.MX) +
-                   g.MX * (cb_wrap(cy + dy, g.MY) + g.MY * cb_wrap(cz + dz, g.MZ));
-          for (int e = start[nc]; e < start[nc + 1]; ++e) {
-            int j = ids[e];
-            if (j == i) continue;
-            double yj[3] = {Y[3 * j], Y[3 * j + 1], Y[3 * j + 2]};
-            double rsq, f[3];
-            if (!cb_pair(yi, yj, ifil, iseg, sfil[j], sseg[j], c, rsq, f))
-              continue;
-            mymin = fmin(mymin, rsq);
-            if (rsq < c.chiRnm2) {
-              if (mode == 0) {
-                fx += f[0];
-                fy += f[1];
-                fz += f[2];
-              } else if (i < j) {
-                int k = atomicAdd(npairs, 1);
-                if (k < max_pairs) {
-                  pairs[2 * k] = i;
-                  pairs[2 * k + 1] = j;
-                }
-              }
+    for (int k = q; k < 27; k += kCbLanes) {
+      const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+      int nc = cb_wrap(cx + dx, g.MX) +
+               g.MX * (cb_wrap(cy + dy, g.MY) + g.MY * cb_wrap(cz + dz, g.MZ));
+      for (int e = start[nc]; e < start[nc + 1]; ++e) {
+        int j = ids[e];
+        if (j == i) continue;
+        double yj[3] = {Y[3 * j], Y[3 * j + 1], Y[3 * j + 2]};
+        double rsq, f[3];
+        if (!cb_pair(yi, yj, ifil, iseg, sfil[j], sseg[j], c, rsq, f))
+          continue;
+        mymin = fmin(mymin, rsq);
+        if (rsq < c.chiRnm2) {
+          if (mode == 0) {
+            fx += f[0];
+            fy += f[1];
+            fz += f[2];
+          } else if (i < j) {
+            int kk = atomicAdd(npairs, 1);
+            if (kk < max_pairs) {
+              pairs[2 * kk] = i;
+              pairs[2 * kk + 1] = j;
             }
           }
         }
-    if (mode == 0) {
-      F[3 * i] += fx;
-      F[3 * i + 1] += fy;
-      F[3 * i + 2] += fz;
+      }
     }
+  }
+  // every lane of the wavefront takes part in the shuffles
+#pragma unroll
+  for (int m = 1; m < kCbLanes; m <<= 1) {
+    fx += __shfl_xor(fx, m, 64);
+    fy += __shfl_xor(fy, m, 64);
+    fz += __shfl_xor(fz, m, 64);
+  }
+  if (i < np && q == 0 && mode == 0) {
+    F[3 * i] += fx;
+    F[3 * i + 1] += fy;
+    F[3 * i + 2] += fz;
   }
   s_min[threadIdx.x] = mymin;
   __syncthreads();
-  for (int off = 32; off > 0; off >>= 1) {
+  for (int off = 128; off > 0; off >>= 1) {
     if (threadIdx.x < off)
       s_min[threadIdx.x] = fmin(s_min[threadIdx.x], s_min[threadIdx.x + off]);
     __syncthreads();
@@ -296,7 +310,7 @@ int ColbarImpl::apply(const double *Y, int nt, double *F, double *min_dist2,
   hipLaunchKernelGGL(k_cb_init_min, dim3(1), dim3(1), 0, st, act,
                      minb.as<unsigned long long>(), ib);
   if (np > 0)
-    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div(np, 64)), dim3(64), 0, st,
+    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div((long)np * kCbLanes, 256)), dim3(256), 0, st,
                        act, np, Y, g, consts(nt), cell.as<int>(), start.as<int>(),
                        ids.as<int>(), sfil.as<int>(), sseg.as<int>(), F,
                        minb.as<unsigned long long>(), 0, (int *)nullptr, 0,
@@ -337,7 +351,7 @@ int ColbarImpl::pairs(const double *Y, int nt, int max_pairs, int *out,
   int *dn = d.as<int>(), *dp = dn + 1;
   hipLaunchKernelGGL(k_cb_fill, dim3(1), dim3(1), 0, 0, dn, 1, 0);
   if (np > 0)
-    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div(np, 64)), dim3(64), 0, 0,
+    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div((long)np * kCbLanes, 256)), dim3(256), 0, 0,
                        (const int *)nullptr, np,
                        Y, g, consts(nt), cell.as<int>(), start.as<int>(),
                        ids.as<int>(), sfil.as<int>(), sseg.as<int>(),
