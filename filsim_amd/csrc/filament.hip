@@ -193,18 +193,21 @@ __device__ __forceinline__ void d_dexpinv(const double *u, const double *w,
 // kinematics
 // ---------------------------------------------------------------------
 // unRobotArmify (Filament.hpp:489-505), run by one lane.
+// Called by the whole workgroup after qs is final: the tangents are formed
+// in parallel into LDS, then one thread accumulates the positions in bead
+// order (the same operations and order as a serial loop).
 __device__ void unrobotarmify(const SysDev &S, int b0, int nw) {
+  __shared__ double s_tan[kMaxNw][3];
+  for (int i = threadIdx.x; i < nw; i += blockDim.x)
+    q_tangent(S.qs + 4 * (b0 + i), s_tan[i]);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   double Xsum[3] = {S.Xs[3 * b0], S.Xs[3 * b0 + 1], S.Xs[3 * b0 + 2]};
-  double t1[3], t2[3];
-  q_tangent(S.qs + 4 * b0, t1);
-  for (int i = 1; i < nw; ++i) {
-    q_tangent(S.qs + 4 * (b0 + i), t2);
+  for (int i = 1; i < nw; ++i)
     for (int d = 0; d < 3; ++d) {
-      Xsum[d] += (t1[d] + t2[d]) * (0.5 * S.dL);
+      Xsum[d] += (s_tan[i - 1][d] + s_tan[i][d]) * (0.5 * S.dL);
       S.Xs[3 * (b0 + i) + d] = Xsum[d];
-      t1[d] = t2[d];
     }
-  }
 }
 
 // initialGuess (Filament.hpp:227-237, Bead.hpp:202-217) + unRobotArmify
@@ -223,7 +226,7 @@ __global__ void k_guess(SysDev S) {
     for (int c = 0; c < 4; ++c) S.qs[4 * g + c] = q[c];
   }
   __syncthreads();
-  if (threadIdx.x == 0) unrobotarmify(S, b0, nw);
+  unrobotarmify(S, b0, nw);
 }
 
 // Filament::update (Filament.hpp:425-452, Bead.hpp:236-256) + unRobotArmify
@@ -250,7 +253,7 @@ __global__ void k_update(SysDev S, const double *__restrict__ u) {
     for (int d = 0; d < 3; ++d)
       S.lam[3 * (l0 + l) + d] -= u[so + 3 * nw + 3 + 3 * l + d];
   __syncthreads();
-  if (threadIdx.x == 0) unrobotarmify(S, b0, nw);
+  unrobotarmify(S, b0, nw);
 }
 
 // setZeroForcesTorques + applyElasticTorques + applyConstraintForcesTorques
