@@ -119,3 +119,19 @@ def test_friction(cuda):
     Vo, Wo = O.friction(fp, F, T)
     assert np.array_equal(V.cpu().numpy(), Vo)
     assert np.array_equal(W.cpu().numpy(), Wo)
+
+
+@pytest.mark.parametrize("mfma", [1, 0])
+@pytest.mark.parametrize("grid", [64, (96, 64, 48)])
+def test_mobility_fp32_reference_resolution(cuda, grid, mfma):
+    """FP32 at the reference's own resolution (dx = a/3.3, ngd = 20): the
+    matrix-core spread's staging and tables at another support."""
+    fp = params.reference_fcm_params(grid)
+    Y, F, T = random_segments(fp, 300, seed=11)
+    Vo, Wo = O.mobility(fp, Y, F, T)
+    with api.Fcm(fp, 300, precision=abi.FS_FP32) as fcm:
+        fcm.set_tuning(spread_mfma=mfma)
+        V, W = fcm.mobility_host(Y, F, T)
+    ev, ew = rel(V, Vo), rel(W, Wo)
+    print("fp32 ref grid", grid, "mfma", mfma, "relV %.3e relW %.3e" % (ev, ew))
+    assert ev < 1e-5 and ew < 1e-5
