@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------
-// spread on the matrix cores (FP32): one workgroup per 8 x 8 x 16 tile,
+// spread on the matrix cores: one workgroup per 8 x 8 x 16 tile,
 // four wavefronts of 16 (x, y) points each. Per component the tile is
 //   R[xy][z] = sum over candidates c and terms t of A_t[xy][c] * B_t[c][z]
 // with the force term   A_0 = F * gx gy,            B_0 = gz,
@@ -322,26 +322,55 @@ __global__ void __launch_bounds__(256)
 // (ax = G_z grad_y, ay = -G_z grad_x, az = G_y grad_x - G_x grad_y, G the
 // half torque: the same sum as k_spread). Each MFMA 16x16x4 step takes four
 // candidates of one term; lane l supplies A[xy = l % 16][c = l / 16] and
-// B[c = l / 16][z = l % 16] and holds R[xy = 4 (l / 16) + r][z = l % 16].
+// B[c = l / 16][z = l % 16] and holds R[xy = Mfma<T>::row(l, r)][z = l % 16].
 // Candidates outside a point's stencil contribute exact zeros (selects,
 // never products with stale LDS).
 // ---------------------------------------------------------------------
-constexpr int kMfmaChunk = 32;  // candidates staged per round (LDS for 8 workgroups per CU)
+// Candidates staged per round: LDS for 8 workgroups per CU at ngd = 12.
+template <typename T>
+constexpr int kMfmaChunk = sizeof(T) == 8 ? 16 : 32;
 
-template <int NGD>
+// One 16x16x4 matrix-core step D += A B in the real type, and the row of
+// D held in accumulator element r of lane l (column l % 16): 4 (l / 16) + r
+// for FP32, 4 r + l / 16 for FP64 (the instructions' output layouts).
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) {
+    return 4 * (lane >> 4) + r;
+  }
+};
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) {
+    return 4 * r + (lane >> 4);
+  }
+};
+
+template <typename T, int NGD>
 __global__ void __launch_bounds__(256)
     k_spread_mfma(const int *__restrict__ act, SpreadGeom G,
-                  const float *__restrict__ tab, const int *__restrict__ origin,
-                  const float *__restrict__ coef, const int *__restrict__ start,
-                  const int *__restrict__ ids, float *__restrict__ R) {
-  typedef float v4f __attribute__((ext_vector_type(4)));
+                  const T *__restrict__ tab, const int *__restrict__ origin,
+                  const T *__restrict__ coef, const int *__restrict__ start,
+                  const int *__restrict__ ids, T *__restrict__ R) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int kMfmaChunk = fs::kMfmaChunk<T>;
   if (act && !*act) return;  // batched sweep: suspension converged
   __shared__ int s_bstart[kMaxBinsPerTile + 1];
   __shared__ int s_bfirst[kMaxBinsPerTile];
   __shared__ int s_ntot;
-  __shared__ float s_xy[kMfmaChunk][6][NGD];  // gx gy gdx gdy grad_x grad_y
-  __shared__ float s_zt[kMfmaChunk][3][16];       // B_0..B_2 at the tile's z
-  __shared__ float s_coef[kMfmaChunk][6];
+  __shared__ T s_xy[kMfmaChunk][6][NGD];  // gx gy gdx gdy grad_x grad_y
+  __shared__ T s_zt[kMfmaChunk][3][16];   // B_0..B_2 at the tile's z
+  __shared__ T s_coef[kMfmaChunk][6];
   __shared__ int s_org[kMfmaChunk][3];
   __shared__ int s_cid[kMfmaChunk];
 
@@ -376,7 +405,7 @@ __global__ void __launch_bounds__(256)
   const int q = lane >> 4, i16 = lane & 15;
   const int xy = wave * 16 + i16;  // this lane's A row
   const int px = bx * 8 + (xy >> 3), py = by * 8 + (xy & 7), pz0 = bz * 16;
-  v4f acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
+  v4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
 
   for (int c0 = 0; c0 < ntot; c0 += kMfmaChunk) {
     const int nc = min(kMfmaChunk, ntot - c0);
@@ -402,9 +431,9 @@ __global__ void __launch_bounds__(256)
       int o = pz0 + z - s_org[c][2];
       if (o < 0) o += G.nz;
       if (o >= G.nz) o -= G.nz;
-      float g = 0.f, gd = 0.f, gg = 0.f;
+      T g = 0, gd = 0, gg = 0;
       if (o < ngd) {
-        const float *tb = tab + (size_t)s_cid[c] * 9 * ngd;
+        const T *tb = tab + (size_t)s_cid[c] * 9 * ngd;
         g = tb[2 * ngd + o];
         gd = tb[5 * ngd + o];
         gg = gd * tb[8 * ngd + o];
@@ -424,31 +453,32 @@ __global__ void __launch_bounds__(256)
       if (iy < 0) iy += G.ny;
       const bool in = live && ix < ngd && iy < ngd;
       const int ixc = in ? ix : 0, iyc = in ? iy : 0;
-      const float *cf = s_coef[cs];
-      const float gxy = s_xy[cs][0][ixc] * s_xy[cs][1][iyc];
-      const float dxy = s_xy[cs][2][ixc] * s_xy[cs][3][iyc];
-      const float xx = s_xy[cs][4][ixc], yy = s_xy[cs][5][iyc];
-      const float ax = cf[5] * yy, ay = -cf[5] * xx, az = cf[4] * xx - cf[3] * yy;
-      const float b0 = live ? s_zt[cs][0][i16] : 0.f;
-      const float b1 = live ? s_zt[cs][1][i16] : 0.f;
-      const float b2 = live ? s_zt[cs][2][i16] : 0.f;
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? cf[0] * gxy : 0.f, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? cf[1] * gxy : 0.f, b0, acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? cf[2] * gxy : 0.f, b0, acc2, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? ax * dxy : 0.f, b1, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? ay * dxy : 0.f, b1, acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? az * dxy : 0.f, b1, acc2, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? -cf[4] * dxy : 0.f, b2, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? cf[3] * dxy : 0.f, b2, acc1, 0, 0, 0);
+      const T *cf = s_coef[cs];
+      const T gxy = s_xy[cs][0][ixc] * s_xy[cs][1][iyc];
+      const T dxy = s_xy[cs][2][ixc] * s_xy[cs][3][iyc];
+      const T xx = s_xy[cs][4][ixc], yy = s_xy[cs][5][iyc];
+      const T ax = cf[5] * yy, ay = -cf[5] * xx, az = cf[4] * xx - cf[3] * yy;
+      const T zero = 0;
+      const T b0 = live ? s_zt[cs][0][i16] : zero;
+      const T b1 = live ? s_zt[cs][1][i16] : zero;
+      const T b2 = live ? s_zt[cs][2][i16] : zero;
+      acc0 = Mfma<T>::step(in ? cf[0] * gxy : zero, b0, acc0);
+      acc1 = Mfma<T>::step(in ? cf[1] * gxy : zero, b0, acc1);
+      acc2 = Mfma<T>::step(in ? cf[2] * gxy : zero, b0, acc2);
+      acc0 = Mfma<T>::step(in ? ax * dxy : zero, b1, acc0);
+      acc1 = Mfma<T>::step(in ? ay * dxy : zero, b1, acc1);
+      acc2 = Mfma<T>::step(in ? az * dxy : zero, b1, acc2);
+      acc0 = Mfma<T>::step(in ? -cf[4] * dxy : zero, b2, acc0);
+      acc1 = Mfma<T>::step(in ? cf[3] * dxy : zero, b2, acc1);
     }
     __syncthreads();  // the next round overwrites the staged candidates
   }
-  // lane holds R[xy = wave*16 + 4q + r][z = i16]: 16 lanes write 16
-  // consecutive z of one (x, y) column
+  // lane holds R[xy = wave*16 + Mfma<T>::row(lane, r)][z = i16]: 16 lanes
+  // write 16 consecutive z of one (x, y) column
   const size_t N = (size_t)G.nx * G.ny * G.nz;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int oxy = wave * 16 + 4 * q + r;
+    const int oxy = wave * 16 + Mfma<T>::row(lane, r);
     const size_t o = ((size_t)(bx * 8 + (oxy >> 3)) * G.ny + by * 8 + (oxy & 7)) *
                          G.nz + pz0 + i16;
     R[o] = acc0[r];
@@ -1162,8 +1192,8 @@ int FcmImpl::set_geometry() {
     }
   FS_CHECK(tz > 0, FS_ERR_ARG, "no spread z run divides nz=%d", p.nz);
   int tx = largest_divisor_le(p.nx, t), ty = largest_divisor_le(p.ny, t);
-  // FP32 matrix-core spread: fixed 8 x 8 x 16 tiles (four 16-point MFMA rows)
-  mfma_spread = precision == FS_FP32 && tune.spread_mfma == 1 &&
+  // matrix-core spread: fixed 8 x 8 x 16 tiles (four 16-point MFMA rows)
+  mfma_spread = tune.spread_mfma == 1 &&
                 p.nx % 8 == 0 && p.ny % 8 == 0 && p.nz % 16 == 0 &&
                 p.nz >= 16 + p.ngd;
   if (mfma_spread) {
@@ -1297,16 +1327,14 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   mark(1);
   const int ntiles = nbins;
   bool okz = true;
-  if constexpr (std::is_same<T, float>::value) {
-    if (mfma_spread)
-      okz = with_NGD(p.ngd, [&](auto g) {
-        constexpr int NG = decltype(g)::value;
-        hipLaunchKernelGGL(k_spread_mfma<NG>, dim3(ntiles), dim3(256), 0, st,
-                           act, geom, tab.as<float>(), origin.as<int>(),
-                           coef.as<float>(), bin_start.as<int>(),
-                           ids.as<int>(), R.as<float>());
-      });
-  }
+  if (mfma_spread)
+    okz = with_NGD(p.ngd, [&](auto g) {
+      constexpr int NG = decltype(g)::value;
+      hipLaunchKernelGGL((k_spread_mfma<T, NG>), dim3(ntiles), dim3(256), 0, st,
+                         act, geom, tab.as<T>(), origin.as<int>(),
+                         coef.as<T>(), bin_start.as<int>(),
+                         ids.as<int>(), R.as<T>());
+    });
   if (!mfma_spread)
     okz = with_TZ(geom.tz, [&](auto z) {
       constexpr int TZ = decltype(z)::value;
@@ -1543,7 +1571,7 @@ int FcmImpl::autotune(int np, int reps, fs_fcm_tuning *best) {
     int fs_fcm_tuning::*m;
     const int *v;
     int n;
-  } fields[] = {{&fs_fcm_tuning::spread_mfma, mfma, precision == FS_FP32 ? 2 : 0},
+  } fields[] = {{&fs_fcm_tuning::spread_mfma, mfma, 2},
                 {&fs_fcm_tuning::spread_tile, tiles, 3},
                 {&fs_fcm_tuning::spread_zrun, zruns, 3},
                 {&fs_fcm_tuning::fft_batch, batches, 4},

@@ -35,7 +35,7 @@ struct FcmImpl {
   // spread tile geometry (mirrors SpreadGeom in fcm.hip)
   SpreadGeom geom{};
   int nbins = 0;
-  bool mfma_spread = false;  // FP32 spread on matrix cores (8 x 8 x 16 tiles)
+  bool mfma_spread = false;  // spread on the matrix cores (8 x 8 x 16 tiles)
   int zP = 1, yB = 1, xQ = 1;  // FFT pass workgroup shapes
   // device flag read by every kernel of a solve: 0 = skip (a converged
   // suspension of a batched sweep); nullptr = always run

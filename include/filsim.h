@@ -84,12 +84,11 @@ typedef struct fs_fcm_tuning {
   int spread_tile;  /* spread tile edge in x and y (4, 8 or 16)        */
   int spread_zrun;  /* largest spread tile extent in z (4..16)         */
   int fft_xlines;   /* x lines (per component) per workgroup, x pass  */
-  int spread_mfma;  /* FP32: 1 spreads with matrix-core products on
-                       8 x 8 x 16 tiles (grids with nx, ny multiples of 8,
-                       nz of 16, nz >= 16 + ngd), 0 with the vector gather;
-                       ignored in FP64. In fs_fcm_set_tuning a negative
-                       value keeps the current setting, as do values <= 0
-                       of the other fields. */
+  int spread_mfma;  /* 1 spreads with matrix-core products on 8 x 8 x 16
+                       tiles (grids with nx, ny multiples of 8, nz of 16,
+                       nz >= 16 + ngd), 0 with the vector gather. In
+                       fs_fcm_set_tuning a negative value keeps the current
+                       setting, as do values <= 0 of the other fields. */
 } fs_fcm_tuning;
 int fs_fcm_get_tuning(fs_fcm *h, fs_fcm_tuning *t);
 int fs_fcm_set_tuning(fs_fcm *h, const fs_fcm_tuning *t);

@@ -35,14 +35,18 @@ def _fp(grid, ngd, dx):
     return params.fcm_params(grid, ngd=ngd)
 
 
+@pytest.mark.parametrize("mfma", [1, 0])
 @pytest.mark.parametrize("grid,ngd,dx", CASES)
-def test_mobility_fp64_matches_oracle(cuda, grid, ngd, dx):
+def test_mobility_fp64_matches_oracle(cuda, grid, ngd, dx, mfma):
+    """mfma=1 (the default) spreads on the matrix cores where the grid
+    takes 8 x 8 x 16 tiles, mfma=0 always with the vector gather."""
     fp = _fp(grid, ngd, dx)
     Y, F, T = random_segments(fp, 200, seed=sum(grid))
     Vo, Wo = O.mobility(fp, Y, F, T)
     with api.Fcm(fp, 200) as fcm:
+        fcm.set_tuning(spread_mfma=mfma)
         V, W = fcm.mobility_host(Y, F, T)
-    print("grid", grid, "relV", rel(V, Vo), "relW", rel(W, Wo))
+    print("grid", grid, "mfma", mfma, "relV", rel(V, Vo), "relW", rel(W, Wo))
     assert rel(V, Vo) <= 1e-12
     assert rel(W, Wo) <= 1e-12
 
@@ -101,7 +105,8 @@ def test_tunings_agree(cuda, batch, tile, zrun):
     Y, F, T = random_segments(fp, 500, seed=2)
     Vo, Wo = O.mobility(fp, Y, F, T)
     with api.Fcm(fp, 500) as fcm:
-        fcm.set_tuning(fft_batch=batch, spread_tile=tile, spread_zrun=zrun)
+        fcm.set_tuning(fft_batch=batch, spread_tile=tile, spread_zrun=zrun,
+                       spread_mfma=0)
         V, W = fcm.mobility_host(Y, F, T)
     assert rel(V, Vo) <= 1e-12 and rel(W, Wo) <= 1e-12
 
