@@ -33,7 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per dispatch of the FP32 spread (k_spread_mfma<12>) at 256^3 /
+# HBM bytes per dispatch of the FP32 spread (k_spread_mfma<float, 12>) at 256^3 /
 # 30,000 segments, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in
 # separate runs, each scaled by the same counter's reading for a 1 GiB copy
 # (profiles/fcm_mobility_256cubed_30k_hbm_bytes.txt): 51.1 MB read (ids,
@@ -495,7 +495,7 @@ def main():
                                "KB=1000, dt=0.005, TOL=1e-6, barrier on",
                    "global_batch": 30000, "seq_len": 256,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_spread_mfma<12> (FP32)",
+        "roofline": {"bound": "hbm", "kernel": "k_spread_mfma<float, 12>",
                      "achieved": xk_bytes / (xk_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": xk_bytes / (xk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
