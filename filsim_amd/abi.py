@@ -12,6 +12,10 @@ LIB_PATH = os.path.join(_HERE, "libfilsim_amd.so")
 
 FS_FP64 = 0
 FS_FP32 = 1
+# barrier pair masks (fs_colbar_set_mask, fs_system_set_barrier)
+FS_BARRIER_ALL = 0
+FS_BARRIER_NOT_SWIMMER = 1
+FS_BARRIER_NOT_TEST_PARTICLES = 2
 
 
 class FcmParams(C.Structure):
@@ -89,6 +93,7 @@ PROTOS = {
     "fs_colbar_link_host": [P, _DP],
     "fs_colbar_apply_host": [P, _DP, I, _DP, _DP],
     "fs_colbar_pairs": [P, P, I, I, IP, IP],
+    "fs_colbar_set_mask": [P, I],
     "fs_system_create": [C.POINTER(SimParams), I, IP, C.POINTER(FilProps),
                          C.POINTER(P)],
     "fs_system_destroy": [P],
@@ -110,6 +115,8 @@ PROTOS = {
     "fs_fil_step": [P, I],
     "fs_system_mobility": [P, I],
     "fs_system_barrier": [P, I, I, _DP],
+    "fs_system_set_barrier": [P, I, I],
+    "fs_system_timestep_trial": [P, I, IP],
     "fs_make_filaments": [_DP, I, IP, D, D, C.c_ulonglong, I, _DP, _DP],
     "fs_make_filaments_gpu": [_DP, I, IP, D, D, C.c_ulonglong, I, _DP, _DP],
     "fs_mlp_create": [I, I, I, C.c_ulonglong, C.POINTER(P)],

@@ -147,6 +147,10 @@ class Colbar:
                                                abi.iptr(lst)))
         return head, lst[:self.np]
 
+    def set_mask(self, mask):
+        """Pair mask of the barrier variants (abi.FS_BARRIER_*)."""
+        abi.check(self.lib.fs_colbar_set_mask(self.h, mask))
+
     def pairs(self, Y, nt, max_pairs=1 << 20):
         out = np.zeros(2 * max_pairs, dtype=np.int32)
         n = C.c_int()
@@ -222,6 +226,15 @@ class System:
         it = C.c_int()
         abi.check(self.lib.fs_system_timestep(self.h, nt, C.byref(it)))
         return it.value
+
+    def timestep_trial(self, nt):
+        """The step without Filament::step (a gait-reset trial pass)."""
+        it = C.c_int()
+        abi.check(self.lib.fs_system_timestep_trial(self.h, nt, C.byref(it)))
+        return it.value
+
+    def set_barrier(self, on, mask=abi.FS_BARRIER_ALL):
+        abi.check(self.lib.fs_system_set_barrier(self.h, 1 if on else 0, mask))
 
     def initial_guess(self):
         abi.check(self.lib.fs_fil_initial_guess(self.h))

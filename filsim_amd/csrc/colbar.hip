@@ -89,6 +89,19 @@ __global__ void k_cb_sort(int ncell, const int *__restrict__ start,
   }
 }
 
+// Pair mask of the barrier variants: apply_collision_barrier_not_swimmer
+// (CollisionBarrierFilament.cpp:495, 585: both or neither in filament 0)
+// and apply_collision_barrier_not_test_particles (:754, 840: neither of
+// the last two segments). A masked pair is not visited at all (no force,
+// no minimum distance), as in the reference.
+__device__ __forceinline__ bool cb_mask_in(const CbConst &c, int i, int j,
+                                           int ifil, int jfil) {
+  if (c.mask == FS_BARRIER_NOT_SWIMMER) return (ifil == 0) == (jfil == 0);
+  if (c.mask == FS_BARRIER_NOT_TEST_PARTICLES)
+    return i < c.np - 2 && j < c.np - 2;
+  return true;
+}
+
 // One pair seen from segment i (CollisionBarrierFilament.cpp:80-150).
 // Returns true when the pair is not an excluded neighbour.
 __device__ __forceinline__ bool cb_pair(const double *yi, const double *yj,
@@ -165,9 +178,11 @@ __global__ void __launch_bounds__(256)
       for (int e = start[nc]; e < start[nc + 1]; ++e) {
         int j = ids[e];
         if (j == i) continue;
+        const int jfil = sfil[j];
+        if (!cb_mask_in(c, i, j, ifil, jfil)) continue;
         double yj[3] = {Y[3 * j], Y[3 * j + 1], Y[3 * j + 2]};
         double rsq, f[3];
-        if (!cb_pair(yi, yj, ifil, iseg, sfil[j], sseg[j], c, rsq, f))
+        if (!cb_pair(yi, yj, ifil, iseg, jfil, sseg[j], c, rsq, f))
           continue;
         mymin = fmin(mymin, rsq);
         if (rsq < c.chiRnm2) {
@@ -297,6 +312,8 @@ CbConst ColbarImpl::consts(int nt) const {
   c.cap2 = p.cap * p.cap;
   c.nt = nt;
   c.ifs = p.initial_friction_steps;
+  c.mask = mask;
+  c.np = np;
   return c;
 }
 
@@ -406,6 +423,15 @@ extern "C" int fs_colbar_destroy(fs_colbar *h) {
   (void)hipDeviceSynchronize();
   h->impl.release();
   delete h;
+  return FS_OK;
+}
+
+extern "C" int fs_colbar_set_mask(fs_colbar *h, int mask) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(mask == FS_BARRIER_ALL || mask == FS_BARRIER_NOT_SWIMMER ||
+               mask == FS_BARRIER_NOT_TEST_PARTICLES,
+           FS_ERR_ARG, "unknown barrier mask %d", mask);
+  h->impl.mask = mask;
   return FS_OK;
 }
 

@@ -13,6 +13,7 @@ struct CbGeom {
 struct CbConst {
   double Lx, Ly, Lz, Rnm2, chiRnm2, FS, cap, cap2;
   int nt, ifs;
+  int mask, np;  // pair mask (FS_BARRIER_*) and segment count
 };
 
 struct ColbarImpl {
@@ -24,6 +25,7 @@ struct ColbarImpl {
   int ncell = 0;
   CbGeom g{};
   bool linked = false;
+  int mask = FS_BARRIER_ALL;
   dbuf cell, slot, ids, cnt, start, scan_tmp, sfil, sseg, minb;
 
   int init(const fs_colbar_params &pp, int n, const int *seg_fil,

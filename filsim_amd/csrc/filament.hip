@@ -1708,7 +1708,7 @@ int SystemImpl::mobility(int nt) {
 
 int SystemImpl::barrier(int nt, int link, double *min_dist2) {
   ProfScope ps_(*this, P_COLBAR);
-  if (!p.barrier_on) {
+  if (!p.barrier_on || barrier_off) {
     if (min_dist2) *min_dist2 = 100000.0;
     return FS_OK;
   }
@@ -1723,6 +1723,29 @@ int SystemImpl::timestep(int nt, int *iters) {
   FS_TRY(broyden_loop(nt, iters));
   FS_TRY(step(nt));
   FS_HIP(hipStreamSynchronize(st));
+  return FS_OK;
+}
+
+// main_gaitreset.cpp:310-812: the passes with gait_reset_i < 2 solve the
+// step and skip Filament::step (:775-781), leaving the previous state.
+int SystemImpl::timestep_trial(int nt, int *iters) {
+  FS_TRY(begin_step(nt));
+  FS_TRY(broyden_loop(nt, iters));
+  FS_HIP(hipStreamSynchronize(st));
+  return FS_OK;
+}
+
+int SystemImpl::set_barrier(int on, int mask) {
+  FS_CHECK(on == 0 || on == 1, FS_ERR_ARG, "on must be 0 or 1");
+  FS_CHECK(on == 0 || p.barrier_on, FS_ERR_STATE,
+           "the system was created with barrier_on = 0");
+  if (p.barrier_on) {
+    FS_CHECK(mask == FS_BARRIER_ALL || mask == FS_BARRIER_NOT_SWIMMER ||
+                 mask == FS_BARRIER_NOT_TEST_PARTICLES,
+             FS_ERR_ARG, "unknown barrier mask %d", mask);
+    cb.mask = mask;
+  }
+  barrier_off = (on == 0);
   return FS_OK;
 }
 

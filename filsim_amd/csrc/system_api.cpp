@@ -16,6 +16,7 @@ using fs::SystemImpl;
 struct fs_system {
   SystemImpl impl;
   fs::WarmStart ws;
+  int batches = 0;  // batches holding this system (captured launches)
 };
 
 namespace {
@@ -167,6 +168,20 @@ namespace fs {
 struct MlpImpl;
 MlpImpl *mlp_impl(fs_mlp *h);
 }  // namespace fs
+
+extern "C" int fs_system_set_barrier(fs_system *h, int on, int mask) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(h->batches == 0, FS_ERR_STATE,
+           "the system belongs to a batch (its launches are captured)");
+  return h->impl.set_barrier(on, mask);
+}
+
+extern "C" int fs_system_timestep_trial(fs_system *h, int nt, int *iters) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(!h->ws.attached(), FS_ERR_STATE,
+           "trial steps run on the plain path (detach the warm start)");
+  return h->impl.timestep_trial(nt, iters);
+}
 
 extern "C" int fs_system_set_warmstart(fs_system *h, fs_mlp *m) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
@@ -498,12 +513,15 @@ extern "C" int fs_batch_create(fs_system **sys, int n, fs_batch **out) {
   // one parent graph 0.8x (its child graphs ran one after another)
   rc = fs_batch_set_lanes(b, std::min(n, 4));
   if (rc != FS_OK) return fail(rc);
+  for (int i = 0; i < n; ++i) ++sys[i]->batches;
   *out = b;
   return FS_OK;
 }
 
 extern "C" int fs_batch_destroy(fs_batch *b) {
-  if (b) batch_release(b);
+  if (!b) return FS_OK;
+  for (fs_system *s : b->sys) --s->batches;
+  batch_release(b);
   return FS_OK;
 }
 

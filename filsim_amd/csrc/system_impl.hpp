@@ -61,7 +61,11 @@ struct SystemImpl {
   int step(int nt);
   int mobility(int nt);
   int barrier(int nt, int link, double *min_dist2);
+  // barrier of the gait-reset passes: skipped while barrier_off is set
+  bool barrier_off = false;
+  int set_barrier(int on, int mask);
   int timestep(int nt, int *iters);
+  int timestep_trial(int nt, int *iters);
   int begin_step(int nt);
   int broyden_loop(int nt, int *iters);
   int check0 = 0;

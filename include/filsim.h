@@ -137,6 +137,18 @@ int fs_colbar_apply_host(fs_colbar *h, const double *Y, int nt, double *F,
                          double *min_dist2);
 /* The reference's head[ncell] and list[np] arrays (host out). */
 int fs_colbar_get_lists(fs_colbar *h, int *head, int *list);
+/* Pair masks: which pairs apply, pairs and the minimum distance visit.
+ * FS_BARRIER_ALL: apply_collision_barrier (CollisionBarrierFilament.cpp:
+ * 24-271), the default. FS_BARRIER_NOT_SWIMMER:
+ * apply_collision_barrier_not_swimmer (:441-696), no pairs between
+ * filament 0 and the others (seg_fil == 0 against seg_fil != 0).
+ * FS_BARRIER_NOT_TEST_PARTICLES: apply_collision_barrier_not_test_particles
+ * (:698-945), no pairs with either of the last two segments (np-2, np-1).
+ * The mask holds for every later apply / pairs call. */
+#define FS_BARRIER_ALL 0
+#define FS_BARRIER_NOT_SWIMMER 1
+#define FS_BARRIER_NOT_TEST_PARTICLES 2
+int fs_colbar_set_mask(fs_colbar *h, int mask);
 /* Unordered pairs (i<j) inside the cutoff r^2 < 1.21(2a)^2 found through the
  * cell list, sorted by (i,j); pairs: host, 2*max_pairs ints. */
 int fs_colbar_pairs(fs_colbar *h, const double *Y, int nt, int max_pairs,
@@ -236,6 +248,20 @@ int fs_fil_step(fs_system *h, int nt);
 int fs_system_mobility(fs_system *h, int nt);
 /* Collision barrier on the system's Y/F (link = 1 rebuilds the list). */
 int fs_system_barrier(fs_system *h, int nt, int link, double *min_dist2);
+/* Barrier of the following timesteps (main_gaitreset.cpp:431-458, 606-622:
+ * off, FS_BARRIER_NOT_SWIMMER or the full barrier per pass): on = 0 skips
+ * it, on = 1 applies it with the pair mask (FS_BARRIER_*). on = 1 needs a
+ * system created with barrier_on = 1; not allowed while the system belongs
+ * to a batch (its launches are captured). */
+int fs_system_set_barrier(fs_system *h, int on, int mask);
+/* The gait-reset drivers' trial pass (main_gaitreset.cpp:310-812 with
+ * gait_reset_i < 2): fs_system_timestep without the final Filament::step.
+ * X, Xt, U, Ut, q, qt, lam1 and lam2 are left as they were; the iterate
+ * Xs, Us, qs, F, TAU and V, W hold the trial's result, and so does lam,
+ * which the next step's initial guess does not reset (Filament.hpp:
+ * 226-236), so the next pass of the same nt starts from it as in the
+ * reference. Plain path only (no warm start attached). */
+int fs_system_timestep_trial(fs_system *h, int nt, int *iters);
 
 /* Deterministic initial configuration (the reference's overlap-rejected
  * placement, Filament.cpp:8-153, filament_initialisation_functions.cpp:

@@ -125,6 +125,18 @@ struct LinkedList {
   void mpi_broadcast() {}
   // LinkedList::apply_collision_barrier (CollisionBarrierFilament.cpp:24-271)
   void apply_collision_barrier(Fcm &fcm, int nt) {
+    masked(fcm, nt, FS_BARRIER_ALL);
+  }
+  // apply_collision_barrier_not_swimmer (:441-696)
+  void apply_collision_barrier_not_swimmer(Fcm &fcm, int nt) {
+    masked(fcm, nt, FS_BARRIER_NOT_SWIMMER);
+  }
+  // apply_collision_barrier_not_test_particles (:698-945)
+  void apply_collision_barrier_not_test_particles(Fcm &fcm, int nt) {
+    masked(fcm, nt, FS_BARRIER_NOT_TEST_PARTICLES);
+  }
+  void masked(Fcm &fcm, int nt, int mask) {
+    check(fs_colbar_set_mask(h, mask), "fs_colbar_set_mask");
     check(fs_colbar_apply_host(h, &fcm.Y[0][0], nt, &fcm.F[0][0], &min_dist2),
           "apply_collision_barrier");
   }
@@ -185,6 +197,23 @@ struct Filaments {
   int timestep(int nt) {
     int it = 0;
     check(fs_system_timestep(h, nt, &it), "timestep");
+    return it;
+  }
+  // main_gaitreset.cpp:310-812: pass gait_reset_i of timestep nt (0: no
+  // barrier, 1: no swimmer/network pairs, both trial passes without
+  // Filament::step; 2: the full barrier and the committed step)
+  int gaitResetPass(int nt, int gait_reset_i) {
+    int it = 0;
+    if (gait_reset_i == 0)
+      check(fs_system_set_barrier(h, 0, FS_BARRIER_ALL), "set_barrier");
+    else
+      check(fs_system_set_barrier(
+                h, 1, gait_reset_i == 1 ? FS_BARRIER_NOT_SWIMMER : FS_BARRIER_ALL),
+            "set_barrier");
+    if (gait_reset_i < 2)
+      check(fs_system_timestep_trial(h, nt, &it), "timestep_trial");
+    else
+      check(fs_system_timestep(h, nt, &it), "timestep");
     return it;
   }
   void free_memory() {

@@ -10,13 +10,17 @@
  *   link (LIFO head/list)      :370-433
  *   apply_collision_barrier    :24-271 (same traversal and accumulation
  *                              order, so forces match the reference's)
+ *   the _not_swimmer (:441-696) and _not_test_particles (:698-945)
+ *   variants                   the same walk with a pair mask
  * The MPI cell-range split (:284-289) collapses to one rank owning every
  * cell; the Allreduce MIN/SUM (:250,256) become the local values.
  *
  * Parity pins (tests/test_oracle_colbar.py): the pair set equals a
  * brute-force O(N^2) minimum-image enumeration (or_colbar_pairs_brute);
  * Newton's third law (sum of forces = 0); closed-form pair force on both
- * sides of the distance cap; head/list LIFO order by construction.
+ * sides of the distance cap; head/list LIFO order by construction; a masked
+ * pair set equals the brute-force set filtered by the mask's rule, and the
+ * masked forces equal those of a barrier over the kept pairs alone.
  */
 #include "oracle.h"
 
@@ -27,6 +31,7 @@
 struct or_colbar {
   fs_colbar_params p;
   int np;
+  int mask; /* FS_BARRIER_* */
   int M[3];
   int ncell;
   int *map, *head, *list;
@@ -131,6 +136,19 @@ typedef struct {
   int nt, ifs;
 } pairconst;
 
+void or_colbar_set_mask(or_colbar *h, int mask) { h->mask = mask; }
+
+/* The variants' pair masks, tested before anything else of a pair:
+ * apply_collision_barrier_not_swimmer (CollisionBarrierFilament.cpp:495,
+ * 585), apply_collision_barrier_not_test_particles (:754, 840). */
+static int mask_in(const or_colbar *h, int i, int j) {
+  if (h->mask == FS_BARRIER_NOT_SWIMMER)
+    return (h->sfil[i] == 0) == (h->sfil[j] == 0);
+  if (h->mask == FS_BARRIER_NOT_TEST_PARTICLES)
+    return i < h->np - 2 && j < h->np - 2;
+  return 1;
+}
+
 static pairconst make_pc(const fs_colbar_params *p, int nt) {
   pairconst c;
   c.Lx = p->box[0];
@@ -195,7 +213,8 @@ double or_colbar_apply(or_colbar *h, const double *Y, int nt, double *F) {
         int j = (pass == 0) ? h->list[i] : h->head[h->map[13 * ic + pass - 1]];
         while (j > -1) {
           double rsq, f[3];
-          if (pair_force(&c, Y + 3 * i, Y + 3 * j, h->sfil[i], h->sseg[i],
+          if (mask_in(h, i, j) &&
+              pair_force(&c, Y + 3 * i, Y + 3 * j, h->sfil[i], h->sseg[i],
                          h->sfil[j], h->sseg[j], &rsq, f)) {
             if (rsq < min_dist2) min_dist2 = rsq;
             if (rsq < c.chiRnm2) {
@@ -246,7 +265,8 @@ int or_colbar_pairs(or_colbar *h, const double *Y, int nt, int max_pairs,
         int j = (pass == 0) ? h->list[i] : h->head[h->map[13 * ic + pass - 1]];
         for (; j > -1; j = h->list[j]) {
           double rsq, f[3];
-          if (pair_force(&c, Y + 3 * i, Y + 3 * j, h->sfil[i], h->sseg[i],
+          if (mask_in(h, i, j) &&
+              pair_force(&c, Y + 3 * i, Y + 3 * j, h->sfil[i], h->sseg[i],
                          h->sfil[j], h->sseg[j], &rsq, f) &&
               rsq < c.chiRnm2)
             push_pair(pairs, max_pairs, &n, i, j);

@@ -272,6 +272,7 @@ struct or_sim {
   double *Y, *Fa, *Ta, *V, *W;
   double *E, *NewE, *Upd, *tmp, *dE;
   or_colbar *cb;
+  int barrier_off; /* gait-reset pass without the barrier */
   int *sfil, *sseg;
   /* spring links: global bead pairs, in link order */
   int nspring;
@@ -1203,13 +1204,20 @@ double or_sim_barrier(or_sim *h, int nt, int link) {
   return or_colbar_apply(h->cb, h->Y, nt, h->Fa);
 }
 
-/* main.cpp:284-754 */
-int or_sim_timestep(or_sim *h, int nt) {
+void or_sim_set_barrier(or_sim *h, int on, int mask) {
+  h->barrier_off = !on;
+  or_colbar_set_mask(h->cb, mask);
+}
+
+/* main.cpp:284-754; commit = 0: a trial pass of main_gaitreset.cpp
+ * (gait_reset_i < 2), which skips Filament::step (:775-781) */
+static int timestep_impl(or_sim *h, int nt, int commit) {
   size_t N6 = 6 * (size_t)h->np;
+  const int bar = h->p.barrier_on && !h->barrier_off;
   or_sim_initial_guess(h);
   or_sim_forces(h, nt);
   or_sim_build_jacobian(h, nt);
-  if (h->p.barrier_on) or_sim_barrier(h, nt, 1);
+  if (bar) or_sim_barrier(h, nt, 1);
   or_sim_mobility(h, nt);
   int check = or_sim_residual(h, nt, h->V, h->W, h->E, NULL);
   int iter = 0;
@@ -1217,13 +1225,18 @@ int or_sim_timestep(or_sim *h, int nt) {
     or_sim_jsolve(h, h->E, h->Upd, iter);
     or_sim_update(h, h->Upd);
     or_sim_forces(h, nt);
-    if (h->p.barrier_on) or_sim_barrier(h, nt, 0);
+    if (bar) or_sim_barrier(h, nt, 0);
     or_sim_mobility(h, nt);
     check = or_sim_residual(h, nt, h->V, h->W, h->NewE, NULL);
     or_sim_broyden_update(h, h->E, h->NewE, h->Upd, iter);
     memcpy(h->E, h->NewE, sizeof(double) * N6);
     ++iter;
   }
-  or_sim_step(h, nt);
+  if (commit) or_sim_step(h, nt);
   return iter;
+}
+
+int or_sim_timestep(or_sim *h, int nt) { return timestep_impl(h, nt, 1); }
+int or_sim_timestep_trial(or_sim *h, int nt) {
+  return timestep_impl(h, nt, 0);
 }

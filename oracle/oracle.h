@@ -57,6 +57,8 @@ void or_colbar_link(or_colbar *h, const double *Xs);
 void or_colbar_get_lists(or_colbar *h, int *head, int *list);
 /* F += barrier forces; returns min_dist2. */
 double or_colbar_apply(or_colbar *h, const double *Y, int nt, double *F);
+/* Pair mask (FS_BARRIER_*) of later apply / pairs calls. */
+void or_colbar_set_mask(or_colbar *h, int mask);
 /* Pairs within cutoff via the cell list (i<j sorted); returns count
  * (writes at most max_pairs). */
 int or_colbar_pairs(or_colbar *h, const double *Y, int nt, int max_pairs,
@@ -76,6 +78,10 @@ void or_sim_set_state(or_sim *h, const fs_state_ptrs *s);
 void or_sim_get_state(or_sim *h, fs_state_ptrs *s);
 /* One whole timestep (main.cpp:284-754); returns Broyden iterations. */
 int or_sim_timestep(or_sim *h, int nt);
+/* The same without the final step (main_gaitreset.cpp trial passes). */
+int or_sim_timestep_trial(or_sim *h, int nt);
+/* Barrier of later timesteps: on = 0 skips it; mask: FS_BARRIER_*. */
+void or_sim_set_barrier(or_sim *h, int on, int mask);
 /* Pieces. */
 void or_sim_initial_guess(or_sim *h);
 void or_sim_forces(or_sim *h, int nt);

@@ -50,6 +50,7 @@ def lib():
         "or_colbar_link": (None, [P, _DP]),
         "or_colbar_get_lists": (None, [P, _IP, _IP]),
         "or_colbar_apply": (C.c_double, [P, _DP, C.c_int, _DP]),
+        "or_colbar_set_mask": (None, [P, C.c_int]),
         "or_colbar_pairs": (C.c_int, [P, _DP, C.c_int, C.c_int, _IP]),
         "or_colbar_pairs_brute": (C.c_int, [C.POINTER(abi.ColbarParams),
                                             C.c_int, _DP, _IP, _IP, C.c_int,
@@ -61,6 +62,8 @@ def lib():
         "or_sim_set_state": (None, [P, C.POINTER(abi.StatePtrs)]),
         "or_sim_get_state": (None, [P, C.POINTER(abi.StatePtrs)]),
         "or_sim_timestep": (C.c_int, [P, C.c_int]),
+        "or_sim_timestep_trial": (C.c_int, [P, C.c_int]),
+        "or_sim_set_barrier": (None, [P, C.c_int, C.c_int]),
         "or_sim_initial_guess": (None, [P]),
         "or_sim_forces": (None, [P, C.c_int]),
         "or_sim_set_spring_links": (C.c_int, [P, C.c_int, _IP, C.c_double,
@@ -211,6 +214,9 @@ class Colbar:
         md = lib().or_colbar_apply(self.h, dp(Y), nt, dp(F))
         return F, md
 
+    def set_mask(self, mask):
+        lib().or_colbar_set_mask(self.h, mask)
+
     def pairs(self, Y, nt, max_pairs=1 << 20):
         Y = np.ascontiguousarray(Y, dtype=np.float64)
         out = np.zeros(2 * max_pairs, dtype=np.int32)
@@ -286,6 +292,12 @@ class Sim:
 
     def timestep(self, nt):
         return lib().or_sim_timestep(self.h, nt)
+
+    def timestep_trial(self, nt):
+        return lib().or_sim_timestep_trial(self.h, nt)
+
+    def set_barrier(self, on, mask=abi.FS_BARRIER_ALL):
+        lib().or_sim_set_barrier(self.h, 1 if on else 0, mask)
 
     def initial_guess(self):
         lib().or_sim_initial_guess(self.h)

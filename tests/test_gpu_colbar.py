@@ -95,3 +95,41 @@ def test_stale_list(cuda):
     Y1 = Y0 + np.random.default_rng(2).normal(0, 1.0, Y0.shape)
     assert np.array_equal(gb.pairs(to_dev(torch, Y1), 5), ob.pairs(Y1, 5))
     gb.close()
+
+
+@pytest.mark.parametrize("mask", [1, 2])
+def test_masked_variants(cuda, mask):
+    """apply_collision_barrier_not_swimmer (mask 1) and
+    _not_test_particles (mask 2) against the oracle: pair sets exact, forces
+    to 1e-12, minimum distance exact; the mask removes pairs inside the
+    cutoff (filament 0 beside filament 1, the last two segments beside
+    segments 3 and 14)."""
+    from test_oracle_colbar import _crowded, _kept
+    torch = cuda
+    fp, cp, X, sf, si = _crowded()
+    sf, si = sf.astype(np.int32), si.astype(np.int32)
+    Y = np.fmod(X, np.array(params.box(fp)))
+    nt = 3
+    allp = O.pairs_brute(cp, Y, sf, si, nt)
+    keep = _kept(mask, allp, sf)
+    assert 0 < keep.sum() < len(allp)
+    ob = O.Colbar(cp, sf, si)
+    ob.link(X)
+    ob.set_mask(mask)
+    gb = api.Colbar(cp, sf, si)
+    dX, dY = to_dev(torch, X), to_dev(torch, Y)
+    gb.link(dX)
+    gb.set_mask(mask)
+    p_g = gb.pairs(dY, nt)
+    assert np.array_equal(p_g, ob.pairs(Y, nt))
+    assert np.array_equal(p_g, allp[keep])
+    F0 = np.random.default_rng(2).standard_normal(Y.shape)
+    Fo, md_o = ob.apply(Y, nt, F0)
+    dF = to_dev(torch, F0)
+    md_g = gb.apply(dY, nt, dF)
+    Fg = dF.cpu().numpy()
+    assert md_g == md_o
+    assert rel(Fg - F0, Fo - F0) <= 1e-12
+    with pytest.raises(Exception):
+        gb.set_mask(3)
+    gb.close()

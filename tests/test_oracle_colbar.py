@@ -4,7 +4,8 @@ import numpy as np
 import pytest
 
 import _oracle as O
-from filsim_amd import api, params
+from filsim_amd import abi, api, params
+from helpers import rel
 
 PI = 3.14159265358979323846
 
@@ -95,3 +96,64 @@ def test_adjacent_exclusion_and_friction_steps():
         cb = O.Colbar(cp, [0, 0, 0], [0, 1, 2])
         cb.link(Y)
         assert len(cb.pairs(Y, nt)) == expect
+
+
+def _crowded(nfil=24, nw=10, seed=7):
+    """Filaments of a 48^3 box with filament 0 laid 1.5 beside filament 1
+    and the last two segments moved next to segments 3 and 14, so that both
+    masks remove pairs inside the cutoff."""
+    fp, cp, X, sf, si = _setup(48, nfil, nw, seed)
+    X[:nw] = X[nw:2 * nw] + np.array([1.5, 0.0, 0.0])
+    X[-2] = X[3] + np.array([0.0, 1.6, 0.0])
+    X[-1] = X[14] + np.array([0.0, 0.0, 1.7])
+    return fp, cp, X, sf, si
+
+
+def _kept(mask, pairs, sf):
+    n = len(sf)
+    if mask == abi.FS_BARRIER_NOT_SWIMMER:
+        return (sf[pairs[:, 0]] == 0) == (sf[pairs[:, 1]] == 0)
+    return (pairs[:, 0] < n - 2) & (pairs[:, 1] < n - 2)
+
+
+@pytest.mark.parametrize("mask", [abi.FS_BARRIER_NOT_SWIMMER,
+                                  abi.FS_BARRIER_NOT_TEST_PARTICLES])
+def test_masked_variants(mask):
+    """apply_collision_barrier_not_swimmer / _not_test_particles
+    (CollisionBarrierFilament.cpp:441-945): the masked pair set is the
+    brute-force set filtered by the mask's rule, the forces are the
+    closed-form pair forces summed over the kept pairs only, and the minimum
+    distance is taken over the kept pairs."""
+    fp, cp, X, sf, si = _crowded()
+    box = np.array(params.box(fp))
+    Y = np.fmod(X, box)
+    nt = 3
+    allp = O.pairs_brute(cp, Y, sf, si, nt)
+    keep = _kept(mask, allp, sf)
+    assert 0 < keep.sum() < len(allp)
+    cb = O.Colbar(cp, sf, si)
+    cb.link(X)
+    cb.set_mask(mask)
+    assert np.array_equal(cb.pairs(Y, nt), allp[keep])
+    F, md = cb.apply(Y, nt, np.zeros_like(Y))
+    R2 = (2 * cp.a) ** 2
+    chi = 1.21 * R2
+    Fe = np.zeros_like(Y)
+    r2min = np.inf
+    for i, j in allp[keep]:
+        d = Y[i] - Y[j]
+        d = d - box * np.floor(d / box + 0.5)
+        r2 = float(d @ d)
+        r2min = min(r2min, r2)
+        if r2 < cp.cap ** 2:
+            d = d / np.sqrt(r2) * cp.cap
+            r2 = cp.cap ** 2
+        f = cp.fs_over_2a * ((chi - r2) / (chi - R2)) ** 4 * d
+        Fe[i] += f
+        Fe[j] -= f
+    assert rel(F, Fe) <= 1e-12
+    assert np.isclose(md, r2min, rtol=1e-14)
+    # the full barrier sees the removed pairs
+    cb.set_mask(abi.FS_BARRIER_ALL)
+    F_all, md_all = cb.apply(Y, nt, np.zeros_like(Y))
+    assert rel(F_all, F) > 1e-3 and md_all <= md

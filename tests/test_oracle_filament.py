@@ -243,3 +243,53 @@ def test_spring_links_hookes_law():
     # the FCM forces carry them (assign_filament_data after the springs)
     Fa = s1.fcm_arrays()[1]
     assert np.array_equal(Fa, s1.get_state()["F"])
+
+
+def test_trial_step_is_the_step_without_step():
+    """A gait-reset trial pass (main_gaitreset.cpp:310-812, gait_reset_i < 2)
+    runs the whole solve and skips Filament::step (:775-781): the committed
+    state and the lambda history stay bit for bit, the trial's iterate is
+    exactly the position the committed step moves to, and lam carries the
+    trial's result (Filament.hpp:226-236 does not reset it)."""
+    sp, nws, props, st = config(grid=48, nfil=5, nw=8, swim=True, seed=4,
+                                tol=1e-9, ifs=100)
+    a, b = O.Sim(sp, nws, props), O.Sim(sp, nws, props)
+    for s in (a, b):
+        s.set_state(**st)
+        s.timestep(1)
+        s.timestep(2)
+    s0 = a.get_state()
+    ia = a.timestep_trial(3)
+    s1 = a.get_state()
+    for k in ("X", "Xt", "U", "Ut", "q", "qt", "lam1", "lam2"):
+        assert np.array_equal(s0[k], s1[k]), k
+    ib = b.timestep(3)
+    s2 = b.get_state()
+    assert ia == ib
+    for k in ("Xs", "Us", "qs", "lam"):
+        assert np.array_equal(s1[k], s2[k]), k
+    assert np.array_equal(s1["Xs"], s2["X"]) and not np.array_equal(s1["Xs"],
+                                                                     s1["X"])
+
+
+def test_barrier_switch_equals_barrier_off_system():
+    """set_barrier(0) steps exactly like a system built without the barrier;
+    set_barrier(1, FS_BARRIER_ALL) exactly like the default."""
+    from filsim_amd import abi
+    sp, nws, props, st = config(grid=48, nfil=5, nw=8, swim=True, seed=4,
+                                tol=1e-9, ifs=100)
+    sp_off, *_ = config(grid=48, nfil=5, nw=8, swim=True, seed=4, tol=1e-9,
+                        ifs=100, barrier=0)
+    on, off, sw = (O.Sim(sp, nws, props), O.Sim(sp_off, nws, props),
+                   O.Sim(sp, nws, props))
+    for s in (on, off, sw):
+        s.set_state(**st)
+    for nt in (1, 2):
+        sw.set_barrier(0)
+        i_sw, i_off = sw.timestep(nt), off.timestep(nt)
+        assert i_sw == i_off
+        assert np.array_equal(sw.get_state()["X"], off.get_state()["X"])
+    sw.set_barrier(1, abi.FS_BARRIER_ALL)
+    on.set_state(**{k: v for k, v in sw.get_state().items()})
+    assert sw.timestep(3) == on.timestep(3)
+    assert np.array_equal(sw.get_state()["X"], on.get_state()["X"])
