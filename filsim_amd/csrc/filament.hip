@@ -854,15 +854,8 @@ __global__ void __launch_bounds__(1024)
 //      column k (base 0, r = 1/pivot) and row p (r_j) are fixed under
 //      wave-uniform conditions; the owners of column k + 1 publish it and
 //      its keys in the other buffers; barrier.
-template <int NB>
-__device__ __forceinline__ double pick(const double (&row)[NB], int b) {
-  double v = 0.0;
-#pragma unroll
-  for (int q = 0; q < NB; ++q)
-    if (q == b) v = row[q];
-  return v;
-}
-
+// The loop over pivot columns runs in unrolled blocks of 32, so column k's
+// registers are known at compile time (selects on 6, not 36, registers).
 // pivot key of row i (< 256) holding v; 0 marks a used row
 __device__ __forceinline__ unsigned long long pivot_key(double v, int i,
                                                         bool used) {
@@ -935,69 +928,74 @@ __global__ void __launch_bounds__(1024)
     }
   }
   __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    const int kb = k >> 5, kt = k & 31;
-    const double *cb = colbuf[k & 1];
-    const unsigned long long *ck = colkey[k & 1];
-    unsigned long long key = 0;
-    for (int i = lane; i < 32 * NB; i += 64) key = ck[i] > key ? ck[i] : key;
-    key = wave_max_u64(key);
-    int p = 255 - (int)(key & 0xFF);
-    const bool bad = (key >> 8) == 0 || p >= n;  // no nonzero unused entry
-    p = bad ? n - 1 - k : p;  // keeps indices in range; result is flagged
-    const int pb = p >> 5, pt = p & 31;
-    if (ty == pt) {
+  // pivot columns in blocks of 32: the block loop is unrolled, so inside
+  // each block column k's registers are A[.][KB] and the next column's
+  // A[.][KB] or A[.][KB + 1] with KB a constant, not selects over all of A
 #pragma unroll
-      for (int a = 0; a < NB; ++a)
-        if (a == pb) {
+  for (int KB = 0; KB < NB; ++KB) {
+    const int KB1 = KB + 1 < NB ? KB + 1 : KB;
+    const int kend = min(32, n - 32 * KB);
+    for (int kt = 0; kt < kend; ++kt) {
+      const int k = 32 * KB + kt;
+      const double *cb = colbuf[k & 1];
+      const unsigned long long *ck = colkey[k & 1];
+      unsigned long long key = 0;
+      for (int i = lane; i < 32 * NB; i += 64) key = ck[i] > key ? ck[i] : key;
+      key = wave_max_u64(key);
+      int p = 255 - (int)(key & 0xFF);
+      const bool bad = (key >> 8) == 0 || p >= n;  // no nonzero unused entry
+      p = bad ? n - 1 - k : p;  // keeps indices in range; result is flagged
+      const int pb = p >> 5, pt = p & 31;
+      if (ty == pt) {
 #pragma unroll
-          for (int b = 0; b < NB; ++b) rowbuf[tx + 32 * b] = A[a][b];
-        }
-    }
-    if (tid == 0) {
-      if (bad) atomicExch(singular, 1);
-      pivrow[k] = p;
-      rowstep[p] = k;
-    }
-    __syncthreads();
-    const double inv = 1.0 / rowbuf[k];
-    double r[NB];
+        for (int a = 0; a < NB; ++a)
+          if (a == pb) {
 #pragma unroll
-    for (int b = 0; b < NB; ++b) r[b] = rowbuf[tx + 32 * b] * inv;
-    const bool colk = (tx == kt);
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-      if (b == kb) {
-        r[b] = colk ? inv : r[b];
-#pragma unroll
-        for (int a = 0; a < NB; ++a) A[a][b] = colk ? 0.0 : A[a][b];
+            for (int b = 0; b < NB; ++b) rowbuf[tx + 32 * b] = A[a][b];
+          }
       }
-#pragma unroll
-    for (int a = 0; a < NB; ++a) {
-      const double fct = cb[ty + 32 * a];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) A[a][b] = fma(-fct, r[b], A[a][b]);
-    }
-    const bool rowp = (ty == pt);
-#pragma unroll
-    for (int a = 0; a < NB; ++a)
-      if (a == pb) {
-#pragma unroll
-        for (int b = 0; b < NB; ++b) A[a][b] = rowp ? r[b] : A[a][b];
+      if (tid == 0) {
+        if (bad) atomicExch(singular, 1);
+        pivrow[k] = p;
+        rowstep[p] = k;
       }
-    if (tid == 0) used[p] = 1;
-    const int k1 = k + 1;
-    if (k1 < n && tx == (k1 & 31)) {
+      __syncthreads();
+      const double inv = 1.0 / rowbuf[k];
+      double r[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) r[b] = rowbuf[tx + 32 * b] * inv;
+      const bool colk = (tx == kt);
+      r[KB] = colk ? inv : r[KB];
+#pragma unroll
+      for (int a = 0; a < NB; ++a) A[a][KB] = colk ? 0.0 : A[a][KB];
 #pragma unroll
       for (int a = 0; a < NB; ++a) {
-        const int i = ty + 32 * a;
-        const double v = pick<NB>(A[a], k1 >> 5);
-        colbuf[k1 & 1][i] = v;
-        // used[p] of this step may not be visible yet: test i == p too
-        colkey[k1 & 1][i] = pivot_key(v, i, i >= n || i == p || used[i]);
+        const double fct = cb[ty + 32 * a];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) A[a][b] = fma(-fct, r[b], A[a][b]);
       }
+      if (ty == pt) {
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+          if (a == pb) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) A[a][b] = r[b];
+          }
+      }
+      if (tid == 0) used[p] = 1;
+      const int k1 = k + 1;
+      if (k1 < n && tx == (k1 & 31)) {
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          const int i = ty + 32 * a;
+          const double v = kt == 31 ? A[a][KB1] : A[a][KB];
+          colbuf[k1 & 1][i] = v;
+          // used[p] of this step may not be visible yet: test i == p too
+          colkey[k1 & 1][i] = pivot_key(v, i, i >= n || i == p || used[i]);
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int a = 0; a < NB; ++a)
