@@ -846,10 +846,11 @@ __global__ void __launch_bounds__(1024)
 // and columns past n are zero padding). Pivot: largest |value| among unused
 // rows, compared on |value| with its 8 lowest mantissa bits dropped (ties
 // within 2^-44 go to the lowest row). Per pivot step:
-//   1. every wave reduces the 64-bit keys of column k (published with the
-//      column in colkey/colbuf[k & 1]) to the same pivot p with DPP row
-//      permutes and four lane reads, so no barrier separates the search from
-//      its use; the owners of row p publish it in rowbuf; barrier;
+//   1. every wave reduces the 64-bit keys of column k (each row group's
+//      largest, published with the column in colkey/colbuf[k & 1]) to the
+//      same pivot p with DPP row permutes and four lane reads, so no
+//      barrier separates the search from its use; the owners of row p
+//      publish it in rowbuf; barrier;
 //   2. branch-free update A -= fct_i * r_j with r_j = A[p][j] / A[p][k];
 //      column k (base 0, r = 1/pivot) and row p (r_j) are fixed under
 //      wave-uniform conditions; the owners of column k + 1 publish it and
@@ -897,12 +898,12 @@ __device__ __forceinline__ unsigned long long wave_max_u64(
   return best;
 }
 
-template <int NB>
-__global__ void __launch_bounds__(1024)
+template <int NB, int TY>
+__global__ void __launch_bounds__(32 * TY)
     k_gj_inverse_reg(SysDev S, const double *__restrict__ Jall,
                      double *__restrict__ JTall, int *__restrict__ singular) {
   __shared__ double colbuf[2][32 * NB], rowbuf[32 * NB];
-  __shared__ unsigned long long colkey[2][32 * NB];
+  __shared__ unsigned long long colkey[2][TY];  // max key per row group
   __shared__ int used[32 * NB], pivrow[32 * NB], rowstep[32 * NB];
   const int f = blockIdx.x;
   const int nw = S.nw[f];
@@ -910,22 +911,26 @@ __global__ void __launch_bounds__(1024)
   const double *J = Jall + S.joff[f];
   double *JT = JTall + S.joff[f];
   const int tid = threadIdx.x, ty = tid >> 5, tx = tid & 31, lane = tid & 63;
-  double A[NB][NB];
+  constexpr int RA = 32 * NB / TY;  // row blocks per thread
+  double A[RA][NB];
 #pragma unroll
-  for (int a = 0; a < NB; ++a)
+  for (int a = 0; a < RA; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      int i = ty + 32 * a, j = tx + 32 * b;
+      int i = ty + TY * a, j = tx + 32 * b;
       A[a][b] = (i < n && j < n) ? J[(size_t)i * n + j] : 0.0;
     }
   for (int i = tid; i < 32 * NB; i += blockDim.x) used[i] = (i < n) ? 0 : 1;
   if (tx == 0) {
+    unsigned long long km = 0;
 #pragma unroll
-    for (int a = 0; a < NB; ++a) {
-      const int i = ty + 32 * a;
+    for (int a = 0; a < RA; ++a) {
+      const int i = ty + TY * a;
       colbuf[0][i] = A[a][0];
-      colkey[0][i] = pivot_key(A[a][0], i, i >= n);
+      const unsigned long long kk = pivot_key(A[a][0], i, i >= n);
+      km = kk > km ? kk : km;
     }
+    colkey[0][ty] = km;
   }
   __syncthreads();
   // pivot columns in blocks of 32: the block loop is unrolled, so inside
@@ -939,16 +944,15 @@ __global__ void __launch_bounds__(1024)
       const int k = 32 * KB + kt;
       const double *cb = colbuf[k & 1];
       const unsigned long long *ck = colkey[k & 1];
-      unsigned long long key = 0;
-      for (int i = lane; i < 32 * NB; i += 64) key = ck[i] > key ? ck[i] : key;
+      unsigned long long key = lane < TY ? ck[lane] : 0ull;
       key = wave_max_u64(key);
       int p = 255 - (int)(key & 0xFF);
       const bool bad = (key >> 8) == 0 || p >= n;  // no nonzero unused entry
       p = bad ? n - 1 - k : p;  // keeps indices in range; result is flagged
-      const int pb = p >> 5, pt = p & 31;
+      const int pb = p / TY, pt = p % TY;
       if (ty == pt) {
 #pragma unroll
-        for (int a = 0; a < NB; ++a)
+        for (int a = 0; a < RA; ++a)
           if (a == pb) {
 #pragma unroll
             for (int b = 0; b < NB; ++b) rowbuf[tx + 32 * b] = A[a][b];
@@ -967,16 +971,16 @@ __global__ void __launch_bounds__(1024)
       const bool colk = (tx == kt);
       r[KB] = colk ? inv : r[KB];
 #pragma unroll
-      for (int a = 0; a < NB; ++a) A[a][KB] = colk ? 0.0 : A[a][KB];
+      for (int a = 0; a < RA; ++a) A[a][KB] = colk ? 0.0 : A[a][KB];
 #pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        const double fct = cb[ty + 32 * a];
+      for (int a = 0; a < RA; ++a) {
+        const double fct = cb[ty + TY * a];
 #pragma unroll
         for (int b = 0; b < NB; ++b) A[a][b] = fma(-fct, r[b], A[a][b]);
       }
       if (ty == pt) {
 #pragma unroll
-        for (int a = 0; a < NB; ++a)
+        for (int a = 0; a < RA; ++a)
           if (a == pb) {
 #pragma unroll
             for (int b = 0; b < NB; ++b) A[a][b] = r[b];
@@ -985,23 +989,27 @@ __global__ void __launch_bounds__(1024)
       if (tid == 0) used[p] = 1;
       const int k1 = k + 1;
       if (k1 < n && tx == (k1 & 31)) {
+        unsigned long long km = 0;
 #pragma unroll
-        for (int a = 0; a < NB; ++a) {
-          const int i = ty + 32 * a;
+        for (int a = 0; a < RA; ++a) {
+          const int i = ty + TY * a;
           const double v = kt == 31 ? A[a][KB1] : A[a][KB];
           colbuf[k1 & 1][i] = v;
           // used[p] of this step may not be visible yet: test i == p too
-          colkey[k1 & 1][i] = pivot_key(v, i, i >= n || i == p || used[i]);
+          const unsigned long long kk =
+              pivot_key(v, i, i >= n || i == p || used[i]);
+          km = kk > km ? kk : km;
         }
+        colkey[k1 & 1][ty] = km;
       }
       __syncthreads();
     }
   }
 #pragma unroll
-  for (int a = 0; a < NB; ++a)
+  for (int a = 0; a < RA; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      int i = ty + 32 * a, j = tx + 32 * b;
+      int i = ty + TY * a, j = tx + 32 * b;
       if (i < n && j < n) {
         int r = rowstep[i], c = pivrow[j];
         double v = A[a][b];
@@ -1576,14 +1584,18 @@ int SystemImpl::build_jacobian_dev(int nt) {
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, sing, 1, 0);
   const int nmax = 6 * max_nw;
   const int NB = (nmax + 31) / 32;
-  if (NB <= 6) {  // whole matrix in registers of one 1024-thread workgroup
+  if (NB <= 6) {  // whole matrix in registers of one workgroup
+    // TY row groups of 32 threads: 32 (6 x 6 tiles); 16 (12 x 6 tiles, two
+    // waves per SIMD) measured slower at 180 x 180 (2.22 vs 1.94 ms for the
+    // build and inverse of 1,000 Jacobians)
+    constexpr int kTY56 = 32;
     switch (NB) {
       case 1:
-      case 2: hipLaunchKernelGGL(k_gj_inverse_reg<2>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
-      case 3: hipLaunchKernelGGL(k_gj_inverse_reg<3>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
-      case 4: hipLaunchKernelGGL(k_gj_inverse_reg<4>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
-      case 5: hipLaunchKernelGGL(k_gj_inverse_reg<5>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
-      default: hipLaunchKernelGGL(k_gj_inverse_reg<6>, dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+      case 2: hipLaunchKernelGGL((k_gj_inverse_reg<2, 32>), dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+      case 3: hipLaunchKernelGGL((k_gj_inverse_reg<3, 32>), dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+      case 4: hipLaunchKernelGGL((k_gj_inverse_reg<4, 32>), dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+      case 5: hipLaunchKernelGGL((k_gj_inverse_reg<5, kTY56>), dim3(nf), dim3(32 * kTY56), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
+      default: hipLaunchKernelGGL((k_gj_inverse_reg<6, kTY56>), dim3(nf), dim3(32 * kTY56), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
     }
   } else {  // larger filaments: the matrix stays in global memory (L2)
     FS_HIP(hipMemcpyAsync(Mb.p, Jb.p, sizeof(double) * nJ,
