@@ -8,7 +8,9 @@
 //   spread    gather form of mono_and_dipole_force_distribution
 //             (:596-646): one workgroup per grid tile sums, in a fixed
 //             order, every segment whose stencil covers the tile, then
-//             writes each grid point once (no atomics, no zeroing pass)
+//             writes each grid point once (no atomics, no zeroing pass);
+//             by default as per-tile matrix-core products (k_spread_mfma,
+//             both precisions), else with the vector gather (k_spread)
 //   fft_z_r2c real->half-complex along z, two real rows per complex FFT
 //   fft_y     strided complex FFT along y (forward)
 //   fft_x_prj forward FFT along x, the Stokes projection of flow_solve
