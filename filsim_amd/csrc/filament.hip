@@ -1747,14 +1747,12 @@ int SystemImpl::timestep_trial(int nt, int *iters) {
 
 int SystemImpl::set_barrier(int on, int mask) {
   FS_CHECK(on == 0 || on == 1, FS_ERR_ARG, "on must be 0 or 1");
+  FS_CHECK(mask == FS_BARRIER_ALL || mask == FS_BARRIER_NOT_SWIMMER ||
+               mask == FS_BARRIER_NOT_TEST_PARTICLES,
+           FS_ERR_ARG, "unknown barrier mask %d", mask);
   FS_CHECK(on == 0 || p.barrier_on, FS_ERR_STATE,
            "the system was created with barrier_on = 0");
-  if (p.barrier_on) {
-    FS_CHECK(mask == FS_BARRIER_ALL || mask == FS_BARRIER_NOT_SWIMMER ||
-                 mask == FS_BARRIER_NOT_TEST_PARTICLES,
-             FS_ERR_ARG, "unknown barrier mask %d", mask);
-    cb.mask = mask;
-  }
+  if (p.barrier_on) cb.mask = mask;
   barrier_off = (on == 0);
   return FS_OK;
 }

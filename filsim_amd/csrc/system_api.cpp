@@ -119,8 +119,11 @@ extern "C" int fs_system_fcm_arrays(fs_system *h, double **Y, double **F,
   return FS_OK;
 }
 
-extern "C" int fs_system_timestep(fs_system *h, int nt, int *iters) {
-  FS_CHECK(h, FS_ERR_ARG, "null handle");
+namespace {
+
+// One timestep (commit = 1) or gait-reset trial pass (commit = 0), with the
+// profiler's per-step buckets when profiling is on.
+int timestep_profiled(fs_system *h, int nt, int *iters, bool commit) {
   SystemImpl &S = h->impl;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (S.prof) {
@@ -129,8 +132,9 @@ extern "C" int fs_system_timestep(fs_system *h, int nt, int *iters) {
     t1 = S.prof_event();
     if (t0) FS_HIP(hipEventRecord(t0, S.st));
   }
-  int rc = h->ws.attached() ? h->ws.timestep(S, nt, iters)
-                            : S.timestep(nt, iters);
+  int rc = !commit ? S.timestep_trial(nt, iters)
+           : h->ws.attached() ? h->ws.timestep(S, nt, iters)
+                              : S.timestep(nt, iters);
   if (rc != FS_OK || !S.prof) return rc;
   if (t1) FS_HIP(hipEventRecord(t1, S.st));
   FS_HIP(hipStreamSynchronize(S.st));
@@ -139,6 +143,13 @@ extern "C" int fs_system_timestep(fs_system *h, int nt, int *iters) {
   if (t0 && t1) FS_HIP(hipEventElapsedTime(&total, t0, t1));
   S.pms[SystemImpl::P_TOTAL] = total;
   return FS_OK;
+}
+
+}  // namespace
+
+extern "C" int fs_system_timestep(fs_system *h, int nt, int *iters) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return timestep_profiled(h, nt, iters, true);
 }
 
 extern "C" int fs_system_set_spring_links(fs_system *h, int n,
@@ -180,7 +191,7 @@ extern "C" int fs_system_timestep_trial(fs_system *h, int nt, int *iters) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
   FS_CHECK(!h->ws.attached(), FS_ERR_STATE,
            "trial steps run on the plain path (detach the warm start)");
-  return h->impl.timestep_trial(nt, iters);
+  return timestep_profiled(h, nt, iters, false);
 }
 
 extern "C" int fs_system_set_warmstart(fs_system *h, fs_mlp *m) {

@@ -86,6 +86,11 @@ def test_trial_step_keeps_committed_state(cuda):
     for k in ("Xs", "Us", "qs", "lam"):
         assert np.array_equal(s1[k], s2[k]), k
     assert np.array_equal(s1["Xs"], s2["X"])
+    # a trial pass fills the profiler's buckets like a committed step
+    a.set_profiling(True)
+    a.timestep_trial(3)
+    pr = a.profile()
+    assert pr["Total"] > 0 and pr["Bld Jinv"] > 0 and pr["ColBarri"] > 0
     a.close()
     b.close()
 
