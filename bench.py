@@ -317,6 +317,62 @@ def c5_bench(args, torch):
                       "train_s": t_train, "holdout": hist[-1][2]}}
 
 
+def c3_bench(args, torch):
+    """Mixed precision against FP64 over the whole swimming run: 1,000
+    swimming filaments x 30 segments, 256^3, KB = 1000, dt = 0.005,
+    TOL 1e-6, barrier on, args.c3_steps steps from one initial state, the
+    FP32-FCM path and the FP64 path (whose mobility solve matches the FP64
+    oracle to 1e-12) side by side. Every 250 steps: the largest position and
+    orientation differences (positions relative to the box) and the
+    relative differences of the segment velocities U and angular velocities
+    Omega of the last mobility solve."""
+    import ctypes as C
+    from filsim_amd import abi, params
+    hip = C.CDLL("libamdhip64.so")
+
+    def d2h(ptr, n):
+        out = np.zeros(n)
+        rc = hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), ptr,
+                           C.c_size_t(8 * n), 2)
+        assert rc == 0, "hipMemcpy failed"
+        return out
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    s32, fp = build_system(abi.FS_FP32)
+    s64, _ = build_system(abi.FS_FP64)
+    L = max(params.box(fp))
+    n3 = 3 * 30000
+    rows, it32, it64 = [], 0, 0
+    t0 = time.perf_counter()
+    for nt in range(1, args.c3_steps + 1):
+        it32 += s32.timestep(nt)
+        it64 += s64.timestep(nt)
+        if nt % 250 == 0 or nt == args.c3_steps:
+            a, b = s32.get_state(), s64.get_state()
+            V32, W32 = (d2h(p, n3) for p in s32.fcm_ptrs()[3:5])
+            V64, W64 = (d2h(p, n3) for p in s64.fcm_ptrs()[3:5])
+            row = {"step": nt,
+                   "pos_max_over_box": float(np.abs(a["X"] - b["X"]).max() / L),
+                   "q_max": float(np.abs(a["q"] - b["q"]).max()),
+                   "U_rel": rel(V32, V64), "Omega_rel": rel(W32, W64),
+                   "mean_iters_fp32": it32 / nt, "mean_iters_fp64": it64 / nt}
+            rows.append(row)
+            log("c3", row, "%.0fs" % (time.perf_counter() - t0))
+    s32.close()
+    s64.close()
+    return {"metric": "mixed precision vs FP64 over the swimming run at "
+                      "256^3, 30k segments (errors; not a rate)",
+            "n_gpus": 1, "steps": args.c3_steps, "data": "synthetic",
+            "config": {"workload": "1000 swimming filaments x 30 segments, "
+                                   "256^3, KB=1000, dt=0.005, TOL=1e-6, "
+                                   "barrier on"},
+            "reference_path": "FP64 on the GPU (mobility = FP64 oracle to "
+                              "1e-12 per solve)",
+            "rows": rows, "wall_s": time.perf_counter() - t0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,10 +383,12 @@ def main():
     ap.add_argument("--record", type=int, default=100,
                     help="plain steps recorded to train the warm start")
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--c3-steps", type=int, default=2000,
+                    help="steps of the mixed-vs-FP64 swimming run (--only c3)")
     ap.add_argument("--skip", type=int, default=10,
                     help="start-up steps run before recording")
     ap.add_argument("--only", choices=["all", "mixed", "fp64", "mobility",
-                                       "copy", "batch", "c5"],
+                                       "copy", "batch", "c5", "c3"],
                     default="all")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -350,6 +408,9 @@ def main():
         return
     if args.only == "c5":
         print(json.dumps(c5_bench(args, torch)), flush=True)
+        return
+    if args.only == "c3":
+        print(json.dumps(c3_bench(args, torch)), flush=True)
         return
     if args.only == "copy":
         # calibration for HBM byte counters: a plain 1 GiB FP32 scale-by-one
