@@ -41,3 +41,28 @@ def test_mobility_symmetric_and_linear_384(cuda):
     e = float((V - V1).norm() / V1.norm())
     print("fp32 vs fp64 at 384^3", e)
     assert e <= 1e-5
+
+
+def test_mobility_256_30k_matches_oracle(cuda):
+    """The mobility microbenchmark's configuration itself: 30,000 segments
+    uniform in the periodic 256^3 box (dx = a/1.86, ngd = 12), F and T
+    i.i.d. N(0, 1), seed 20261202; FP64 against the oracle to 1e-12
+    normwise (element maximum reported), FP32 to 1e-5."""
+    import _oracle as O
+    from helpers import random_segments, rel
+    fp = params.fcm_params(256)
+    Y, F, T = random_segments(fp, 30000, seed=20261202)
+    Vo, Wo = O.mobility(fp, Y, F, T)
+    with api.Fcm(fp, 30000) as fcm:
+        V, W = fcm.mobility_host(Y, F, T)
+    ev, ew = rel(V, Vo), rel(W, Wo)
+    mv = float(np.abs(V - Vo).max() / np.abs(Vo).max())
+    mw = float(np.abs(W - Wo).max() / np.abs(Wo).max())
+    print("256^3 / 30k FP64: relV %.2e relW %.2e, element max %.2e %.2e"
+          % (ev, ew, mv, mw), flush=True)
+    assert ev <= 1e-12 and ew <= 1e-12
+    with api.Fcm(fp, 30000, precision=abi.FS_FP32) as f32:
+        V32, W32 = f32.mobility_host(Y, F, T)
+    print("256^3 / 30k FP32: relV %.2e relW %.2e"
+          % (rel(V32, Vo), rel(W32, Wo)), flush=True)
+    assert rel(V32, Vo) <= 1e-5 and rel(W32, Wo) <= 1e-5
