@@ -905,6 +905,7 @@ __global__ void __launch_bounds__(32 * TY)
   __shared__ double colbuf[2][32 * NB], rowbuf[32 * NB];
   __shared__ unsigned long long colkey[2][TY];  // max key per row group
   __shared__ int used[32 * NB], pivrow[32 * NB], rowstep[32 * NB];
+  __shared__ double s_inv;  // 1 / pivot of the current step
   const int f = blockIdx.x;
   const int nw = S.nw[f];
   const int n = 6 * nw;
@@ -956,6 +957,8 @@ __global__ void __launch_bounds__(32 * TY)
           if (a == pb) {
 #pragma unroll
             for (int b = 0; b < NB; ++b) rowbuf[tx + 32 * b] = A[a][b];
+            // the pivot's owner divides once, before the barrier
+            if (tx == kt) s_inv = 1.0 / A[a][KB];
           }
       }
       if (tid == 0) {
@@ -964,7 +967,7 @@ __global__ void __launch_bounds__(32 * TY)
         rowstep[p] = k;
       }
       __syncthreads();
-      const double inv = 1.0 / rowbuf[k];
+      const double inv = s_inv;
       double r[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) r[b] = rowbuf[tx + 32 * b] * inv;
