@@ -21,18 +21,28 @@
  *
  * Parity pins (tests/test_oracle_filament.py): quaternion identities
  * (|q| = 1, expm of an axis-angle equals the rotation-matrix action,
- * midpoint^2 = relative rotation, tangent = R(q) e_x); a straight untwisted
- * filament with no preferred curvature has zero elastic torque; constraint
- * forces sum to zero; dexpinv inverts the exp-map derivative (finite
- * differences); the residual vanishes on exact friction-mobility
- * translation; J * J^-1 = I; the Broyden secant condition; the analytic
- * Jacobian blocks against finite differences of the residual (the parts
- * where the approximation is exact); whole steps converge to TOL; spring
- * forces against Hooke's law on hand-placed beads, across the periodic
- * boundary, equal and opposite.
- * Parity unpinned: the exact numbers of the approximate BlockE terms (no
- * reference output exists to compare against; pinned only by re-derivation
- * from the source text and by Broyden convergence).
+ * midpoint^2 = relative rotation, tangent = R(q) e_x); the elastic torque
+ * of a link turned by theta about a body axis equals the closed form
+ * R(q_m) diag(C, kap, kap) ((4/dL) sin(theta/4) e - p) (twist and both
+ * bends, either sign, straight and turned past the preferred strain p) and
+ * vanishes when every link carries its preferred strain; constraint forces
+ * sum to zero; dexpinv inverts the exp-map derivative (finite differences);
+ * the residual vanishes on exact friction-mobility translation; J * J^-1 =
+ * I; the Broyden secant condition; the analytic Jacobian blocks against
+ * finite differences of the residual: first-bead rows, BlockC, BlockD and
+ * BlockF where the approximation is exact; BlockE exactly without elastic
+ * moduli (constraint torques, bent and twisted filaments, nonzero lambda
+ * and TAU), and on a straight untwisted filament with no preferred strain
+ * its twist blocks are exact but for the sign of the left off-diagonal
+ * block and its bend blocks exactly half the derivative (properties of the
+ * reference's approximation, restated as it is); whole steps converge to
+ * TOL; spring forces against Hooke's law on hand-placed beads, across the
+ * periodic boundary, equal and opposite.
+ * Parity unpinned: the preferred-strain terms of BlockE (beta's
+ * -StrainTwist(2), the StrainTwist(0) rcross(n) + StrainTwist(1) rcross(b)
+ * terms) and the curvature parts of beta and CrossPart on bent frames: they
+ * are approximations with no exact counterpart to compare against, pinned
+ * only by re-derivation from the source text and by Broyden convergence.
  */
 #include "oracle.h"
 

@@ -107,6 +107,185 @@ def test_straight_filament_no_torque_constraint_sum():
     assert np.abs(s2.get_state()["TAU"]).max() < 1e-14
 
 
+def _chain(q0, axes, thetas):
+    """Bead frames q_{i+1} = q_i * (cos th_i/2, sin th_i/2 e_i): each link
+    turns the next bead by th_i about the body-frame axis e_i."""
+    qs = [q0]
+    for e, th in zip(axes, thetas):
+        qs.append(qmul(qs[-1], np.concatenate([[np.cos(th / 2)],
+                                               np.sin(th / 2) * e])))
+    return np.array(qs)
+
+
+def _elastic_only(sp, nws, props, q):
+    st = dict(X=np.zeros((len(q), 3)), Xs=np.zeros((len(q), 3)),
+              Xt=np.zeros((len(q), 3)), q=q, qs=q, qt=q,
+              lam=np.zeros((len(q) - 1, 3)))
+    s = O.Sim(sp, nws, props)
+    s.set_state(**st)
+    s.forces(1)
+    return s.get_state()["TAU"]
+
+
+def test_elastic_torque_of_a_uniformly_strained_link():
+    """applyElasticTorques (Filament.hpp:289-342) on a link whose second
+    bead is the first turned by theta about body axis e_k: the midpoint frame
+    is q_i * (cos th/4, sin th/4 e_k), the discrete strain 2 vec(q_m^*
+    (q_j - q_i)/dL) is exactly (4/dL) sin(th/4) e_k, and the torque is
+    R(q_m) diag(C, kap, kap) strain, added to bead i and taken from bead j.
+    C != kap, so twist (k = x) and bend (k = y, z) tell the moduli apart; a
+    sign error, a swapped modulus or a wrong midpoint frame fails."""
+    nw, C, kap = 6, 2.0, 3.0
+    fp = params.fcm_params(48)
+    sp = params.sim_params(fp, barrier_on=0)
+    props = params.fil_props(1, kap=kap, c=C)
+    rng = np.random.default_rng(1)
+    q0 = unit_q(rng)
+    D = np.diag([C, kap, kap])
+    for k in range(3):
+        e = np.eye(3)[k]
+        for th in (0.3, -0.3, 1.7):
+            q = _chain(q0, [e] * (nw - 1), [th] * (nw - 1))
+            tau = _elastic_only(sp, [nw], props, q)
+            want = np.zeros((nw, 3))
+            for i in range(nw - 1):
+                qm = qmul(q[i], np.concatenate([[np.cos(th / 4)],
+                                                np.sin(th / 4) * e]))
+                t = rotmat(qm) @ D @ (4 / sp.dL * np.sin(th / 4) * e)
+                want[i] += t
+                want[i + 1] -= t
+            assert np.abs(tau - want).max() < 1e-14 * max(1, np.abs(want).max())
+            # interior beads of a uniform chain feel no net torque; the ends
+            # feel +-R(q_m) D strain
+            assert np.abs(tau[1:-1]).max() < 1e-13
+            assert np.abs(want[0]).max() > 0.1
+
+
+def test_elastic_torque_vanishes_at_the_preferred_strain():
+    """The swimming drive (Filament.hpp:301-326): preferred strain-twist
+    (STRAINTWISTZ, beta facPref cos(k s - w t + phi), alpha facPref sin(...))
+    with facPref = T0 (L - s)/L. Turning each link by the rotation whose
+    discrete strain equals its own preferred vector p_i (axis p_i/|p_i|,
+    angle 4 asin(dL |p_i|/4)) gives zero torque on every bead. Straight and
+    turned the other way, the torque is R(q_m) D ((4/dL) sin(th/4) e - p)
+    with q_m the half-turned frame."""
+    nw = 8
+    fp = params.fcm_params(48)
+    sp = params.sim_params(fp, barrier_on=0, strain_twist0=0.05)
+    L = nw * sp.dL
+    props = params.fil_props(1, kap=3.0, c=2.0, T0=1.5 / L, omega=2 * PI,
+                             wavenumber=2 * PI / L, phase=0.7, alpha=0.4,
+                             beta=1.0)
+    nt = 3
+    tt = nt * sp.dt
+    Lf = (nw - 1) * sp.dL
+    pref = []
+    for i in range(nw - 1):
+        s = sp.dL * i + 0.5 * sp.dL
+        fac = props[0].T0 * (Lf - s) / Lf
+        ph = props[0].wavenumber * s - props[0].omega * tt + props[0].phase
+        pref.append([sp.strain_twist0, props[0].beta * fac * np.cos(ph),
+                     props[0].alpha * fac * np.sin(ph)])
+    pref = np.array(pref)
+    mag = np.linalg.norm(pref, axis=1)
+    axes = pref / mag[:, None]
+    th = 4 * np.arcsin(sp.dL * mag / 4)
+    q0 = unit_q(np.random.default_rng(5))
+
+    def tau_of(q):
+        st = dict(X=np.zeros((nw, 3)), Xs=np.zeros((nw, 3)),
+                  Xt=np.zeros((nw, 3)), q=q, qs=q, qt=q,
+                  lam=np.zeros((nw - 1, 3)))
+        s = O.Sim(sp, [nw], props)
+        s.set_state(**st)
+        s.forces(nt)
+        return s.get_state()["TAU"]
+    D = np.diag([2.0, 3.0, 3.0])
+
+    def closed_form(q, sgn):
+        want = np.zeros((nw, 3))
+        for i in range(nw - 1):
+            h = sgn * th[i] / 4
+            qm = qmul(q[i], np.concatenate([[np.cos(h)], np.sin(h) * axes[i]]))
+            strain = 4 / sp.dL * np.sin(h) * axes[i]
+            t = rotmat(qm) @ D @ (strain - pref[i])
+            want[i] += t
+            want[i + 1] -= t
+        return want
+    t_pref = tau_of(_chain(q0, axes, th))
+    q_straight = _chain(q0, axes, 0 * th)
+    q_back = _chain(q0, axes, -th)
+    t_straight, t_back = tau_of(q_straight), tau_of(q_back)
+    scale = np.abs(t_straight).max()
+    assert scale > 0.01
+    assert np.abs(t_pref).max() < 1e-13 * scale
+    assert np.abs(t_straight - closed_form(q_straight, 0)).max() < 1e-13 * scale
+    assert np.abs(t_back - closed_form(q_back, -1)).max() < 1e-13 * scale
+
+
+def _blockE_vs_fd(kap, c, straight, nw=6):
+    """Rows [3Nw, 6Nw) x the U columns of the oracle's J and of a central
+    finite difference of the residual, at Us = 0 under the friction
+    mobility, both minus the identity."""
+    sp, nws, props, st0 = config(grid=64, nfil=1, nw=nw, seed=6, ifs=1000,
+                                 barrier=0, weight=(0, 0, 0), kap=kap)
+    props[0].c = c
+    st = rich_state(st0, nws, seed=3)
+    if straight:
+        st.update(q=st0["q"], qt=st0["q"], lam=np.zeros_like(st["lam"]))
+    st["Us"] = np.zeros_like(st["Us"])
+    st["qs"] = st["q"]
+    nt = 5
+    s = O.Sim(sp, nws, props)
+    s.set_state(**st)
+    s.forces(nt)
+    s.build_jacobian(nt)
+    J = s.jacobian(0)
+    FD = _fd_jacobian(sp, nws, props, st, nt)
+    U = slice(3, 3 * nw + 3)
+    eye = np.eye(3 * nw)
+    return J[3 * nw:, U] - eye, FD[3 * nw:, U] - eye
+
+
+def test_blockE_is_exact_without_elastic_moduli():
+    """With kap = C = 0 the Lie rows of the residual depend on U only through
+    the constraint torques -dL/2 t_i x (lam_{i-1} + lam_i) and dexpinv; at
+    Us = 0 their exact derivative is I - 2dt/3 (dOmega + [Omega x]/2) with
+    dOmega = -dL/2 Tfac [lam x][t x], which is what BlockE
+    (Filament.hpp:949-1095) evaluates. Bent, twisted filament, nonzero lam
+    and TAU: BlockE equals the finite-difference Jacobian."""
+    E, Efd = _blockE_vs_fd(0.0, 0.0, straight=False)
+    assert np.abs(Efd).max() > 1e-5
+    assert np.abs(E - Efd).max() < 1e-7 * np.abs(Efd).max()
+
+
+def test_blockE_elastic_terms_on_a_straight_filament():
+    """On a straight, untwisted filament with no preferred strain and lam = 0
+    the exact derivative of the Lie residual is the linearised beam
+    stiffness. The reference's approximate blocks relate to it in a fixed
+    pattern, which pins the restatement's signs and moduli:
+    twist (kap = 0): the diagonal and right off-diagonal blocks are exact and
+    the left off-diagonal blocks (bead k w.r.t. k-1, Filament.hpp:1026-1034,
+    1075-1081) carry the opposite sign;
+    bend (C = 0): every block is exactly half the derivative
+    (0.5*myKAP*rcross(t_+-1)*rcross(t)/DL where both links would give
+    myKAP)."""
+    nw = 6
+    E, Efd = _blockE_vs_fd(0.0, 2.0, straight=True, nw=nw)
+    scale = np.abs(Efd).max()
+    assert scale > 1e-5
+    for i in range(nw):
+        for j in range(max(0, i - 1), min(nw, i + 2)):
+            a = E[3 * i:3 * i + 3, 3 * j:3 * j + 3]
+            b = Efd[3 * i:3 * i + 3, 3 * j:3 * j + 3]
+            sign = -1.0 if j == i - 1 else 1.0
+            assert np.abs(a - sign * b).max() < 1e-7 * scale, (i, j)
+    E, Efd = _blockE_vs_fd(3.0, 0.0, straight=True, nw=nw)
+    scale = np.abs(Efd).max()
+    assert scale > 1e-5
+    assert np.abs(E - 0.5 * Efd).max() < 1e-7 * scale
+
+
 def test_residual_zero_on_exact_translation():
     """BDF2 integrates uniform sedimentation under the friction mobility
     exactly (MobilityConstraints.cpp:616-760), so the extrapolated guess
