@@ -117,6 +117,7 @@ PROTOS = {
     "fs_system_barrier": [P, I, I, _DP],
     "fs_system_set_barrier": [P, I, I],
     "fs_system_timestep_trial": [P, I, IP],
+    "fs_system_step_status": [P, IP, IP, IP, _DP],
     "fs_make_filaments": [_DP, I, IP, D, D, C.c_ulonglong, I, _DP, _DP],
     "fs_make_filaments_gpu": [_DP, I, IP, D, D, C.c_ulonglong, I, _DP, _DP],
     "fs_mlp_create": [I, I, I, C.c_ulonglong, C.POINTER(P)],

@@ -187,11 +187,17 @@ class System:
         self.nfil, self.np, self.nlam = a.value, b.value, c.value
 
     def close(self):
+        """Frees the system; raises (and keeps it) while a live Batch holds
+        it, since the batch's captured launches use its buffers."""
         if self.h:
-            self.lib.fs_system_destroy(self.h)
+            abi.check(self.lib.fs_system_destroy(self.h))
             self.h = C.c_void_p()
 
-    __del__ = close
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # still held by a batch at interpreter exit
+            pass
 
     def _rows(self, k):
         return self.nlam if k.startswith("lam") else self.np
@@ -232,6 +238,16 @@ class System:
         it = C.c_int()
         abi.check(self.lib.fs_system_timestep_trial(self.h, nt, C.byref(it)))
         return it.value
+
+    def step_status(self):
+        """Outcome of the last timestep (fs_system_step_status): dict of
+        converged, exploded, nonfinite (bools) and res_inf."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        r = C.c_double()
+        abi.check(self.lib.fs_system_step_status(
+            self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(r)))
+        return dict(converged=bool(a.value), exploded=bool(b.value),
+                    nonfinite=bool(c.value), res_inf=r.value)
 
     def set_barrier(self, on, mask=abi.FS_BARRIER_ALL):
         abi.check(self.lib.fs_system_set_barrier(self.h, 1 if on else 0, mask))

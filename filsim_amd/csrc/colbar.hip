@@ -36,6 +36,9 @@ __global__ void k_cb_fill(int *p, int n, int v) {
 
 // Cell of one unwrapped position (CollisionBarrierFilament.cpp:393-395,
 // 407-409); contraction off so every product rounds as in the reference.
+// A coordinate that is not finite or lies 10^9 boxes out (a diverged
+// iterate; the reference prints "segfault imminent", :398-404) is binned
+// into cell 0 of that axis, so no conversion leaves the int range.
 __device__ int cb_cell(const double *x, CbGeom g) {
 #pragma clang fp contract(off)
   const int large_number = 100000;
@@ -44,6 +47,10 @@ __device__ int cb_cell(const double *x, CbGeom g) {
   const int M[3] = {g.MX, g.MY, g.MZ};
   for (int d = 0; d < 3; ++d) {
     double xr = x[d];
+    if (!(fabs(xr / L[d]) < 1e9)) {
+      c[d] = 0;
+      continue;
+    }
     xr = xr - L[d] * ((double)((int)(xr / L[d] + large_number) - large_number));
     int ci = (int)((xr / L[d]) * (double)M[d]);
     if (ci >= M[d]) ci = M[d] - 1;

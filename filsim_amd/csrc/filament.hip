@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -393,6 +394,7 @@ __global__ void __launch_bounds__(256)
   __shared__ double s_m[256];
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   double mx = 0.0;
+  bool nonfinite = false;
   if (g < S.np) {
     const int f = S.sfil[g], i = S.sseg[g];
     const int nw = S.nw[f];
@@ -415,11 +417,14 @@ __global__ void __launch_bounds__(256)
       }
       err[k0 + 3 * i + d] = e1;
       err[k0 + 3 * nw + 3 * i + d] = e2;
+      // the reference's test |e| > TOL never counts a NaN, and fmax drops
+      // it the same way; non-finite entries are reported on their own
       mx = fmax(mx, fmax(fabs(e1), fabs(e2)));
+      nonfinite = nonfinite || !(fabs(e1) <= DBL_MAX) || !(fabs(e2) <= DBL_MAX);
     }
   }
   s_m[threadIdx.x] = mx;
-  __syncthreads();
+  if (__syncthreads_or(nonfinite) && threadIdx.x == 0) atomicOr(maxbits + 3, 1ull);
   for (int off = 128; off > 0; off >>= 1) {
     if (threadIdx.x < off)
       s_m[threadIdx.x] = fmax(s_m[threadIdx.x], s_m[threadIdx.x + off]);
@@ -429,10 +434,12 @@ __global__ void __launch_bounds__(256)
     atomicMax(maxbits, (unsigned long long)__double_as_longlong(s_m[0]));
 }
 
-__global__ void k_set_u64(const int *__restrict__ act, unsigned long long *p,
-                          unsigned long long v) {
+// clears the residual's maximum (word 0) and non-finite flag (word 3)
+__global__ void k_resid_reset(const int *__restrict__ act,
+                              unsigned long long *mb) {
   if (act && !*act) return;
-  *p = v;
+  mb[0] = 0ull;
+  mb[3] = 0ull;
 }
 
 __global__ void k_fill_i32(int *p, int n, int v) {
@@ -786,10 +793,14 @@ __global__ void __launch_bounds__(1024)
     for (int i = tid; i < n; i += blockDim.x) colbuf[i] = M[(size_t)i * n + k];
     __syncthreads();
     if (tid < 64) {
-      double best = -1.0;
+      // used rows rank -1 and unused NaN entries -0.5, so the pivot is
+      // always an unused row (n - k of them remain) and stays in range; a
+      // pivot that is not > 0 or not finite raises the singular flag
+      double best = -2.0;
       int bi = n;
       for (int i = tid; i < n; i += 64) {
-        double v = used[i] ? -1.0 : fabs(colbuf[i]);
+        const double a = fabs(colbuf[i]);
+        const double v = used[i] ? -1.0 : (a == a ? a : -0.5);
         if (v > best) {
           best = v;
           bi = i;
@@ -804,7 +815,9 @@ __global__ void __launch_bounds__(1024)
         }
       }
       if (tid == 0) {
-        if (!(best > 0.0)) atomicExch(singular, 1);
+        if (!(best > 0.0) || !(best <= DBL_MAX) || bi >= n)
+          atomicExch(singular, 1);
+        bi = bi < n ? bi : n - 1;
         s_p = bi;
         used[bi] = 1;
         pivrow[k] = bi;
@@ -948,8 +961,10 @@ __global__ void __launch_bounds__(32 * TY)
       unsigned long long key = lane < TY ? ck[lane] : 0ull;
       key = wave_max_u64(key);
       int p = 255 - (int)(key & 0xFF);
-      const bool bad = (key >> 8) == 0 || p >= n;  // no nonzero unused entry
-      p = bad ? n - 1 - k : p;  // keeps indices in range; result is flagged
+      // no nonzero unused entry, or a pivot that is NaN or infinite (the
+      // key's top bits are the exponent of |pivot|)
+      const bool bad = (key >> 8) == 0 || p >= n || (key >> 52) >= 0x7FFull;
+      p = p >= n ? n - 1 - k : p;  // keeps indices in range; result is flagged
       const int pb = p / TY, pt = p % TY;
       if (ty == pt) {
 #pragma unroll
@@ -1621,7 +1636,8 @@ int SystemImpl::build_jacobian(int nt) {
                      dim3(256), 0, st, Dm.as<double>(), nh, 0.0);
   FS_LAUNCHED();
   int *sing = flags.as<int>() + 4;
-  FS_HIP(hipMemcpyAsync(hflags + 4, sing, sizeof(int), hipMemcpyDeviceToHost, st));
+  FS_HIP(hipMemcpyAsync((int *)hflags + 4, sing, sizeof(int),
+                        hipMemcpyDeviceToHost, st));
   FS_HIP(hipStreamSynchronize(st));
   FS_CHECK(((int *)hflags)[4] == 0, FS_ERR_ARG,
            "approximate Jacobian is singular");
@@ -1690,17 +1706,20 @@ int SystemImpl::residual(int nt, const double *V, const double *W, double *err,
                          int *check, double *maxabs) {
   ProfScope ps_(*this, P_ERRCHK);
   unsigned long long *mb = flags.as<unsigned long long>();
-  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, d.act, mb, 0ull);
+  hipLaunchKernelGGL(k_resid_reset, dim3(1), dim3(1), 0, st, d.act, mb);
   hipLaunchKernelGGL(k_residual, dim3(ceil_div(np, 256)), dim3(256), 0, st, d,
                      nt, V, W, err, mb);
   FS_LAUNCHED();
-  FS_HIP(hipMemcpyAsync(hflags, mb, sizeof(unsigned long long),
+  FS_HIP(hipMemcpyAsync(hflags, mb, 4 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, st));
   FS_HIP(hipStreamSynchronize(st));
   double m;
   memcpy(&m, hflags, sizeof m);
   if (maxabs) *maxabs = m;
   if (check) *check = (m > p.tol) ? 1 : 0;
+  // MobilityConstraints.cpp:764-767 ("Prepare for explosion")
+  if (m > 1e6) step_exploded = 1;
+  if (hflags[3]) step_nonfinite = 1;
   return FS_OK;
 }
 
@@ -1762,6 +1781,9 @@ int SystemImpl::set_barrier(int on, int mask) {
 
 // Initial guess, forces, Jacobian, link + barrier, mobility, residual E.
 int SystemImpl::begin_step(int nt) {
+  step_exploded = 0;
+  step_nonfinite = 0;
+  step_check = 1;
   FS_TRY(initial_guess());
   FS_TRY(forces(nt));
   FS_TRY(build_jacobian(nt));
@@ -1807,6 +1829,7 @@ int SystemImpl::broyden_loop(int nt, int *iters) {
     }
     std::swap(E, NewE);
   }
+  step_check = check;
   if (iters) *iters = iter;
   return FS_OK;
 }
@@ -1837,8 +1860,8 @@ int SystemImpl::begin_step_dev(int *status) {
   FS_TRY(barrier(nt, 1, nullptr));
   FS_TRY(mobility(nt));
   unsigned long long *mb = flags.as<unsigned long long>();
-  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, (const int *)nullptr,
-                     mb, 0ull);
+  hipLaunchKernelGGL(k_resid_reset, dim3(1), dim3(1), 0, st,
+                     (const int *)nullptr, mb);
   hipLaunchKernelGGL(k_residual, dim3(ceil_div(np, 256)), dim3(256), 0, st, d,
                      nt, d.V, d.W, E, mb);
   hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(1), 0, st, ctl.as<int>(), mb,
@@ -1866,7 +1889,7 @@ int SystemImpl::iteration_dev(int *status) {
   FS_TRY(barrier(nt, 0, nullptr));
   FS_TRY(mobility(nt));
   unsigned long long *mb = flags.as<unsigned long long>();
-  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, act, mb, 0ull);
+  hipLaunchKernelGGL(k_resid_reset, dim3(1), dim3(1), 0, st, act, mb);
   hipLaunchKernelGGL(k_residual, dim3(ceil_div(np, 256)), dim3(256), 0, st, d,
                      nt, d.V, d.W, NewE, mb);
   hipLaunchKernelGGL(k_ctl_after, dim3(1), dim3(1), 0, st, ctl.as<int>(), mb,

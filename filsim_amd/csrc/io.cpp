@@ -177,7 +177,8 @@ extern "C" int fs_io_write_par(const char *path, const char *name,
                "%d %g %g %g %g %g %g %g %g %g %d %d %d %d %g %g %g %g %g \n\n",
                name ? name : "filsim", tag ? tag : "run", nf, nworm, a, a,
                p->dL, 0.0, kap, c, p->fcm.mu, 0.0, 0.0, time_steps, 0.0,
-               p->tol, 0.0, p->broyden_maxiter, 0,
+               p->tol, 0 /* gmres_tol: the int 0, file_functions.cpp:42 */,
+               p->broyden_maxiter, 0,
                p->fcm.nx * p->fcm.dx, plot_steps, 0.0, 0.0, p->strain_twist0,
                0.0,
                p->fcm.nz * p->fcm.dx, p->fcm.ny * p->fcm.dx, 0.0, 0.0, p->dt,

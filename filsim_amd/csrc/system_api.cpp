@@ -46,6 +46,8 @@ extern "C" int fs_system_create(const fs_sim_params *p, int nfil, const int *nw,
 
 extern "C" int fs_system_destroy(fs_system *h) {
   if (!h) return FS_OK;
+  FS_CHECK(h->batches == 0, FS_ERR_STATE,
+           "the system belongs to a batch (its launches are captured)");
   h->ws.release();
   h->impl.release();
   delete h;
@@ -156,11 +158,15 @@ extern "C" int fs_system_set_spring_links(fs_system *h, int n,
                                           const int *links, double k,
                                           double natural_length) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(h->batches == 0, FS_ERR_STATE,
+           "the system belongs to a batch (its launches are captured)");
   return h->impl.set_spring_links(n, links, k, natural_length);
 }
 
 extern "C" int fs_system_set_profiling(fs_system *h, int on) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(h->batches == 0, FS_ERR_STATE,
+           "the system belongs to a batch (its launches are captured)");
   FS_HIP(hipStreamSynchronize(h->impl.st));
   h->impl.prof = on ? 1 : 0;
   h->impl.fcm.prof = on ? 1 : 0;
@@ -196,11 +202,15 @@ extern "C" int fs_system_timestep_trial(fs_system *h, int nt, int *iters) {
 
 extern "C" int fs_system_set_warmstart(fs_system *h, fs_mlp *m) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(h->batches == 0, FS_ERR_STATE,
+           "the system belongs to a batch (its launches are captured)");
   return h->ws.attach(h->impl, fs::mlp_impl(m));
 }
 
 extern "C" int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(h->batches == 0, FS_ERR_STATE,
+           "the system belongs to a batch (its launches are captured)");
   return h->impl.fcm.autotune(h->impl.np, reps, best);
 }
 
@@ -273,6 +283,8 @@ extern "C" int fs_system_read_checkpoint(fs_system *h, const char *path,
 
 extern "C" int fs_system_set_tuning(fs_system *h, const fs_fcm_tuning *t) {
   FS_CHECK(h && t, FS_ERR_ARG, "null argument");
+  FS_CHECK(h->batches == 0, FS_ERR_STATE,
+           "the system belongs to a batch (its launches are captured)");
   return h->impl.fcm.set_tuning(*t);
 }
 
@@ -289,6 +301,8 @@ extern "C" int fs_system_warmstart_dims(fs_system *h, int *n_in, int *n_out) {
 extern "C" int fs_system_ws_record(fs_system *h, float *X, float *T,
                                    int capacity) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(h->batches == 0, FS_ERR_STATE,
+           "the system belongs to a batch (its launches are captured)");
   return h->ws.record(h->impl, X, T, capacity);
 }
 
@@ -304,6 +318,17 @@ extern "C" int fs_system_last_step_info(fs_system *h, int *used_mlp,
   if (used_mlp) *used_mlp = h->ws.used_mlp;
   if (res_plain) *res_plain = h->ws.tracking ? h->ws.res_plain : h->impl.last_res0;
   if (res_mlp) *res_mlp = h->ws.res_mlp;
+  return FS_OK;
+}
+
+extern "C" int fs_system_step_status(fs_system *h, int *converged,
+                                     int *exploded, int *nonfinite,
+                                     double *res_inf) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  if (converged) *converged = h->impl.step_check ? 0 : 1;
+  if (exploded) *exploded = h->impl.step_exploded;
+  if (nonfinite) *nonfinite = h->impl.step_nonfinite;
+  if (res_inf) *res_inf = h->impl.last_res;
   return FS_OK;
 }
 

@@ -69,6 +69,9 @@ struct SystemImpl {
   int begin_step(int nt);
   int broyden_loop(int nt, int *iters);
   int check0 = 0;
+  // outcome of the last step: the final residual still above TOL, a
+  // residual above 1e6, a residual holding NaN or infinity
+  int step_check = 0, step_exploded = 0, step_nonfinite = 0;
 
   // spring links between beads (spring_link_functions.cpp:24-45)
   dbuf springb;

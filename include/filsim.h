@@ -263,6 +263,22 @@ int fs_system_set_barrier(fs_system *h, int on, int mask);
  * reference. Plain path only (no warm start attached). */
 int fs_system_timestep_trial(fs_system *h, int nt, int *iters);
 
+/* Outcome of the last fs_system_timestep / fs_system_timestep_trial, which
+ * the reference reports on stdout only. converged: 1 when the final
+ * residual met TOL, 0 when the loop stopped at broyden_maxiter (the step
+ * is committed anyway, as main.cpp:503, 676-679 does). exploded: 1 when a
+ * residual of the step exceeded 1e6 in the infinity norm ("Prepare for
+ * explosion", MobilityConstraints.cpp:764-767). nonfinite: 1 when a
+ * residual held a NaN or an infinity; the reference's test |e| > TOL never
+ * counts a NaN, and neither does TOL here, so a step whose residual is NaN
+ * throughout reports converged = 1 with nonfinite = 1. res_inf: infinity
+ * norm of the final residual over its finite entries. A state that is not
+ * finite usually stops the step earlier: its approximate Jacobian is
+ * singular and fs_system_timestep returns FS_ERR_ARG. Pointers may be
+ * NULL. */
+int fs_system_step_status(fs_system *h, int *converged, int *exploded,
+                          int *nonfinite, double *res_inf);
+
 /* Deterministic initial configuration (the reference's overlap-rejected
  * placement, Filament.cpp:8-153, filament_initialisation_functions.cpp:
  * 631-650): nfil straight filaments of nw[f] segments, heads uniform in the

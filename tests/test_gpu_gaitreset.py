@@ -108,8 +108,29 @@ def test_barrier_switch_rules(cuda):
     with pytest.raises(Exception):
         s.set_barrier(1, 7)        # unknown mask
     batch = api.Batch([s])
+    # every call that would change what the captured launches read, or free
+    # it, is refused while the system belongs to a batch
     with pytest.raises(Exception):
-        s.set_barrier(0)           # captured in a batch
+        s.set_barrier(0)
+    with pytest.raises(Exception):
+        s.set_spring_links([[0, 0, 1, 0]], 1.0, 2.2)
+    with pytest.raises(Exception):
+        s.set_tuning(abi.FcmTuning(-1, -1, -1, -1, -1))
+    with pytest.raises(Exception):
+        s.autotune(1)
+    with pytest.raises(Exception):
+        s.set_profiling(True)
+    n_in, n_out = api.system_warmstart_dims(s)
+    m = api.Mlp(n_in, n_out, width=64)
+    with pytest.raises(Exception):
+        api.system_set_warmstart(s, m)
+    with pytest.raises(Exception):
+        s.close()
+    assert s.h
+    its = batch.timestep(1)        # the batch still steps the system
+    assert its[0] >= 0
     batch.close()
     s.set_barrier(0)
+    s.set_spring_links([[0, 0, 1, 0]], 1.0, 2.2)
+    m.close()
     s.close()
