@@ -12,6 +12,12 @@ LIB_PATH = os.path.join(_HERE, "libfilsim_amd.so")
 
 FS_FP64 = 0
 FS_FP32 = 1
+# status codes (include/filsim.h)
+FS_OK = 0
+FS_ERR_ARG = -1
+FS_ERR_HIP = -2
+FS_ERR_SIZE = -3
+FS_ERR_STATE = -4
 # barrier pair masks (fs_colbar_set_mask, fs_system_set_barrier)
 FS_BARRIER_ALL = 0
 FS_BARRIER_NOT_SWIMMER = 1
@@ -117,6 +123,8 @@ PROTOS = {
     "fs_system_barrier": [P, I, I, _DP],
     "fs_system_set_barrier": [P, I, I],
     "fs_system_timestep_trial": [P, I, IP],
+    "fs_colbar_barrier_noalloc": [C.POINTER(ColbarParams), I, P, P, P, P, I,
+                                  I, P, P, P],
     "fs_system_step_status": [P, IP, IP, IP, _DP],
     "fs_make_filaments": [_DP, I, IP, D, D, C.c_ulonglong, I, _DP, _DP],
     "fs_make_filaments_gpu": [_DP, I, IP, D, D, C.c_ulonglong, I, _DP, _DP],

@@ -160,6 +160,20 @@ class Colbar:
         return out[:2 * n.value].reshape(n.value, 2)
 
 
+def colbar_barrier_noalloc(cp, seg_fil, seg_idx, Xs, Y, nt, F,
+                           min_dist2=None, mask=abi.FS_BARRIER_ALL,
+                           stream=None):
+    """Link + apply without device allocation (fs_colbar_barrier_noalloc):
+    every array argument is a device tensor; F += barrier forces, and
+    min_dist2 (a one-element float64 device tensor, optional) receives the
+    minimum squared distance."""
+    lib = abi.load()
+    abi.check(lib.fs_colbar_barrier_noalloc(
+        C.byref(cp), int(Xs.shape[0]), _ptr(seg_fil), _ptr(seg_idx), _ptr(Xs),
+        _ptr(Y), nt, mask, _ptr(F),
+        _ptr(min_dist2) if min_dist2 is not None else None, _stream(stream)))
+
+
 STATE_FIELDS = ("X", "Xs", "Xt", "U", "Us", "Ut", "q", "qs", "qt", "lam",
                 "lam1", "lam2", "F", "TAU")
 

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "colbar_impl.hpp"
@@ -239,8 +240,9 @@ __global__ void k_cb_init_min(const int *__restrict__ act,
   *m = v;
 }
 
-int ColbarImpl::init(const fs_colbar_params &pp, int n, const int *seg_fil,
-                     const int *seg_idx) {
+// Cell grid of LinkedList::initialise (CollisionBarrierFilament.cpp:
+// 274-289): M = int(L / 2.5a) cells per axis.
+int ColbarImpl::geometry(const fs_colbar_params &pp, int n) {
   p = pp;
   np = n;
   FS_CHECK(n >= 0, FS_ERR_ARG, "np < 0");
@@ -250,13 +252,25 @@ int ColbarImpl::init(const fs_colbar_params &pp, int n, const int *seg_fil,
   g.Lx = p.box[0];
   g.Ly = p.box[1];
   g.Lz = p.box[2];
+  FS_CHECK(p.box[0] / Rref < 2048 && p.box[1] / Rref < 2048 &&
+               p.box[2] / Rref < 2048,
+           FS_ERR_SIZE, "box too large for the cell list (2047 cells per axis)");
   g.MX = (int)(p.box[0] / Rref);
   g.MY = (int)(p.box[1] / Rref);
   g.MZ = (int)(p.box[2] / Rref);
   FS_CHECK(g.MX >= 3 && g.MY >= 3 && g.MZ >= 3, FS_ERR_ARG,
            "box too small for the cell list: %d x %d x %d cells (need >= 3)",
            g.MX, g.MY, g.MZ);
-  ncell = g.MX * g.MY * g.MZ;
+  const long nc = (long)g.MX * g.MY * g.MZ;
+  FS_CHECK(nc < (long)kScanBlock * kScanMaxBlocks, FS_ERR_SIZE,
+           "%ld barrier cells exceed the cell-count scan", nc);
+  ncell = (int)nc;
+  return FS_OK;
+}
+
+int ColbarImpl::init(const fs_colbar_params &pp, int n, const int *seg_fil,
+                     const int *seg_idx) {
+  FS_TRY(geometry(pp, n));
   FS_TRY(cell.alloc(sizeof(int) * std::max(n, 1)));
   FS_TRY(slot.alloc(sizeof(int) * std::max(n, 1)));
   FS_TRY(ids.alloc(sizeof(int) * std::max(n, 1)));
@@ -400,9 +414,145 @@ int ColbarImpl::pairs(const double *Y, int nt, int max_pairs, int *out,
   return FS_OK;
 }
 
+// ---------------------------------------------------------------------
+// No-allocation barrier: link + apply with the cell list in a file-scope
+// device array instead of buffers of a handle. No counterpart in the
+// reference; for callers that cannot allocate device memory.
+// ---------------------------------------------------------------------
+constexpr size_t kCbScratchBytes = size_t(32) << 20;  // per GPU, process life
+static __device__ __align__(256) unsigned char scratch[kCbScratchBytes];
+
+__global__ void k_cb_store_min(const unsigned long long *__restrict__ min_bits,
+                               double *__restrict__ out) {
+  *out = __longlong_as_double((long long)*min_bits);
+}
+
+namespace {
+
+constexpr int kCbMaxDevices = 64;
+std::mutex g_cb_scratch_mu;
+unsigned char *g_cb_scratch[kCbMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current. The array's host-side name is a
+// stand-in, not a device address: kernels get only this pointer.
+int cb_scratch_ptr(unsigned char **out) {
+  int dev = -1;
+  FS_HIP(hipGetDevice(&dev));
+  FS_CHECK(dev >= 0 && dev < kCbMaxDevices, FS_ERR_ARG,
+           "device %d beyond the %d the barrier scratch table holds", dev,
+           kCbMaxDevices);
+  std::lock_guard<std::mutex> lock(g_cb_scratch_mu);
+  if (!g_cb_scratch[dev]) {
+    void *ptr = nullptr;
+    FS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    FS_CHECK(ptr != nullptr, FS_ERR_HIP,
+             "no device address for the barrier scratch on device %d", dev);
+    g_cb_scratch[dev] = static_cast<unsigned char *>(ptr);
+  }
+  *out = g_cb_scratch[dev];
+  return FS_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Byte offsets of the scratch areas for ncell cells and np segments; the
+// total is returned.
+struct CbScratchLayout {
+  size_t minb, cnt, start, scan, cell, slot, ids, total;
+  CbScratchLayout(long ncell, long np) {
+    size_t o = 0;
+    minb = o;
+    o += align256(sizeof(unsigned long long));
+    cnt = o;
+    o += align256(sizeof(int) * (size_t)ncell);
+    start = o;
+    o += align256(sizeof(int) * (size_t)(ncell + 1));
+    scan = o;
+    o += align256(sizeof(int) * (size_t)(ncell / kScanBlock + 4));
+    cell = o;
+    o += align256(sizeof(int) * (size_t)std::max(np, 1L));
+    slot = o;
+    o += align256(sizeof(int) * (size_t)std::max(np, 1L));
+    ids = o;
+    o += align256(sizeof(int) * (size_t)std::max(np, 1L));
+    total = o;
+  }
+};
+
+}  // namespace
+
+int colbar_barrier_noalloc(const fs_colbar_params *pp, int np,
+                           const int *seg_fil, const int *seg_idx,
+                           const double *Xs, const double *Y, int nt, int mask,
+                           double *F, double *min_dist2, hipStream_t st) {
+  FS_CHECK(pp, FS_ERR_ARG, "null parameters");
+  FS_CHECK(mask == FS_BARRIER_ALL || mask == FS_BARRIER_NOT_SWIMMER ||
+               mask == FS_BARRIER_NOT_TEST_PARTICLES,
+           FS_ERR_ARG, "unknown barrier mask %d", mask);
+  ColbarImpl cb;  // geometry and constants only: its buffers stay empty
+  FS_TRY(cb.geometry(*pp, np));
+  cb.mask = mask;
+  FS_CHECK(np == 0 || (seg_fil && seg_idx && Xs && Y && F), FS_ERR_ARG,
+           "null array");
+  const CbScratchLayout L(cb.ncell, np);
+  FS_CHECK(L.total <= kCbScratchBytes, FS_ERR_SIZE,
+           "%d cells and %d segments need %zu bytes of barrier scratch; the "
+           "no-allocation entry point has %zu",
+           cb.ncell, np, L.total, kCbScratchBytes);
+  unsigned char *base = nullptr;
+  FS_TRY(cb_scratch_ptr(&base));
+  unsigned long long *minb = reinterpret_cast<unsigned long long *>(base + L.minb);
+  int *cnt = reinterpret_cast<int *>(base + L.cnt);
+  int *start = reinterpret_cast<int *>(base + L.start);
+  int *scan = reinterpret_cast<int *>(base + L.scan);
+  int *cell = reinterpret_cast<int *>(base + L.cell);
+  int *slot = reinterpret_cast<int *>(base + L.slot);
+  int *ids = reinterpret_cast<int *>(base + L.ids);
+  // link: the cell counts start from zero on every call (fill kernel, so a
+  // captured graph replays it)
+  hipLaunchKernelGGL(k_cb_fill, dim3(ceil_div(cb.ncell, 256)), dim3(256), 0,
+                     st, cnt, cb.ncell, 0);
+  if (np > 0)
+    hipLaunchKernelGGL(k_cb_bin, dim3(ceil_div(np, 256)), dim3(256), 0, st, np,
+                       Xs, cb.g, cell, cnt, slot);
+  FS_TRY(device_scan(cnt, cb.ncell, start, scan, st));
+  if (np > 0)
+    hipLaunchKernelGGL(k_cb_scatter, dim3(ceil_div(np, 256)), dim3(256), 0, st,
+                       np, cell, slot, start, ids);
+  hipLaunchKernelGGL(k_cb_sort, dim3(ceil_div(cb.ncell, 256)), dim3(256), 0,
+                     st, cb.ncell, start, ids);
+  // apply
+  double init = 100000.0;
+  unsigned long long ib;
+  memcpy(&ib, &init, sizeof ib);
+  hipLaunchKernelGGL(k_cb_init_min, dim3(1), dim3(1), 0, st,
+                     (const int *)nullptr, minb, ib);
+  if (np > 0)
+    hipLaunchKernelGGL(k_cb_apply, dim3(ceil_div((long)np * kCbLanes, 256)),
+                       dim3(256), 0, st, (const int *)nullptr, np, Y, cb.g,
+                       cb.consts(nt), cell, start, ids, seg_fil, seg_idx, F,
+                       minb, 0, (int *)nullptr, 0, (int *)nullptr);
+  if (min_dist2)
+    hipLaunchKernelGGL(k_cb_store_min, dim3(1), dim3(1), 0, st, minb,
+                       min_dist2);
+  FS_LAUNCHED();
+  return FS_OK;
+}
+
 }  // namespace fs
 
 using fs::ColbarImpl;
+
+extern "C" int fs_colbar_barrier_noalloc(const fs_colbar_params *p, int np,
+                                         const int *seg_fil,
+                                         const int *seg_idx, const double *Xs,
+                                         const double *Y, int nt, int mask,
+                                         double *F, double *min_dist2,
+                                         void *stream) {
+  return fs::colbar_barrier_noalloc(p, np, seg_fil, seg_idx, Xs, Y, nt, mask,
+                                    F, min_dist2, (hipStream_t)stream);
+}
 
 struct fs_colbar {
   ColbarImpl impl;

@@ -28,6 +28,8 @@ struct ColbarImpl {
   int mask = FS_BARRIER_ALL;
   dbuf cell, slot, ids, cnt, start, scan_tmp, sfil, sseg, minb;
 
+  // cell grid and sizes only (no allocation)
+  int geometry(const fs_colbar_params &pp, int n);
   int init(const fs_colbar_params &pp, int n, const int *seg_fil,
            const int *seg_idx);
   void release();

@@ -154,6 +154,28 @@ int fs_colbar_set_mask(fs_colbar *h, int mask);
 int fs_colbar_pairs(fs_colbar *h, const double *Y, int nt, int max_pairs,
                     int *pairs, int *npairs);
 
+/* No-allocation barrier. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). One call does what
+ * fs_colbar_link(Xs) followed by fs_colbar_apply(Y, nt, F) does on a handle
+ * created with the same parameters and pair mask (FS_BARRIER_*), with the
+ * same kernels and results, but keeps the cell list in a 32 MiB device
+ * array of the library instead of buffers it allocates: each GPU the call
+ * runs on holds that array until the process ends. It calls no hipMalloc,
+ * hipFree or allocating library routine and no host synchronisation.
+ * seg_fil, seg_idx (np ints each: filament and segment index of every
+ * segment), Xs, Y, F (np x 3) and min_dist2 (one double, may be NULL;
+ * receives the minimum squared distance over the visited pairs) are device
+ * pointers. Returns FS_ERR_SIZE when the cell list of the box (M = int(L /
+ * 2.5a) cells per axis) and np segments need more than 32 MiB (about 3.8M
+ * cells with 240,000 segments). Every call on one GPU shares its array:
+ * calls that overlap on one GPU (other streams, other host threads) race. */
+int fs_colbar_barrier_noalloc(const fs_colbar_params *p, int np,
+                              const int *seg_fil, const int *seg_idx,
+                              const double *Xs, const double *Y, int nt,
+                              int mask, double *F, double *min_dist2,
+                              void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Filament system + implicit timestep (replaces class Filament,       */
 /* filamentJsolver, ErrorCheckFCM and the Broyden loop of main.cpp)    */
