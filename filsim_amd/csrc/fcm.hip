@@ -358,31 +358,23 @@ struct Mfma<double> {
   }
 };
 
-template <typename T, int NGD>
-__global__ void __launch_bounds__(256)
-    k_spread_mfma(const int *__restrict__ act, SpreadGeom G,
-                  const T *__restrict__ tab, const int *__restrict__ origin,
-                  const T *__restrict__ coef, const int *__restrict__ start,
-                  const int *__restrict__ ids, T *__restrict__ R) {
-  typedef typename Mfma<T>::v4 v4;
-  constexpr int kMfmaChunk = fs::kMfmaChunk<T>;
-  if (act && !*act) return;  // batched sweep: suspension converged
-  __shared__ int s_bstart[kMaxBinsPerTile + 1];
-  __shared__ int s_bfirst[kMaxBinsPerTile];
-  __shared__ int s_ntot;
-  __shared__ T s_xy[kMfmaChunk][6][NGD];  // gx gy gdx gdy grad_x grad_y
-  __shared__ T s_zt[kMfmaChunk][3][16];   // B_0..B_2 at the tile's z
-  __shared__ T s_coef[kMfmaChunk][6];
-  __shared__ int s_org[kMfmaChunk][3];
-  __shared__ int s_cid[kMfmaChunk];
+// Whether a stencil of ngd points starting at o overlaps the te points
+// starting at p0 on a periodic axis of n points (o, p0 in [0, n)).
+__device__ __forceinline__ bool stencil_meets(int o, int p0, int te, int ngd,
+                                             int n) {
+  int d = o - p0;
+  if (d < 0) d += n;
+  return d < te || d > n - ngd;
+}
 
-  const int ntiles = G.nbx * G.nby * G.nbz;
-  int bid = blockIdx.x;  // XCD-aware order, as k_spread
-  if ((ntiles & 7) == 0) bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
-  const int bx = bid % G.nbx;
-  const int by = (bid / G.nbx) % G.nby;
-  const int bz = bid / (G.nbx * G.nby);
-  constexpr int ngd = NGD;
+// Candidate lists of the 8 x 8 x 16 matrix-core tiles (spread and
+// interpolation). tile_bins: the bins whose segments' stencil origins can
+// reach tile (bx, by, bz), their first list entries and running counts;
+// returns the number of candidates (256 threads take part).
+__device__ __forceinline__ int tile_bins(const SpreadGeom &G, int bx, int by,
+                                         int bz, const int *__restrict__ start,
+                                         int *s_bfirst, int *s_bstart,
+                                         int *s_ntot) {
   const int cx = min(G.hx + 1, G.nbx), cy = min(G.hy + 1, G.nby),
             cz = min(G.hz + 1, G.nbz);
   const int nbins = cx * cy * cz;
@@ -399,81 +391,153 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) {
     s_bstart[0] = 0;
     for (int t = 0; t < nbins; ++t) s_bstart[t + 1] += s_bstart[t];
-    s_ntot = s_bstart[nbins];
+    *s_ntot = s_bstart[nbins];
   }
   __syncthreads();
-  const int ntot = s_ntot;
+  return *s_ntot;
+}
+
+// One pass over candidates [base, base + 256) of a tile starting at (px0,
+// py0, pz0): keeps the segments whose stencil meets the tile, in candidate
+// order (s_kid: segment, s_korg: origin packed 10 bits per axis) and
+// returns how many. The bins over-include; the kept list skips only
+// segments that contribute exact zeros to the tile.
+__device__ __forceinline__ int tile_filter_pass(
+    const SpreadGeom &G, int px0, int py0, int pz0, int base, int ntot,
+    const int *__restrict__ ids, const int *__restrict__ origin,
+    const int *s_bfirst, const int *s_bstart, int *s_wsum, int *s_kid,
+    int *s_korg) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gc = base + threadIdx.x;
+  bool keep = false;
+  int sid = 0, org = 0;
+  if (gc < ntot) {
+    int t = 0;
+    while (gc >= s_bstart[t + 1]) ++t;
+    sid = ids[s_bfirst[t] + gc - s_bstart[t]];
+    const int ox = origin[3 * sid + 0], oy = origin[3 * sid + 1],
+              oz = origin[3 * sid + 2];
+    keep = stencil_meets(ox, px0, 8, G.ngd, G.nx) &&
+           stencil_meets(oy, py0, 8, G.ngd, G.ny) &&
+           stencil_meets(oz, pz0, 16, G.ngd, G.nz);
+    org = ox | (oy << 10) | (oz << 20);
+  }
+  const unsigned long long kb = __ballot(keep);
+  const int below = __popcll(kb & ((1ull << lane) - 1ull));
+  if (lane == 0) s_wsum[wave] = __popcll(kb);
+  __syncthreads();
+  int off = 0;
+  for (int w = 0; w < wave; ++w) off += s_wsum[w];
+  const int nk = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+  if (keep) {
+    s_kid[off + below] = sid;
+    s_korg[off + below] = org;
+  }
+  __syncthreads();
+  return nk;
+}
+
+template <typename T, int NGD>
+__global__ void __launch_bounds__(256)
+    k_spread_mfma(const int *__restrict__ act, SpreadGeom G,
+                  const T *__restrict__ tab, const int *__restrict__ origin,
+                  const T *__restrict__ coef, const int *__restrict__ start,
+                  const int *__restrict__ ids, T *__restrict__ R) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int kMfmaChunk = fs::kMfmaChunk<T>;
+  if (act && !*act) return;  // batched sweep: suspension converged
+  __shared__ int s_bstart[kMaxBinsPerTile + 1];
+  __shared__ int s_bfirst[kMaxBinsPerTile];
+  __shared__ int s_ntot;
+  __shared__ int s_wsum[4];
+  // candidates of one pass that overlap the tile, in candidate order: id
+  // and stencil origin packed 10 bits per axis
+  __shared__ int s_kid[256], s_korg[256];
+  __shared__ T s_xy[kMfmaChunk][6][NGD];  // gx gy gdx gdy grad_x grad_y
+  __shared__ T s_zt[kMfmaChunk][3][16];   // B_0..B_2 at the tile's z
+  __shared__ T s_coef[kMfmaChunk][6];
+
+  const int ntiles = G.nbx * G.nby * G.nbz;
+  int bid = blockIdx.x;  // XCD-aware order, as k_spread
+  if ((ntiles & 7) == 0) bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
+  const int bx = bid % G.nbx;
+  const int by = (bid / G.nbx) % G.nby;
+  const int bz = bid / (G.nbx * G.nby);
+  constexpr int ngd = NGD;
+  const int ntot = tile_bins(G, bx, by, bz, start, s_bfirst, s_bstart, &s_ntot);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int q = lane >> 4, i16 = lane & 15;
   const int xy = wave * 16 + i16;  // this lane's A row
-  const int px = bx * 8 + (xy >> 3), py = by * 8 + (xy & 7), pz0 = bz * 16;
+  const int px0 = bx * 8, py0 = by * 8, pz0 = bz * 16;
+  const int px = px0 + (xy >> 3), py = py0 + (xy & 7);
   v4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
 
-  for (int c0 = 0; c0 < ntot; c0 += kMfmaChunk) {
-    const int nc = min(kMfmaChunk, ntot - c0);
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-      const int gc = c0 + c;
-      int t = 0;
-      while (gc >= s_bstart[t + 1]) ++t;
-      const int s = ids[s_bfirst[t] + gc - s_bstart[t]];
-      s_cid[c] = s;
-      for (int d = 0; d < 3; ++d) s_org[c][d] = origin[3 * s + d];
-      for (int r = 0; r < 6; ++r) s_coef[c][r] = coef[6 * s + r];
-    }
-    __syncthreads();
-    // x/y rows of the tables (rows 0, 1, 3, 4, 6, 7 of [9][ngd])
-    for (int e = threadIdx.x; e < nc * 6 * ngd; e += blockDim.x) {
-      const int c = e / (6 * ngd), r = (e / ngd) % 6, i = e % ngd;
-      const int row = r + (r >> 1);  // 0 1 3 4 6 7
-      s_xy[c][r][i] = tab[(size_t)s_cid[c] * 9 * ngd + row * ngd + i];
-    }
-    // z functions at the tile's 16 z (nz >= 16 + ngd: one overlap interval)
-    for (int e = threadIdx.x; e < nc * 16; e += blockDim.x) {
-      const int c = e >> 4, z = e & 15;
-      int o = pz0 + z - s_org[c][2];
-      if (o < 0) o += G.nz;
-      if (o >= G.nz) o -= G.nz;
-      T g = 0, gd = 0, gg = 0;
-      if (o < ngd) {
-        const T *tb = tab + (size_t)s_cid[c] * 9 * ngd;
-        g = tb[2 * ngd + o];
-        gd = tb[5 * ngd + o];
-        gg = gd * tb[8 * ngd + o];
+  // candidates in passes of 256, only the stencils that meet the tile: the
+  // sums run in candidate order and skip only exact zeros
+  for (int base = 0; base < ntot; base += 256) {
+    const int nk = tile_filter_pass(G, px0, py0, pz0, base, ntot, ids, origin,
+                                    s_bfirst, s_bstart, s_wsum, s_kid, s_korg);
+    for (int c0 = 0; c0 < nk; c0 += kMfmaChunk) {
+      const int nc = min(kMfmaChunk, nk - c0);
+      // one level of loads: coefficients, x/y table rows (0, 1, 3, 4, 6, 7
+      // of [9][ngd]) and the z functions at the tile's 16 z
+      for (int e = threadIdx.x; e < nc * 6; e += blockDim.x) {
+        const int c = e / 6, r = e - 6 * c;
+        s_coef[c][r] = coef[6 * s_kid[c0 + c] + r];
       }
-      s_zt[c][0][z] = g;
-      s_zt[c][1][z] = gd;
-      s_zt[c][2][z] = gg;
+      for (int e = threadIdx.x; e < nc * 6 * ngd; e += blockDim.x) {
+        const int c = e / (6 * ngd), r = (e / ngd) % 6, i = e % ngd;
+        const int row = r + (r >> 1);  // 0 1 3 4 6 7
+        s_xy[c][r][i] = tab[(size_t)s_kid[c0 + c] * 9 * ngd + row * ngd + i];
+      }
+      for (int e = threadIdx.x; e < nc * 16; e += blockDim.x) {
+        const int c = e >> 4, z = e & 15;
+        int o = pz0 + z - (s_korg[c0 + c] >> 20);
+        if (o < 0) o += G.nz;
+        if (o >= G.nz) o -= G.nz;
+        T g = 0, gd = 0, gg = 0;
+        if (o < ngd) {
+          const T *tb = tab + (size_t)s_kid[c0 + c] * 9 * ngd;
+          g = tb[2 * ngd + o];
+          gd = tb[5 * ngd + o];
+          gg = gd * tb[8 * ngd + o];
+        }
+        s_zt[c][0][z] = g;
+        s_zt[c][1][z] = gd;
+        s_zt[c][2][z] = gg;
+      }
+      __syncthreads();
+      for (int c4 = 0; c4 < nc; c4 += 4) {
+        const int c = c4 + q;
+        const bool live = c < nc;
+        const int cs = live ? c : 0;
+        const int og = s_korg[c0 + cs];
+        int ix = px - (og & 1023);
+        if (ix < 0) ix += G.nx;
+        int iy = py - ((og >> 10) & 1023);
+        if (iy < 0) iy += G.ny;
+        const bool in = live && ix < ngd && iy < ngd;
+        const int ixc = in ? ix : 0, iyc = in ? iy : 0;
+        const T *cf = s_coef[cs];
+        const T gxy = s_xy[cs][0][ixc] * s_xy[cs][1][iyc];
+        const T dxy = s_xy[cs][2][ixc] * s_xy[cs][3][iyc];
+        const T xx = s_xy[cs][4][ixc], yy = s_xy[cs][5][iyc];
+        const T ax = cf[5] * yy, ay = -cf[5] * xx, az = cf[4] * xx - cf[3] * yy;
+        const T zero = 0;
+        const T b0 = live ? s_zt[cs][0][i16] : zero;
+        const T b1 = live ? s_zt[cs][1][i16] : zero;
+        const T b2 = live ? s_zt[cs][2][i16] : zero;
+        acc0 = Mfma<T>::step(in ? cf[0] * gxy : zero, b0, acc0);
+        acc1 = Mfma<T>::step(in ? cf[1] * gxy : zero, b0, acc1);
+        acc2 = Mfma<T>::step(in ? cf[2] * gxy : zero, b0, acc2);
+        acc0 = Mfma<T>::step(in ? ax * dxy : zero, b1, acc0);
+        acc1 = Mfma<T>::step(in ? ay * dxy : zero, b1, acc1);
+        acc2 = Mfma<T>::step(in ? az * dxy : zero, b1, acc2);
+        acc0 = Mfma<T>::step(in ? -cf[4] * dxy : zero, b2, acc0);
+        acc1 = Mfma<T>::step(in ? cf[3] * dxy : zero, b2, acc1);
+      }
+      __syncthreads();  // the next round overwrites the staged candidates
     }
-    __syncthreads();
-    for (int c4 = 0; c4 < nc; c4 += 4) {
-      const int c = c4 + q;
-      const bool live = c < nc;
-      const int cs = live ? c : 0;
-      int ix = px - s_org[cs][0];
-      if (ix < 0) ix += G.nx;
-      int iy = py - s_org[cs][1];
-      if (iy < 0) iy += G.ny;
-      const bool in = live && ix < ngd && iy < ngd;
-      const int ixc = in ? ix : 0, iyc = in ? iy : 0;
-      const T *cf = s_coef[cs];
-      const T gxy = s_xy[cs][0][ixc] * s_xy[cs][1][iyc];
-      const T dxy = s_xy[cs][2][ixc] * s_xy[cs][3][iyc];
-      const T xx = s_xy[cs][4][ixc], yy = s_xy[cs][5][iyc];
-      const T ax = cf[5] * yy, ay = -cf[5] * xx, az = cf[4] * xx - cf[3] * yy;
-      const T zero = 0;
-      const T b0 = live ? s_zt[cs][0][i16] : zero;
-      const T b1 = live ? s_zt[cs][1][i16] : zero;
-      const T b2 = live ? s_zt[cs][2][i16] : zero;
-      acc0 = Mfma<T>::step(in ? cf[0] * gxy : zero, b0, acc0);
-      acc1 = Mfma<T>::step(in ? cf[1] * gxy : zero, b0, acc1);
-      acc2 = Mfma<T>::step(in ? cf[2] * gxy : zero, b0, acc2);
-      acc0 = Mfma<T>::step(in ? ax * dxy : zero, b1, acc0);
-      acc1 = Mfma<T>::step(in ? ay * dxy : zero, b1, acc1);
-      acc2 = Mfma<T>::step(in ? az * dxy : zero, b1, acc2);
-      acc0 = Mfma<T>::step(in ? -cf[4] * dxy : zero, b2, acc0);
-      acc1 = Mfma<T>::step(in ? cf[3] * dxy : zero, b2, acc1);
-    }
-    __syncthreads();  // the next round overwrites the staged candidates
   }
   // lane holds R[xy = wave*16 + Mfma<T>::row(lane, r)][z = i16]: 16 lanes
   // write 16 consecutive z of one (x, y) column
@@ -1408,8 +1472,8 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   double dx3 = p.dx * p.dx * p.dx;
   bool okn = with_NGD(p.ngd, [&](auto g) {
     constexpr int NG = decltype(g)::value;
-    hipLaunchKernelGGL((k_interp<T, NG>), dim3(8 * ceil_div(ceil_div(np, 4), 8)),
-                       dim3(256), 0,
+    hipLaunchKernelGGL((k_interp<T, NG>),
+                       dim3(8 * ceil_div(ceil_div(np, 4), 8)), dim3(256), 0,
                        st, act, np, geom, dx3, tab.as<T>(), origin.as<int>(),
                        ids.as<int>(), R.as<T>(), V, W);
   });
