@@ -42,6 +42,7 @@ constexpr double kPi = 3.14159265358979323846264338327950288;
 constexpr int kMaxNgd = 24;     // largest Gaussian support handled
 constexpr int kCandChunk = 16;  // segments staged in LDS per spread chunk
 constexpr int kMaxBinsPerTile = 64;
+constexpr int kCoef = 8;  // spread coefficients per segment (6 used)
 
 // ---------------------------------------------------------------------
 // prepare: tables [seg][9][ngd] (ga x,y,z | gd x,y,z | gr x,y,z), stencil
@@ -111,11 +112,14 @@ __global__ void k_prepare(const int *__restrict__ act, int np, const double *__r
     o[d] = w;
     origin[3 * s + d] = w;
   }
-  // spread coefficients: F and half torque (GA_setup: GA = T/2 pattern)
+  // spread coefficients: F and half torque (GA_setup: GA = T/2 pattern),
+  // kCoef per segment so a record is whole 16-byte words
   for (int d = 0; d < 3; ++d) {
-    coef[6 * s + d] = (T)F[3 * s + d];
-    coef[6 * s + 3 + d] = (T)(0.5 * Tq[3 * s + d]);
+    coef[kCoef * s + d] = (T)F[3 * s + d];
+    coef[kCoef * s + 3 + d] = (T)(0.5 * Tq[3 * s + d]);
   }
+  coef[kCoef * s + 6] = (T)0;
+  coef[kCoef * s + 7] = (T)0;
   int b = spread_bin(o[0] / tx, o[1] / ty, o[2] / tz, nbx, nby, nbz);
   bin[s] = b;
   slot[s] = atomicAdd(&bin_count[b], 1);
@@ -135,9 +139,13 @@ __global__ void k_scatter(int np, const int *__restrict__ bin,
   ids[start[bin[s]] + slot[s]] = s;
 }
 
-// Ascending segment order inside each bin (atomic slots are unordered).
+// Ascending segment order inside each bin (atomic slots are unordered);
+// idorg (may be null) receives (segment, stencil origin packed 10 bits per
+// axis) in the same order, so a tile's candidates come with one load.
 __global__ void k_sort_bins(int nbins, const int *__restrict__ start,
-                            int *__restrict__ ids) {
+                            int *__restrict__ ids,
+                            const int *__restrict__ origin,
+                            int2 *__restrict__ idorg) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbins) return;
   int lo = start[b], hi = start[b + 1];
@@ -150,6 +158,12 @@ __global__ void k_sort_bins(int nbins, const int *__restrict__ start,
     }
     ids[j + 1] = v;
   }
+  if (idorg)
+    for (int i = lo; i < hi; ++i) {
+      const int sid = ids[i];
+      idorg[i] = make_int2(sid, origin[3 * sid] | (origin[3 * sid + 1] << 10) |
+                                    (origin[3 * sid + 2] << 20));
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -228,7 +242,7 @@ __global__ void __launch_bounds__(256)
       s_cid[c] = s;
       s_org[c][0] = origin[3 * s + 0];
       s_org[c][1] = origin[3 * s + 1];
-      for (int r = 0; r < 6; ++r) s_coef[c][r] = coef[6 * s + r];
+      for (int r = 0; r < 6; ++r) s_coef[c][r] = coef[kCoef * s + r];
       // z overlap of the stencil [oz, oz+ngd) with [pz0, pz0+TZ) (mod nz;
       // nz >= TZ + ngd so at most one interval)
       const int oz = origin[3 * s + 2];
@@ -367,60 +381,58 @@ __device__ __forceinline__ bool stencil_meets(int o, int p0, int te, int ngd,
   return d < te || d > n - ngd;
 }
 
-// Candidate lists of the 8 x 8 x 16 matrix-core tiles (spread and
-// interpolation). tile_bins: the bins whose segments' stencil origins can
-// reach tile (bx, by, bz), their first list entries and running counts;
-// returns the number of candidates (256 threads take part).
+// Candidate lists of the 8 x 8 x 16 matrix-core tiles. tile_bins (wave 0):
+// the bins whose segments' stencil origins can reach tile (bx, by, bz)
+// (nbins <= 64), their first list entries and sizes; returns the largest
+// bin size. The whole workgroup calls it (one barrier).
 __device__ __forceinline__ int tile_bins(const SpreadGeom &G, int bx, int by,
                                          int bz, const int *__restrict__ start,
-                                         int *s_bfirst, int *s_bstart,
-                                         int *s_ntot) {
-  const int cx = min(G.hx + 1, G.nbx), cy = min(G.hy + 1, G.nby),
-            cz = min(G.hz + 1, G.nbz);
-  const int nbins = cx * cy * cz;
-  for (int t = threadIdx.x; t < nbins; t += blockDim.x) {
-    int dz = t / (cx * cy), dy = (t / cx) % cy, dx = t % cx;
-    int ix = ((bx - G.hx + dx) % G.nbx + G.nbx) % G.nbx;
-    int iy = ((by - G.hy + dy) % G.nby + G.nby) % G.nby;
-    int iz = ((bz - G.hz + dz) % G.nbz + G.nbz) % G.nbz;
-    int b = spread_bin(ix, iy, iz, G.nbx, G.nby, G.nbz);
-    s_bfirst[t] = start[b];
-    s_bstart[t + 1] = start[b + 1] - start[b];
+                                         int *s_bfirst, int *s_bcnt,
+                                         int *s_maxc) {
+  if (threadIdx.x < 64) {
+    const int cx = min(G.hx + 1, G.nbx), cy = min(G.hy + 1, G.nby),
+              cz = min(G.hz + 1, G.nbz);
+    const int t = threadIdx.x;
+    int n = 0;
+    if (t < cx * cy * cz) {
+      int dz = t / (cx * cy), dy = (t / cx) % cy, dx = t % cx;
+      int ix = ((bx - G.hx + dx) % G.nbx + G.nbx) % G.nbx;
+      int iy = ((by - G.hy + dy) % G.nby + G.nby) % G.nby;
+      int iz = ((bz - G.hz + dz) % G.nbz + G.nbz) % G.nbz;
+      int b = spread_bin(ix, iy, iz, G.nbx, G.nby, G.nbz);
+      const int f = start[b];
+      n = start[b + 1] - f;
+      s_bfirst[t] = f;
+      s_bcnt[t] = n;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) n = max(n, __shfl_xor(n, off, 64));
+    if (t == 0) *s_maxc = n;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    s_bstart[0] = 0;
-    for (int t = 0; t < nbins; ++t) s_bstart[t + 1] += s_bstart[t];
-    *s_ntot = s_bstart[nbins];
-  }
-  __syncthreads();
-  return *s_ntot;
+  return *s_maxc;
 }
 
-// One pass over candidates [base, base + 256) of a tile starting at (px0,
-// py0, pz0): keeps the segments whose stencil meets the tile, in candidate
-// order (s_kid: segment, s_korg: origin packed 10 bits per axis) and
-// returns how many. The bins over-include; the kept list skips only
-// segments that contribute exact zeros to the tile.
+// One pass of the tile's candidates: thread t takes entry k0 + t % K of bin
+// t / K (K = 256 / nbins slots per bin and pass), so the candidate order
+// is (pass, bin, position in bin). Keeps the segments whose stencil meets
+// the tile starting at (px0, py0, pz0), in that order (s_kid: segment,
+// s_korg: origin packed 10 bits per axis), and returns how many. The bins
+// over-include; the kept list skips only segments that contribute exact
+// zeros to the tile.
 __device__ __forceinline__ int tile_filter_pass(
-    const SpreadGeom &G, int px0, int py0, int pz0, int base, int ntot,
-    const int *__restrict__ ids, const int *__restrict__ origin,
-    const int *s_bfirst, const int *s_bstart, int *s_wsum, int *s_kid,
-    int *s_korg) {
+    const SpreadGeom &G, int px0, int py0, int pz0, int k0, int K, int nbins,
+    const int2 *__restrict__ idorg, const int *s_bfirst, const int *s_bcnt,
+    int *s_wsum, int *s_kid, int *s_korg) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int gc = base + threadIdx.x;
+  const int b = threadIdx.x / K, k = k0 + threadIdx.x - b * K;
   bool keep = false;
-  int sid = 0, org = 0;
-  if (gc < ntot) {
-    int t = 0;
-    while (gc >= s_bstart[t + 1]) ++t;
-    sid = ids[s_bfirst[t] + gc - s_bstart[t]];
-    const int ox = origin[3 * sid + 0], oy = origin[3 * sid + 1],
-              oz = origin[3 * sid + 2];
-    keep = stencil_meets(ox, px0, 8, G.ngd, G.nx) &&
-           stencil_meets(oy, py0, 8, G.ngd, G.ny) &&
-           stencil_meets(oz, pz0, 16, G.ngd, G.nz);
-    org = ox | (oy << 10) | (oz << 20);
+  int2 e = make_int2(0, 0);
+  if (b < nbins && k < s_bcnt[b]) {
+    e = idorg[s_bfirst[b] + k];
+    keep = stencil_meets(e.y & 1023, px0, 8, G.ngd, G.nx) &&
+           stencil_meets((e.y >> 10) & 1023, py0, 8, G.ngd, G.ny) &&
+           stencil_meets(e.y >> 20, pz0, 16, G.ngd, G.nz);
   }
   const unsigned long long kb = __ballot(keep);
   const int below = __popcll(kb & ((1ull << lane) - 1ull));
@@ -430,32 +442,57 @@ __device__ __forceinline__ int tile_filter_pass(
   for (int w = 0; w < wave; ++w) off += s_wsum[w];
   const int nk = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
   if (keep) {
-    s_kid[off + below] = sid;
-    s_korg[off + below] = org;
+    s_kid[off + below] = e.x;
+    s_korg[off + below] = e.y;
   }
   __syncthreads();
   return nk;
 }
 
+// Copies the records (WORDS units of U each) of segments s_kid[c0 ..
+// c0 + n) to dst[0 .. n) with independent loads; 256 threads.
+template <typename U, int WORDS>
+__device__ __forceinline__ void stage_records(const U *__restrict__ src,
+                                              const int *s_kid, int c0, int n,
+                                              U *dst) {
+  for (int e = threadIdx.x; e < n * WORDS; e += 256) {
+    const int c = e / WORDS, w = e - c * WORDS;
+    dst[e] = src[(size_t)s_kid[c0 + c] * WORDS + w];
+  }
+}
+
+// Record of R reals staged as 16-byte words when it is made of whole words.
+template <typename T, int R>
+__device__ __forceinline__ void stage_reals(const T *__restrict__ src,
+                                            const int *s_kid, int c0, int n,
+                                            T *dst) {
+  if constexpr ((R * sizeof(T)) % 16 == 0)
+    stage_records<int4, R * sizeof(T) / 16>(
+        reinterpret_cast<const int4 *>(src), s_kid, c0, n,
+        reinterpret_cast<int4 *>(dst));
+  else
+    stage_records<T, R>(src, s_kid, c0, n, dst);
+}
+
 template <typename T, int NGD>
 __global__ void __launch_bounds__(256)
     k_spread_mfma(const int *__restrict__ act, SpreadGeom G,
-                  const T *__restrict__ tab, const int *__restrict__ origin,
-                  const T *__restrict__ coef, const int *__restrict__ start,
-                  const int *__restrict__ ids, T *__restrict__ R) {
+                  const T *__restrict__ tab, const T *__restrict__ coef,
+                  const int *__restrict__ start,
+                  const int2 *__restrict__ idorg, T *__restrict__ R) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int kMfmaChunk = fs::kMfmaChunk<T>;
+  constexpr int ngd = NGD;
+  constexpr int kTab = 9 * NGD;  // table record [9][ngd]
   if (act && !*act) return;  // batched sweep: suspension converged
-  __shared__ int s_bstart[kMaxBinsPerTile + 1];
+  __shared__ int s_bcnt[kMaxBinsPerTile];
   __shared__ int s_bfirst[kMaxBinsPerTile];
-  __shared__ int s_ntot;
+  __shared__ int s_maxc;
   __shared__ int s_wsum[4];
-  // candidates of one pass that overlap the tile, in candidate order: id
-  // and stencil origin packed 10 bits per axis
+  // candidates of one pass that meet the tile, in candidate order
   __shared__ int s_kid[256], s_korg[256];
-  __shared__ T s_xy[kMfmaChunk][6][NGD];  // gx gy gdx gdy grad_x grad_y
-  __shared__ T s_zt[kMfmaChunk][3][16];   // B_0..B_2 at the tile's z
-  __shared__ T s_coef[kMfmaChunk][6];
+  __shared__ __align__(16) T s_tab[kMfmaChunk][kTab];
+  __shared__ __align__(16) T s_coef[kMfmaChunk][kCoef];
 
   const int ntiles = G.nbx * G.nby * G.nbz;
   int bid = blockIdx.x;  // XCD-aware order, as k_spread
@@ -463,51 +500,29 @@ __global__ void __launch_bounds__(256)
   const int bx = bid % G.nbx;
   const int by = (bid / G.nbx) % G.nby;
   const int bz = bid / (G.nbx * G.nby);
-  constexpr int ngd = NGD;
-  const int ntot = tile_bins(G, bx, by, bz, start, s_bfirst, s_bstart, &s_ntot);
+  const int maxc = tile_bins(G, bx, by, bz, start, s_bfirst, s_bcnt, &s_maxc);
+  const int nbins = min(G.hx + 1, G.nbx) * min(G.hy + 1, G.nby) *
+                    min(G.hz + 1, G.nbz);
+  const int K = 256 / nbins;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int q = lane >> 4, i16 = lane & 15;
   const int xy = wave * 16 + i16;  // this lane's A row
   const int px0 = bx * 8, py0 = by * 8, pz0 = bz * 16;
-  const int px = px0 + (xy >> 3), py = py0 + (xy & 7);
+  const int px = px0 + (xy >> 3), py = py0 + (xy & 7), pz = pz0 + i16;
   v4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
 
-  // candidates in passes of 256, only the stencils that meet the tile: the
-  // sums run in candidate order and skip only exact zeros
-  for (int base = 0; base < ntot; base += 256) {
-    const int nk = tile_filter_pass(G, px0, py0, pz0, base, ntot, ids, origin,
-                                    s_bfirst, s_bstart, s_wsum, s_kid, s_korg);
+  for (int k0 = 0; k0 < maxc; k0 += K) {
+    const int nk = tile_filter_pass(G, px0, py0, pz0, k0, K, nbins, idorg,
+                                    s_bfirst, s_bcnt, s_wsum, s_kid, s_korg);
     for (int c0 = 0; c0 < nk; c0 += kMfmaChunk) {
       const int nc = min(kMfmaChunk, nk - c0);
-      // one level of loads: coefficients, x/y table rows (0, 1, 3, 4, 6, 7
-      // of [9][ngd]) and the z functions at the tile's 16 z
-      for (int e = threadIdx.x; e < nc * 6; e += blockDim.x) {
-        const int c = e / 6, r = e - 6 * c;
-        s_coef[c][r] = coef[6 * s_kid[c0 + c] + r];
-      }
-      for (int e = threadIdx.x; e < nc * 6 * ngd; e += blockDim.x) {
-        const int c = e / (6 * ngd), r = (e / ngd) % 6, i = e % ngd;
-        const int row = r + (r >> 1);  // 0 1 3 4 6 7
-        s_xy[c][r][i] = tab[(size_t)s_kid[c0 + c] * 9 * ngd + row * ngd + i];
-      }
-      for (int e = threadIdx.x; e < nc * 16; e += blockDim.x) {
-        const int c = e >> 4, z = e & 15;
-        int o = pz0 + z - (s_korg[c0 + c] >> 20);
-        if (o < 0) o += G.nz;
-        if (o >= G.nz) o -= G.nz;
-        T g = 0, gd = 0, gg = 0;
-        if (o < ngd) {
-          const T *tb = tab + (size_t)s_kid[c0 + c] * 9 * ngd;
-          g = tb[2 * ngd + o];
-          gd = tb[5 * ngd + o];
-          gg = gd * tb[8 * ngd + o];
-        }
-        s_zt[c][0][z] = g;
-        s_zt[c][1][z] = gd;
-        s_zt[c][2][z] = gg;
-      }
+      // one level of 16-byte loads: table and coefficient records
+      stage_reals<T, kTab>(tab, s_kid, c0, nc, &s_tab[0][0]);
+      stage_reals<T, kCoef>(coef, s_kid, c0, nc, &s_coef[0][0]);
       __syncthreads();
       for (int c4 = 0; c4 < nc; c4 += 4) {
+        // lane group q takes candidate c4 + q; every read is in range
+        // (clamped indices) and the selects zero what lies outside
         const int c = c4 + q;
         const bool live = c < nc;
         const int cs = live ? c : 0;
@@ -516,17 +531,23 @@ __global__ void __launch_bounds__(256)
         if (ix < 0) ix += G.nx;
         int iy = py - ((og >> 10) & 1023);
         if (iy < 0) iy += G.ny;
+        int iz = pz - (og >> 20);
+        if (iz < 0) iz += G.nz;
         const bool in = live && ix < ngd && iy < ngd;
-        const int ixc = in ? ix : 0, iyc = in ? iy : 0;
+        const bool zin = live && iz < ngd;
+        const int ixc = ix < ngd ? ix : 0, iyc = iy < ngd ? iy : 0,
+                  izc = iz < ngd ? iz : 0;
+        const T *tb = s_tab[cs];
         const T *cf = s_coef[cs];
-        const T gxy = s_xy[cs][0][ixc] * s_xy[cs][1][iyc];
-        const T dxy = s_xy[cs][2][ixc] * s_xy[cs][3][iyc];
-        const T xx = s_xy[cs][4][ixc], yy = s_xy[cs][5][iyc];
+        const T gxy = tb[ixc] * tb[ngd + iyc];
+        const T dxy = tb[3 * ngd + ixc] * tb[4 * ngd + iyc];
+        const T xx = tb[6 * ngd + ixc], yy = tb[7 * ngd + iyc];
+        const T gz = tb[2 * ngd + izc], gdz = tb[5 * ngd + izc],
+                grz = tb[8 * ngd + izc];
         const T ax = cf[5] * yy, ay = -cf[5] * xx, az = cf[4] * xx - cf[3] * yy;
         const T zero = 0;
-        const T b0 = live ? s_zt[cs][0][i16] : zero;
-        const T b1 = live ? s_zt[cs][1][i16] : zero;
-        const T b2 = live ? s_zt[cs][2][i16] : zero;
+        const T b0 = zin ? gz : zero, b1 = zin ? gdz : zero,
+                b2 = zin ? gdz * grz : zero;
         acc0 = Mfma<T>::step(in ? cf[0] * gxy : zero, b0, acc0);
         acc1 = Mfma<T>::step(in ? cf[1] * gxy : zero, b0, acc1);
         acc2 = Mfma<T>::step(in ? cf[2] * gxy : zero, b0, acc2);
@@ -1217,7 +1238,8 @@ int FcmImpl::init(const fs_fcm_params &pp, int maxnp, int prec) {
   }
   FS_TRY(tab.alloc((size_t)maxnp * 9 * p.ngd * tsz));
   FS_TRY(origin.alloc((size_t)maxnp * 3 * sizeof(int)));
-  FS_TRY(coef.alloc((size_t)maxnp * 6 * tsz));
+  FS_TRY(coef.alloc((size_t)maxnp * kCoef * tsz));
+  FS_TRY(idorg.alloc((size_t)std::max(maxnp, 1) * sizeof(int2)));
   FS_TRY(bin.alloc((size_t)maxnp * sizeof(int)));
   FS_TRY(slot.alloc((size_t)maxnp * sizeof(int)));
   FS_TRY(ids.alloc((size_t)maxnp * sizeof(int)));
@@ -1331,6 +1353,7 @@ void FcmImpl::release() {
   bin.release();
   slot.release();
   ids.release();
+  idorg.release();
   bin_count.release();
   bin_start.release();
   scan_tmp.release();
@@ -1389,7 +1412,9 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                      bin.as<int>(), slot.as<int>(), bin_start.as<int>(),
                      ids.as<int>());
   hipLaunchKernelGGL(k_sort_bins, dim3(ceil_div(nbins, 64)), dim3(64), 0, st,
-                     nbins, bin_start.as<int>(), ids.as<int>());
+                     nbins, bin_start.as<int>(), ids.as<int>(),
+                     (const int *)origin.as<int>(),
+                     mfma_spread ? idorg.as<int2>() : (int2 *)nullptr);
   mark(1);
   const int ntiles = nbins;
   bool okz = true;
@@ -1397,9 +1422,8 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
     okz = with_NGD(p.ngd, [&](auto g) {
       constexpr int NG = decltype(g)::value;
       hipLaunchKernelGGL((k_spread_mfma<T, NG>), dim3(ntiles), dim3(256), 0, st,
-                         act, geom, tab.as<T>(), origin.as<int>(),
-                         coef.as<T>(), bin_start.as<int>(),
-                         ids.as<int>(), R.as<T>());
+                         act, geom, tab.as<T>(), coef.as<T>(),
+                         bin_start.as<int>(), idorg.as<int2>(), R.as<T>());
     });
   if (!mfma_spread)
     okz = with_TZ(geom.tz, [&](auto z) {

@@ -42,6 +42,7 @@ struct FcmImpl {
   const int *act = nullptr;
   dbuf R, C, twx, twy, twz;
   dbuf tab, origin, coef, bin, slot, ids, bin_count, bin_start, scan_tmp;
+  dbuf idorg;  // (segment, packed stencil origin) in bin order
   hipEvent_t ev[8] = {};
 
   int init(const fs_fcm_params &pp, int maxnp, int prec);

@@ -854,187 +854,252 @@ __global__ void __launch_bounds__(1024)
   (void)idt;
 }
 
-// Register-resident variant for n <= 32*NB: 1024 threads as a 32 x 32
-// grid; thread (ty, tx) owns A[i][j] for i = ty + 32a, j = tx + 32b (rows
-// and columns past n are zero padding). Pivot: largest |value| among unused
-// rows, compared on |value| with its 8 lowest mantissa bits dropped (ties
-// within 2^-44 go to the lowest row). Per pivot step:
-//   1. every wave reduces the 64-bit keys of column k (each row group's
-//      largest, published with the column in colkey/colbuf[k & 1]) to the
-//      same pivot p with DPP row permutes and four lane reads, so no
-//      barrier separates the search from its use; the owners of row p
-//      publish it in rowbuf; barrier;
-//   2. branch-free update A -= fct_i * r_j with r_j = A[p][j] / A[p][k];
-//      column k (base 0, r = 1/pivot) and row p (r_j) are fixed under
-//      wave-uniform conditions; the owners of column k + 1 publish it and
-//      its keys in the other buffers; barrier.
-// The loop over pivot columns runs in unrolled blocks of 32, so column k's
-// registers are known at compile time (selects on 6, not 36, registers).
-// pivot key of row i (< 256) holding v; 0 marks a used row
-__device__ __forceinline__ unsigned long long pivot_key(double v, int i,
-                                                        bool used) {
-  const unsigned long long m =
-      ((unsigned long long)__double_as_longlong(fabs(v)) & ~0xFFull) |
-      (unsigned long long)(255 - i);
-  return used ? 0ull : m;
+// Block-tridiagonal factorisation of the approximate Jacobian.
+// Differencing the position rows of beads k >= 2 (row k minus row k - 1)
+// turns the cumulative BlockC rows (Filament.hpp:885-909) into two blocks
+// and BlockD into three; ordering the unknowns per bead, [X0 U_0 lam_0 |
+// U_1 lam_1 | .. | U_{Nw-1}], and the equations per bead, [P_k L_k] (P_0
+// the first-bead rows), leaves A^ = P_r T J P_c^T with at most 8 sub- and 8
+// superdiagonals (the zeros outside are exact: the cumulative entries of
+// neighbouring rows are equal). Cut into blocks of kBlk = 12 rows and
+// columns, A^ is block tridiagonal, and
+//   J^-1 b = P_c^T A^^-1 P_r T b
+// follows from the block LU recursion (block Thomas):
+//   D_K = A_KK - A_K,K-1 G_K-1,  G_K = D_K^-1 A_K,K+1,
+//   y_K = D_K^-1 (b_K - A_K,K-1 y_K-1),  x_K = y_K - G_K x_K+1,
+// with each D_K inverted by Gauss-Jordan with partial pivoting inside the
+// block. That replaces the dense 6Nw x 6Nw inverse (arma::inv,
+// Filament.hpp:862): O(Nw) work and storage, and a solve reads 3 small
+// blocks per 12 rows instead of a dense row of J^-1 per row. (The solution
+// agrees with the dense inverse to rounding on the reference's Jacobians,
+// including stiff swimmers; a block D_K that is singular or not finite
+// raises the singular flag.)
+constexpr int kBlk = 12;
+constexpr int kBlkSz = kBlk * kBlk;
+constexpr int kBlkRec = 3 * kBlkSz;  // D^-1, G, A_K,K-1 per block row
+constexpr int kBandMaxNwBlk = 32;  // block rows of the longest filament (Nw <= 64)
+
+// Original unknown (column of J) of permuted unknown j, and original
+// equation (row of J) of permuted equation i; *diff: the row is differenced
+// with row - 3.
+__device__ __forceinline__ int band_col(int j, int nw) {
+  if (j < 3) return j;  // X0
+  const int k = (j - 3) / 6, w = (j - 3) - 6 * k;
+  return w < 3 ? 3 + 3 * k + w : 3 * nw + 3 * k + w;  // U_k, lam_k
+}
+__device__ __forceinline__ int band_row(int i, int nw, bool *diff) {
+  const int k = i / 6, w = i - 6 * k;
+  *diff = (w < 3) && k >= 2;
+  return w < 3 ? 3 * k + w : 3 * nw + 3 * k + (w - 3);  // P_k, L_k
+}
+// A^(i, j) from the dense J (n x n row-major); 0 outside [0, n)^2
+__device__ __forceinline__ double band_entry(const double *J, int n, int nw,
+                                             int i, int j) {
+  if (i >= n || j >= n || i < 0 || j < 0) return 0.0;
+  bool diff;
+  const int ro = band_row(i, nw, &diff), co = band_col(j, nw);
+  double v = J[(size_t)ro * n + co];
+  if (diff) v -= J[(size_t)(ro - 3) * n + co];
+  return v;
+}
+// Block factors of filament f start here (block rows <= Nw / 2 + 1 each).
+__device__ __forceinline__ size_t blk_offset(const SysDev &S, int f) {
+  return ((size_t)S.boff[f] + 2 * (size_t)f) * (kBlkRec / 2);
 }
 
-template <int CTRL>
-__device__ __forceinline__ unsigned long long dpp_max_u64(
-    unsigned long long k) {
-  const int lo = (int)(unsigned)k, hi = (int)(unsigned)(k >> 32);
-  const unsigned lo2 =
-      (unsigned)__builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-  const unsigned hi2 =
-      (unsigned)__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
-  const unsigned long long k2 = ((unsigned long long)hi2 << 32) | lo2;
-  return k2 > k ? k2 : k;
-}
-
-// maximum over the 64 lanes, returned wave-uniform
-__device__ __forceinline__ unsigned long long wave_max_u64(
-    unsigned long long k) {
-  k = dpp_max_u64<0xB1>(k);   // quad_perm [1,0,3,2]
-  k = dpp_max_u64<0x4E>(k);   // quad_perm [2,3,0,1]
-  k = dpp_max_u64<0x141>(k);  // row_half_mirror
-  k = dpp_max_u64<0x140>(k);  // row_mirror: every row of 16 holds its max
-  const int lo = (int)(unsigned)k, hi = (int)(unsigned)(k >> 32);
-  unsigned long long best = 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const unsigned long long v =
-        ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 16 * r)
-         << 32) |
-        (unsigned)__builtin_amdgcn_readlane(lo, 16 * r);
-    best = v > best ? v : best;
-  }
-  return best;
-}
-
-template <int NB, int TY>
-__global__ void __launch_bounds__(32 * TY)
-    k_gj_inverse_reg(SysDev S, const double *__restrict__ Jall,
-                     double *__restrict__ JTall, int *__restrict__ singular) {
-  __shared__ double colbuf[2][32 * NB], rowbuf[32 * NB];
-  __shared__ unsigned long long colkey[2][TY];  // max key per row group
-  __shared__ int used[32 * NB], pivrow[32 * NB], rowstep[32 * NB];
-  __shared__ double s_inv;  // 1 / pivot of the current step
+// One workgroup of 256 threads per filament: block rows in order, the
+// 12 x 12 operations spread over the threads (padding rows past n carry an
+// identity diagonal).
+__global__ void __launch_bounds__(256)
+    k_blk_factor(SysDev S, const double *__restrict__ Jall,
+                 double *__restrict__ Fall, int *__restrict__ singular) {
+  if (S.act && !*S.act) return;
+  __shared__ double sA[kBlk][2 * kBlk];  // [D | I] -> [I | D^-1]
+  __shared__ double sL[kBlkSz], sU[kBlkSz], sG[kBlkSz];
+  __shared__ int s_piv;
   const int f = blockIdx.x;
-  const int nw = S.nw[f];
-  const int n = 6 * nw;
+  const int nw = S.nw[f], n = 6 * nw, nb = (n + kBlk - 1) / kBlk;
   const double *J = Jall + S.joff[f];
-  double *JT = JTall + S.joff[f];
-  const int tid = threadIdx.x, ty = tid >> 5, tx = tid & 31, lane = tid & 63;
-  constexpr int RA = 32 * NB / TY;  // row blocks per thread
-  double A[RA][NB];
-#pragma unroll
-  for (int a = 0; a < RA; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      int i = ty + TY * a, j = tx + 32 * b;
-      A[a][b] = (i < n && j < n) ? J[(size_t)i * n + j] : 0.0;
+  double *F = Fall + blk_offset(S, f);
+  const int tid = threadIdx.x;
+  bool bad = false;
+  for (int e = tid; e < kBlkSz; e += 256) sG[e] = 0.0;
+  for (int K = 0; K < nb; ++K) {
+    const int r0 = K * kBlk;
+    // A_K,K-1 and A_K,K+1 of this block row; D = A_KK (identity padding)
+    for (int e = tid; e < kBlkSz; e += 256) {
+      const int i = e / kBlk, j = e - i * kBlk;
+      sL[e] = band_entry(J, n, nw, r0 + i, r0 - kBlk + j);
+      sU[e] = band_entry(J, n, nw, r0 + i, r0 + kBlk + j);
+      double d = band_entry(J, n, nw, r0 + i, r0 + j);
+      if (r0 + i >= n && i == j) d = 1.0;
+      sA[i][j] = d;
+      sA[i][kBlk + j] = i == j ? 1.0 : 0.0;
     }
-  for (int i = tid; i < 32 * NB; i += blockDim.x) used[i] = (i < n) ? 0 : 1;
-  if (tx == 0) {
-    unsigned long long km = 0;
+    __syncthreads();
+    // D -= A_K,K-1 G_K-1
+    if (K > 0)
+      for (int e = tid; e < kBlkSz; e += 256) {
+        const int i = e / kBlk, j = e - i * kBlk;
+        double acc = 0.0;
 #pragma unroll
-    for (int a = 0; a < RA; ++a) {
-      const int i = ty + TY * a;
-      colbuf[0][i] = A[a][0];
-      const unsigned long long kk = pivot_key(A[a][0], i, i >= n);
-      km = kk > km ? kk : km;
-    }
-    colkey[0][ty] = km;
-  }
-  __syncthreads();
-  // pivot columns in blocks of 32: the block loop is unrolled, so inside
-  // each block column k's registers are A[.][KB] and the next column's
-  // A[.][KB] or A[.][KB + 1] with KB a constant, not selects over all of A
-#pragma unroll
-  for (int KB = 0; KB < NB; ++KB) {
-    const int KB1 = KB + 1 < NB ? KB + 1 : KB;
-    const int kend = min(32, n - 32 * KB);
-    for (int kt = 0; kt < kend; ++kt) {
-      const int k = 32 * KB + kt;
-      const double *cb = colbuf[k & 1];
-      const unsigned long long *ck = colkey[k & 1];
-      unsigned long long key = lane < TY ? ck[lane] : 0ull;
-      key = wave_max_u64(key);
-      int p = 255 - (int)(key & 0xFF);
-      // no nonzero unused entry, or a pivot that is NaN or infinite (the
-      // key's top bits are the exponent of |pivot|)
-      const bool bad = (key >> 8) == 0 || p >= n || (key >> 52) >= 0x7FFull;
-      p = p >= n ? n - 1 - k : p;  // keeps indices in range; result is flagged
-      const int pb = p / TY, pt = p % TY;
-      if (ty == pt) {
-#pragma unroll
-        for (int a = 0; a < RA; ++a)
-          if (a == pb) {
-#pragma unroll
-            for (int b = 0; b < NB; ++b) rowbuf[tx + 32 * b] = A[a][b];
-            // the pivot's owner divides once, before the barrier
-            if (tx == kt) s_inv = 1.0 / A[a][KB];
-          }
+        for (int m = 0; m < kBlk; ++m) acc += sL[i * kBlk + m] * sG[m * kBlk + j];
+        sA[i][j] -= acc;
       }
-      if (tid == 0) {
-        if (bad) atomicExch(singular, 1);
-        pivrow[k] = p;
-        rowstep[p] = k;
-      }
-      __syncthreads();
-      const double inv = s_inv;
-      double r[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) r[b] = rowbuf[tx + 32 * b] * inv;
-      const bool colk = (tx == kt);
-      r[KB] = colk ? inv : r[KB];
-#pragma unroll
-      for (int a = 0; a < RA; ++a) A[a][KB] = colk ? 0.0 : A[a][KB];
-#pragma unroll
-      for (int a = 0; a < RA; ++a) {
-        const double fct = cb[ty + TY * a];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) A[a][b] = fma(-fct, r[b], A[a][b]);
-      }
-      if (ty == pt) {
-#pragma unroll
-        for (int a = 0; a < RA; ++a)
-          if (a == pb) {
-#pragma unroll
-            for (int b = 0; b < NB; ++b) A[a][b] = r[b];
-          }
-      }
-      if (tid == 0) used[p] = 1;
-      const int k1 = k + 1;
-      if (k1 < n && tx == (k1 & 31)) {
-        unsigned long long km = 0;
-#pragma unroll
-        for (int a = 0; a < RA; ++a) {
-          const int i = ty + TY * a;
-          const double v = kt == 31 ? A[a][KB1] : A[a][KB];
-          colbuf[k1 & 1][i] = v;
-          // used[p] of this step may not be visible yet: test i == p too
-          const unsigned long long kk =
-              pivot_key(v, i, i >= n || i == p || used[i]);
-          km = kk > km ? kk : km;
+    __syncthreads();
+    // [D | I] -> [I | D^-1], partial pivoting over the rows below
+    for (int c = 0; c < kBlk; ++c) {
+      if (tid < 64) {
+        double a = -1.0;
+        int t = kBlk;
+        if (tid >= c && tid < kBlk) {
+          const double x = fabs(sA[tid][c]);
+          a = x == x ? x : -0.5;
+          t = tid;
         }
-        colkey[k1 & 1][ty] = km;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const double oa = __shfl_xor(a, off, 64);
+          const int ot = __shfl_xor(t, off, 64);
+          if (oa > a || (oa == a && ot < t)) {
+            a = oa;
+            t = ot;
+          }
+        }
+        if (tid == 0) {
+          bad = bad || !(a > 0.0) || !(a <= DBL_MAX);
+          s_piv = t < kBlk ? t : c;
+        }
       }
       __syncthreads();
+      const int p = s_piv;
+      if (p != c && tid < 2 * kBlk) {
+        const double x = sA[c][tid];
+        sA[c][tid] = sA[p][tid];
+        sA[p][tid] = x;
+      }
+      __syncthreads();
+      const double inv = 1.0 / sA[c][c];
+      // eliminate column c from the other rows (12 x 24 entries, two per
+      // thread at most), then scale row c
+      double upd[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = tid + 256 * h, ui = e / (2 * kBlk), uj = e - ui * 2 * kBlk;
+        upd[h] = (e < kBlk * 2 * kBlk && ui != c)
+                     ? sA[ui][uj] - sA[ui][c] * inv * sA[c][uj]
+                     : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = tid + 256 * h, ui = e / (2 * kBlk), uj = e - ui * 2 * kBlk;
+        if (e < kBlk * 2 * kBlk && ui != c) sA[ui][uj] = upd[h];
+      }
+      __syncthreads();
+      if (tid < 2 * kBlk) sA[c][tid] *= inv;
+      __syncthreads();
+    }
+    // G_K = D^-1 A_K,K+1; store D^-1, G_K, A_K,K-1
+    double *FK = F + (size_t)K * kBlkRec;
+    for (int e = tid; e < kBlkSz; e += 256) {
+      const int i = e / kBlk, j = e - i * kBlk;
+      double acc = 0.0;
+#pragma unroll
+      for (int m = 0; m < kBlk; ++m) acc += sA[i][kBlk + m] * sU[m * kBlk + j];
+      sG[e] = acc;
+      FK[e] = sA[i][kBlk + j];
+      FK[kBlkSz + e] = acc;
+      FK[2 * kBlkSz + e] = sL[e];
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && bad) atomicExch(singular, 1);
+}
+
+// x = J^-1 b for every filament (FilamentJacobianSolver.hpp:49-73 without
+// the history), lambda rows scaled by 1/dt as the reference's inverse
+// (Filament.hpp:863-865). One wavefront per filament, forward then backward
+// block sweeps: lane 4 row + part (row < 12) sums 3 of the 12 products of
+// a row of a 12 x 12 factor, the four parts combine by lane exchange; the
+// right-hand side lives in LDS, each lane's factor entries in registers,
+// loaded one block row ahead of their use.
+__global__ void __launch_bounds__(64)
+    k_blk_solve(SysDev S, const double *__restrict__ Fall,
+                const double *__restrict__ b, double *__restrict__ x) {
+  if (S.act && !*S.act) return;
+  __shared__ double sy[kBandMaxNwBlk * kBlk];  // permuted b, then y, x
+  __shared__ double sr[kBlk];
+  const int f = blockIdx.x;
+  const int nw = S.nw[f], n = 6 * nw, nb = (n + kBlk - 1) / kBlk;
+  const size_t so = 6 * (size_t)S.boff[f];
+  const double *F = Fall + blk_offset(S, f);
+  const int lane = threadIdx.x;
+  for (int i = lane; i < nb * kBlk; i += 64) {
+    double y = 0.0;
+    if (i < n) {
+      bool diff;
+      const int ro = band_row(i, nw, &diff);
+      y = b[so + ro];
+      if (diff) y -= b[so + ro - 3];
+    }
+    sy[i] = y;
+  }
+  const int row = lane >> 2, part = lane & 3;
+  const bool act = row < kBlk;
+  const int mo = act ? row * kBlk + 3 * part : 0;  // this lane's 3 entries
+  auto load3 = [&](const double *M, double *o) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) o[m] = act ? M[mo + m] : 0.0;
+  };
+  auto dot3 = [&](const double *o, const double *v) {
+    double a = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) a += o[m] * v[3 * part + m];
+    a += __shfl_xor(a, 1, 64);
+    a += __shfl_xor(a, 2, 64);
+    return a;
+  };
+  // forward: y_K = D_K^-1 (b_K - A_K,K-1 y_K-1)
+  double dc[3], lc[3], dn[3], ln[3];
+  load3(F, dc);
+  load3(F + 2 * kBlkSz, lc);
+  __syncthreads();
+  for (int K = 0; K < nb; ++K) {
+    if (K + 1 < nb) {
+      load3(F + (size_t)(K + 1) * kBlkRec, dn);
+      load3(F + (size_t)(K + 1) * kBlkRec + 2 * kBlkSz, ln);
+    }
+    double *yK = sy + K * kBlk;
+    const double t = K > 0 ? dot3(lc, yK - kBlk) : 0.0;
+    if (act && part == 0) sr[row] = yK[row] - t;
+    __syncthreads();
+    const double y = dot3(dc, sr);
+    if (act && part == 0) yK[row] = y;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      dc[m] = dn[m];
+      lc[m] = ln[m];
     }
   }
+  // backward: x_K = y_K - G_K x_K+1
+  double gc[3], gn[3];
+  if (nb >= 2) load3(F + (size_t)(nb - 2) * kBlkRec + kBlkSz, gc);
+  for (int K = nb - 2; K >= 0; --K) {
+    if (K >= 1) load3(F + (size_t)(K - 1) * kBlkRec + kBlkSz, gn);
+    double *yK = sy + K * kBlk;
+    const double t = dot3(gc, yK + kBlk);
+    if (act && part == 0) yK[row] -= t;
+    __syncthreads();
 #pragma unroll
-  for (int a = 0; a < RA; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      int i = ty + TY * a, j = tx + 32 * b;
-      if (i < n && j < n) {
-        int r = rowstep[i], c = pivrow[j];
-        double v = A[a][b];
-        if (r >= 3 * nw + 3) v /= S.dt;
-        JT[(size_t)c * n + r] = v;
-      }
-    }
+    for (int m = 0; m < 3; ++m) gc[m] = gn[m];
+  }
+  const double idt = 1.0 / S.dt;
+  for (int i = lane; i < n; i += 64) {
+    const int co = band_col(i, nw);
+    x[so + co] = co >= 3 * nw + 3 ? sy[i] * idt : sy[i];
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -1373,9 +1438,13 @@ int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
   FS_HIP(hipMemcpy(propsb.p, props, sizeof(fs_fil_props) * nf, hipMemcpyHostToDevice));
   d.props = propsb.as<fs_fil_props>();
   FS_TRY(Jb.alloc(sizeof(double) * (size_t)hjoff[nf]));
-  FS_TRY(JTb.alloc(sizeof(double) * (size_t)hjoff[nf]));
   max_nw = *std::max_element(hnw.begin(), hnw.end());
-  if (6 * max_nw > 32 * 6) FS_TRY(Mb.alloc(sizeof(double) * (size_t)hjoff[nf]));
+  if (banded()) {
+    FS_TRY(Fb.alloc(sizeof(double) * ((size_t)np + 2 * (size_t)nf) * (kBlkRec / 2)));
+  } else {
+    FS_TRY(JTb.alloc(sizeof(double) * (size_t)hjoff[nf]));
+    FS_TRY(Mb.alloc(sizeof(double) * (size_t)hjoff[nf]));
+  }
   FS_TRY(Cm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
   FS_TRY(Dm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
   dot_chunk = 2048;
@@ -1426,6 +1495,7 @@ void SystemImpl::release() {
   Jb.release();
   Mb.release();
   JTb.release();
+  Fb.release();
   Cm.release();
   Dm.release();
   partial.release();
@@ -1600,22 +1670,10 @@ int SystemImpl::build_jacobian_dev(int nt) {
   FS_LAUNCHED();
   int *sing = flags.as<int>() + 4;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, sing, 1, 0);
-  const int nmax = 6 * max_nw;
-  const int NB = (nmax + 31) / 32;
-  if (NB <= 6) {  // whole matrix in registers of one workgroup
-    // TY row groups of 32 threads: 32 (6 x 6 tiles); 16 (12 x 6 tiles, two
-    // waves per SIMD) measured slower at 180 x 180 (2.22 vs 1.94 ms for the
-    // build and inverse of 1,000 Jacobians)
-    constexpr int kTY56 = 32;
-    switch (NB) {
-      case 1:
-      case 2: hipLaunchKernelGGL((k_gj_inverse_reg<2, 32>), dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
-      case 3: hipLaunchKernelGGL((k_gj_inverse_reg<3, 32>), dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
-      case 4: hipLaunchKernelGGL((k_gj_inverse_reg<4, 32>), dim3(nf), dim3(1024), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
-      case 5: hipLaunchKernelGGL((k_gj_inverse_reg<5, kTY56>), dim3(nf), dim3(32 * kTY56), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
-      default: hipLaunchKernelGGL((k_gj_inverse_reg<6, kTY56>), dim3(nf), dim3(32 * kTY56), 0, st, d, Jb.as<double>(), Jinv_cm(), sing); break;
-    }
-  } else {  // larger filaments: the matrix stays in global memory (L2)
+  if (banded()) {  // block-tridiagonal factorisation, one workgroup each
+    hipLaunchKernelGGL(k_blk_factor, dim3(nf), dim3(256), 0, st, d,
+                       Jb.as<double>(), Fb.as<double>(), sing);
+  } else {  // longer filaments: dense Gauss-Jordan in global memory (L2)
     FS_HIP(hipMemcpyAsync(Mb.p, Jb.p, sizeof(double) * nJ,
                           hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_gj_inverse, dim3(nf), dim3(1024), 0, st, d,
@@ -1644,6 +1702,48 @@ int SystemImpl::build_jacobian(int nt) {
   return FS_OK;
 }
 
+// Dense J^-1 of filament f (row-major, host), the reference's inverse with
+// its lambda rows scaled by 1/dt: the stored inverse, or with the banded
+// factors one solve per unit vector (a test and inspection path).
+int SystemImpl::jinv_dense(int f, double *out) {
+  FS_HIP(hipStreamSynchronize(st));
+  const int n = 6 * hnw[f];
+  const size_t so = 6 * (size_t)hboff[f], n6 = 6 * (size_t)np;
+  if (!banded()) {
+    std::vector<double> m((size_t)n * n);
+    FS_HIP(hipMemcpy(m.data(), JTb.as<double>() + hjoff[f],
+                     sizeof(double) * m.size(), hipMemcpyDeviceToHost));
+    for (int r = 0; r < n; ++r)  // stored column-major
+      for (int c = 0; c < n; ++c) out[(size_t)r * n + c] = m[(size_t)c * n + r];
+    return FS_OK;
+  }
+  dbuf work;
+  FS_TRY(work.alloc(sizeof(double) * 2 * n6));
+  double *b = work.as<double>(), *x = b + n6;
+  std::vector<double> col(n);
+  int rc = FS_OK;
+  for (int c = 0; c < n && rc == FS_OK; ++c) {
+    hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (n6 + 255) / 256)),
+                       dim3(256), 0, st, b, n6, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(1), 0, st, b + so + c, (size_t)1,
+                       1.0);
+    rc = jsolve_impl(b, x, 0, nullptr, 0);
+    if (rc == FS_OK &&
+        hipMemcpyAsync(col.data(), x + so, sizeof(double) * n,
+                       hipMemcpyDeviceToHost, st) != hipSuccess) {
+      set_error("copy of an inverse column failed");
+      rc = FS_ERR_HIP;
+    }
+    if (rc == FS_OK && hipStreamSynchronize(st) != hipSuccess) {
+      set_error("inverse column solve failed");
+      rc = FS_ERR_HIP;
+    }
+    for (int r = 0; r < n && rc == FS_OK; ++r) out[(size_t)r * n + c] = col[r];
+  }
+  work.release();
+  return rc;
+}
+
 int SystemImpl::jsolve(const double *b, double *x, int iter) {
   FS_CHECK(iter >= 0 && iter <= p.broyden_maxiter, FS_ERR_ARG, "bad iter %d",
            iter);
@@ -1658,8 +1758,13 @@ int SystemImpl::jsolve_impl(const double *b, double *x, int ncol,
                             const int *ncolp, int noff) {
   ProfScope ps_(*this, P_SLVJ);
   const int *act = d.act;
-  hipLaunchKernelGGL(k_jsolve_block, dim3(nf), dim3(256), 0, st, d, Jinv_cm(),
-                     b, x);
+  if (banded()) {
+    hipLaunchKernelGGL(k_blk_solve, dim3(nf), dim3(64), 0, st, d,
+                       Fb.as<double>(), b, x);
+  } else {
+    hipLaunchKernelGGL(k_jsolve_block, dim3(nf), dim3(256), 0, st, d,
+                       Jinv_cm(), b, x);
+  }
   FS_LAUNCHED();
   const int gy = ncolp ? p.broyden_maxiter : ncol;
   if (gy > 0) {

@@ -350,19 +350,11 @@ extern "C" int fs_fil_build_jacobian(fs_system *h, int nt) {
 static int get_matrix(fs_system *h, int f, double *out, bool inverse) {
   FS_CHECK(h && out, FS_ERR_ARG, "null argument");
   FS_CHECK(f >= 0 && f < h->impl.nf, FS_ERR_ARG, "bad filament %d", f);
+  if (inverse) return h->impl.jinv_dense(f, out);
   FS_HIP(hipStreamSynchronize(h->impl.st));
   int n = 6 * h->impl.hnw[f];
-  size_t nn = (size_t)n * n;
-  std::vector<double> m(nn);
-  const double *src = (inverse ? h->impl.JTb.as<double>() : h->impl.Jb.as<double>()) +
-                      h->impl.hjoff[f];
-  FS_HIP(hipMemcpy(m.data(), src, sizeof(double) * nn, hipMemcpyDeviceToHost));
-  if (inverse) {  // stored column-major
-    for (int r = 0; r < n; ++r)
-      for (int c = 0; c < n; ++c) out[(size_t)r * n + c] = m[(size_t)c * n + r];
-  } else {
-    std::copy(m.begin(), m.end(), out);
-  }
+  FS_HIP(hipMemcpy(out, h->impl.Jb.as<double>() + h->impl.hjoff[f],
+                   sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost));
   return FS_OK;
 }
 

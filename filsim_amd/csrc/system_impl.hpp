@@ -35,6 +35,11 @@ struct SystemImpl {
   std::vector<long> hjoff;
   SysDev d{};
   dbuf arena, ints, joffb, propsb, Jb, Mb, JTb, Cm, Dm, partial, coefb, flags;
+  // block-tridiagonal factors of the Jacobians, used while every filament
+  // has at most 64 segments (the solve keeps a filament's factors in LDS);
+  // the dense inverse JTb (Gauss-Jordan) otherwise
+  dbuf Fb;
+  bool banded() const { return max_nw <= 64; }
   double *E = nullptr, *NewE = nullptr, *Upd = nullptr, *tmp = nullptr,
          *dE = nullptr, *scratch6 = nullptr;
   unsigned long long *hflags = nullptr;
@@ -54,6 +59,7 @@ struct SystemImpl {
   int update(const double *u);
   int build_jacobian(int nt);
   int jsolve(const double *b, double *x, int iter);
+  int jinv_dense(int f, double *out);
   int broyden_update(const double *Eo, const double *En, const double *upd,
                      int iter);
   int residual(int nt, const double *V, const double *W, double *err,
