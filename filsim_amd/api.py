@@ -213,6 +213,12 @@ class System:
         except Exception:  # still held by a batch at interpreter exit
             pass
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
     def _rows(self, k):
         return self.nlam if k.startswith("lam") else self.np
 

@@ -20,6 +20,7 @@
 // like double_array_2D (c_array_functions.cpp:12-21).
 #pragma once
 
+#include <cmath>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -95,6 +96,27 @@ struct Fcm {
         W[i][d] = a2 * T[i][d];
       }
   }
+  // Fcm::assign_filament_data (FCMfunctions.cpp:882-918): Y = fmod(Xs, L),
+  // F, T from the driver's own filament objects. Any container of objects
+  // with length() and beads[i].Xs(d), .F(d), .TAU(d) (the reference's
+  // Filament and Bead) works; it is read in the caller's translation unit,
+  // so this header needs none of the caller's libraries.
+  template <class FilamentVector>
+  void assign_filament_data(const FilamentVector &filaments) {
+    const double L[3] = {params.nx * params.dx, params.ny * params.dx,
+                         params.nz * params.dx};
+    int j = 0;
+    for (const auto &fil : filaments)
+      for (int i = 0; i < (int)fil.length(); ++i, ++j) {
+        if (j >= Np) throw Error("assign_filament_data: more segments than Np");
+        for (int d = 0; d < 3; ++d) {
+          Y[j][d] = std::fmod(fil.beads[i].Xs(d), L[d]);
+          F[j][d] = fil.beads[i].F(d);
+          T[j][d] = fil.beads[i].TAU(d);
+        }
+      }
+    if (j != Np) throw Error("assign_filament_data: fewer segments than Np");
+  }
   void mpi_broadcast() {}
   // Fcm::free_memory (FCMfunctions.cpp:468-519)
   void free_memory() {
@@ -110,6 +132,11 @@ struct LinkedList {
   int Np = 0;
   fs_colbar_params params{};
 
+  LinkedList() = default;
+  // the box, radius and barrier constants the reference takes from
+  // config.hpp, for the filament-vector initialise below
+  explicit LinkedList(const fs_colbar_params &p) : params(p) {}
+
   // LinkedList::initialise + set_global_segment_number
   // (CollisionBarrierFilament.cpp:274-367)
   void initialise(const fs_colbar_params &p, const std::vector<int> &seg_fil,
@@ -119,9 +146,39 @@ struct LinkedList {
     check(fs_colbar_create(&p, Np, seg_fil.data(), seg_idx.data(), &h),
           "fs_colbar_create");
   }
+  // The reference's own call, linked_list.initialise(myrank, totalnodes,
+  // filaments) (main.cpp:251), with the parameters given at construction:
+  // segment g of filament i_fil, index i_seg, in filament order (:345-367).
+  template <class FilamentVector>
+  void initialise(int myrank, int totalnodes, const FilamentVector &filaments) {
+    (void)myrank;
+    (void)totalnodes;
+    std::vector<int> seg_fil, seg_idx;
+    int i_fil = 0;
+    for (const auto &fil : filaments) {
+      for (int i_seg = 0; i_seg < (int)fil.length(); ++i_seg) {
+        seg_fil.push_back(i_fil);
+        seg_idx.push_back(i_seg);
+      }
+      ++i_fil;
+    }
+    initialise(params, seg_fil, seg_idx);
+  }
   // LinkedList::link (CollisionBarrierFilament.cpp:370-433), from the
   // unwrapped positions Xs (Np x 3)
   void link(double **Xs) { check(fs_colbar_link_host(h, &Xs[0][0]), "link"); }
+  // The reference's own call, linked_list.link(filaments) (main.cpp:394):
+  // the beads' Xs in filament order.
+  template <class FilamentVector>
+  void link(const FilamentVector &filaments) {
+    std::vector<double> xs;
+    xs.reserve(3 * (size_t)Np);
+    for (const auto &fil : filaments)
+      for (int i = 0; i < (int)fil.length(); ++i)
+        for (int d = 0; d < 3; ++d) xs.push_back(fil.beads[i].Xs(d));
+    if ((int)xs.size() != 3 * Np) throw Error("link: segment count != Np");
+    check(fs_colbar_link_host(h, xs.data()), "link");
+  }
   void mpi_broadcast() {}
   // LinkedList::apply_collision_barrier (CollisionBarrierFilament.cpp:24-271)
   void apply_collision_barrier(Fcm &fcm, int nt) {

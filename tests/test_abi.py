@@ -93,3 +93,14 @@ def test_make_filaments_deterministic_no_overlap():
     seg = np.linalg.norm(np.diff(X1.reshape(40, 15, 3), axis=1), axis=-1)
     assert np.allclose(seg, 2.2)
     assert np.allclose(np.linalg.norm(q1, axis=1), 1)
+
+
+def test_shim_compiles_reference_call_sites():
+    """include/filsim_shim.hpp with the reference driver's own calls on a
+    filament container (a stand-in for class Filament: beads[i].Xs(d), .F(d),
+    .TAU(d), length()) compiles with the host compiler alone."""
+    import subprocess
+    src = os.path.join(ROOT, "filsim_amd", "host", "driver.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror",
+                        "-fsyntax-only", src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -1,0 +1,149 @@
+"""The benchmark configurations run at their own sizes on the GPU:
+the swimming run at 256^3 / 1,000 x 30 segments in FP32 and FP64 side by
+side (and its final mobility solve against the oracle), the 64-suspension
+batched sweep against stepping the same suspensions one by one, and the
+learned warm start at 240,000 segments."""
+import math
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from filsim_amd import abi, api, params, warmstart
+from helpers import rel
+from sysfix import d2h
+
+pytestmark = pytest.mark.gpu
+
+
+def _swimmers(precision, grid=256, nfil=1000, nw=30, seed=20261202,
+              gpu_place=False):
+    """The benchmark workload: swimming filaments (amplitude 0.5/L,
+    wavenumber 2 pi/L, period 1), KB = 1000, dt = 0.005, TOL 1e-6,
+    barrier on, overlap-free random placement at 2.2a."""
+    fp = params.fcm_params(grid)
+    sp = params.sim_params(fp, dt=0.005, tol=1e-6, maxiter=100,
+                           precision=precision, weight=(0.0, 0.0, 0.0),
+                           barrier_on=1)
+    rng = np.random.default_rng(seed)
+    props = params.swimming_props(nfil, nw, kap=1000.0,
+                                  phases=list(rng.uniform(0, 2 * math.pi,
+                                                          nfil)))
+    X, q = api.make_filaments(params.box(fp), [nw] * nfil, dL=2.2,
+                              min_sep=2.2, seed=seed, gpu=gpu_place)
+    st = dict(X=X, Xs=X, Xt=X, q=q, qs=q, qt=q)
+    return fp, sp, [nw] * nfil, props, st
+
+
+def test_swimming_256_fp32_beside_fp64(cuda):
+    """Three swimming steps at 256^3 / 30,000 segments from one state in
+    FP32 and FP64 FCM: iteration counts within one, positions within 1e-6
+    of the box; the last mobility solve of each path against the FP64
+    oracle on that path's own segment positions, forces and torques (FP64
+    to 1e-12, FP32 to 1e-5)."""
+    out = {}
+    for prec in (abi.FS_FP32, abi.FS_FP64):
+        fp, sp, nws, props, st = _swimmers(prec)
+        s = api.System(sp, nws, props)
+        s.set_state(**st)
+        its = [s.timestep(nt) for nt in (1, 2, 3)]
+        Yp, Fp, Tp, Vp, Wp = s.fcm_ptrs()
+        n3 = 3 * s.np
+        arrs = [d2h(p, n3).reshape(-1, 3) for p in (Yp, Fp, Tp, Vp, Wp)]
+        out[prec] = (its, s.get_state(), arrs, s.step_status())
+        print("precision", prec, "iterations", its, flush=True)
+        s.close()
+    (i32, s32, a32, st32), (i64, s64, a64, st64) = out[abi.FS_FP32], out[abi.FS_FP64]
+    assert all(abs(a - b) <= 1 for a, b in zip(i32, i64))
+    assert st32["converged"] and st64["converged"]
+    L = max(params.box(params.fcm_params(256)))
+    dx = np.abs(s32["X"] - s64["X"]).max() / L
+    print("max |X32 - X64| / box %.3e" % dx, flush=True)
+    assert dx <= 1e-6
+    for prec, (Y, F, T, V, W), tol in ((abi.FS_FP32, a32, 1e-5),
+                                       (abi.FS_FP64, a64, 1e-12)):
+        Vo, Wo = O.mobility(params.fcm_params(256), Y, F, T)
+        ev, ew = rel(V, Vo), rel(W, Wo)
+        print("precision", prec, "final solve vs oracle: U %.3e Omega %.3e"
+              % (ev, ew), flush=True)
+        assert ev <= tol and ew <= tol
+
+
+def test_batched_sweep_64_suspensions(cuda):
+    """The 64 sweep points of the batched benchmark (grids 32^3-128^3,
+    8-200 filaments of 15-40 segments capped by box size, KB log-uniform in
+    [0.1, 10], gravity N(0, 0.2^2), FP32 FCM + FP64 Broyden, TOL 1e-6):
+    two steps as one batch and two steps one suspension after another from
+    the same states give the same iteration counts and the same bits."""
+    pts = [params.sweep_point(4000 + k, precision=abi.FS_FP32, tol=1e-6)
+           for k in range(64)]
+
+    def fresh():
+        g = []
+        for fp, sp, nws, props, st, info in pts:
+            s = api.System(sp, nws, props)
+            s.set_state(**st)
+            g.append(s)
+        return g
+    seq = fresh()
+    its_seq = [[s.timestep(nt) for s in seq] for nt in (1, 2)]
+    bat = fresh()
+    batch = api.Batch(bat)
+    its_bat = [batch.timestep(nt) for nt in (1, 2)]
+    print("segments", sum(p[5]["segments"] for p in pts), "grids",
+          sorted(set(p[5]["grid"] for p in pts)), "mean iterations",
+          np.mean(its_seq), flush=True)
+    assert its_bat == its_seq
+    for a, b in zip(seq, bat):
+        sa, sb = a.get_state(), b.get_state()
+        for k in ("X", "q", "lam"):
+            assert np.array_equal(sa[k], sb[k]), k
+    batch.close()
+    for s in seq + bat:
+        s.close()
+
+
+def test_learned_warm_start_240k_segments(cuda):
+    """8,000 swimmers x 30 segments (240,000) at the 256^3 workload's density
+    (512^3; at 384^3 these filaments fill 11% of the box and the method's
+    per-filament approximate Jacobian no longer makes Broyden converge, on
+    the oracle as on the GPU). Start-up, two recorded steps, a short
+    training on the GPU, then two steps from the network's guess: each
+    converges below broyden_maxiter with a finite state, and where the
+    guess was rejected the step equals the plain path's bit for bit (where
+    it was used, its residual was the lower one)."""
+    torch = cuda
+    fp, sp, nws, props, st = _swimmers(abi.FS_FP32, grid=512, nfil=8000,
+                                       gpu_place=True)
+    s = api.System(sp, nws, props)
+    s.set_state(**st)
+    X, T, its_rec = warmstart.record(s, 2, nt0=1, torch=torch, skip=3)
+    mlp, hist = warmstart.train(X, T, torch, epochs=3, batch=1024, lr=1e-3)
+    print("recorded iterations", its_rec, "holdout", hist[-1][2], flush=True)
+    state = s.get_state()
+    keep = {k: state[k] for k in ("X", "Xt", "U", "Ut", "q", "qt", "lam",
+                                  "lam1", "lam2")}
+    keep.update(Xs=state["X"], Us=state["U"], qs=state["q"])
+    plain = api.System(sp, nws, props)
+    api.system_set_warmstart(s, mlp)
+    for nt in (6, 7):
+        plain.set_state(**{k: v for k, v in s.get_state().items()
+                           if k in keep})
+        it = s.timestep(nt)
+        used, rp, rm = api.system_last_step_info(s)
+        stat = s.step_status()
+        ip = plain.timestep(nt)
+        print("step", nt, "learned", it, "used", used, "res plain %.3e "
+              "mlp %.3e" % (rp, rm), "plain", ip, flush=True)
+        assert 0 < it < sp.broyden_maxiter and stat["converged"]
+        assert not stat["nonfinite"]
+        sl = s.get_state()
+        assert np.isfinite(sl["X"]).all() and np.isfinite(sl["q"]).all()
+        if used:
+            assert rm < rp
+        else:
+            assert it == ip
+            assert np.array_equal(sl["X"], plain.get_state()["X"])
+    s.close()
+    plain.close()
+    mlp.close()

@@ -37,3 +37,35 @@ def test_driver_matches_oracle(cuda, tmp_path):
     so = o.get_state()
     assert np.abs(X - so["X"]).max() <= 1e-8 * np.abs(so["X"]).max()
     assert np.abs(q - so["q"]).max() <= 1e-8
+
+
+def test_reference_call_sites_through_the_shim(cuda, tmp_path):
+    """main.cpp's own calls on its filament container, unchanged:
+    linked_list.initialise(myrank, totalnodes, filaments), fcm.assign_
+    filament_data(filaments), linked_list.link(filaments), then the barrier
+    and the mobility solve, on a stand-in Filament holding the final Xs, F,
+    TAU of a run; against the oracle on the same arrays."""
+    grid, nfil, nw, steps, seed, tol = 48, 6, 10, 2, 5, 1e-8
+    calls = tmp_path / "calls.bin"
+    r = subprocess.run([DRIVER, str(grid), str(nfil), str(nw), str(steps),
+                        "0", str(seed), repr(tol), str(tmp_path / "s.bin"),
+                        str(calls)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    npt = nfil * nw
+    Xs, F, T, Fb, V, W = np.fromfile(calls).reshape(6, npt, 3)
+    fp = params.fcm_params(grid)
+    L = np.array(params.box(fp))
+    Y = np.fmod(Xs, L)
+    cp = params.colbar_params(fp)
+    sf = np.repeat(np.arange(nfil), nw).astype(np.int32)
+    si = np.tile(np.arange(nw), nfil).astype(np.int32)
+    cb = O.Colbar(cp, sf, si)
+    cb.link(Xs)
+    Fo, _ = cb.apply(Y, steps, F)
+    assert np.abs(Fb - Fo).max() <= 1e-12 * max(1.0, np.abs(Fo).max())
+    assert np.abs(Fb - F).max() > 0  # the barrier acted
+    Vo, Wo = O.mobility(fp, Y, Fo, T)
+    assert np.linalg.norm(V - Vo) <= 1e-12 * np.linalg.norm(Vo)
+    assert np.linalg.norm(W - Wo) <= 1e-12 * np.linalg.norm(Wo)

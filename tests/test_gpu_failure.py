@@ -171,11 +171,14 @@ def test_nonfinite_state_flags_singular_jacobian(cuda, nw):
     ref.close()
 
 
-def test_long_filaments_jacobian_and_step_match_oracle(cuda):
-    """34 segments per filament (J is 204 x 204) take the global-memory
-    Gauss-Jordan kernel: J, its inverse and a timestep under the friction
-    mobility against the oracle."""
-    sp, nws, props, st = config(grid=160, nfil=3, nw=34, seed=5, tol=1e-9,
+@pytest.mark.parametrize("grid,nw", [(160, 34), (270, 66)])
+def test_long_filaments_jacobian_and_step_match_oracle(cuda, grid, nw):
+    """Long filaments against the oracle: J, its inverse and timesteps under
+    the friction mobility. 34 segments (J is 204 x 204) use the
+    block-tridiagonal factorisation with a partly filled last block; 66
+    segments (396 x 396) are past its 64-segment limit and take the dense
+    Gauss-Jordan inverse in global memory."""
+    sp, nws, props, st = config(grid=grid, nfil=3, nw=nw, seed=5, tol=1e-9,
                                 ifs=1000, swim=True)
     o, g = pair(sp, nws, props, st)
     nt = 14
