@@ -376,13 +376,16 @@ def c3_bench(args, torch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mob-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--record", type=int, default=100,
                     help="plain steps recorded to train the warm start")
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--refresh", type=int, default=100,
+                    help="learned path: fine-tune on its own last REFRESH "
+                         "steps every REFRESH steps (0: never)")
     ap.add_argument("--c3-steps", type=int, default=2000,
                     help="steps of the mixed-vs-FP64 swimming run (--only c3)")
     ap.add_argument("--skip", type=int, default=10,
@@ -460,7 +463,7 @@ def main():
             (X.shape[0], t_rec, its_rec[-5:]))
         t0 = time.perf_counter()
         mlp, hist = warmstart.train(X, T, torch, epochs=args.epochs,
-                                    batch=1024, lr=1e-3, log=log)
+                                    batch=1024, lr=1e-3, log=log, linear=True)
         t_train = time.perf_counter() - t0
         del X, T
         nt0 = args.skip + args.record + 1
@@ -473,7 +476,14 @@ def main():
                          "record_s": t_rec,
                          "train_s": t_train, "epochs": args.epochs,
                          "holdout_loss_every_10_epochs":
-                             [h[2] for h in hist[::10]] + [hist[-1][2]]}
+                             [h[2] for h in hist[::10]] + [hist[-1][2]],
+                         "holdout_loss_network": hist[-1][2],
+                         "holdout_loss_ridge_alone": mlp.ridge_holdout,
+                         "holdout_loss_ridge_through_network":
+                             mlp.ridge_holdout_net,
+                         "holdout_loss_extrapolation_alone":
+                             mlp.zero_holdout,
+                         "online_refresh_every_steps": args.refresh}
         # 2) the same state, three paths, same TOL
         finals = {}
         for name, prec in (("learned", None), ("mixed", abi.FS_FP32),
@@ -488,15 +498,28 @@ def main():
             for _ in range(args.warmup):
                 s.timestep(nt)
                 nt += 1
+            ref = None
+            if name == "learned" and args.refresh > 0:
+                ref = warmstart.OnlineRefresh(s, mlp, torch,
+                                              window=args.refresh)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(args.steps):
                 its.append(s.timestep(nt))
                 if name == "learned":
                     used += api.system_last_step_info(s)[0]
+                    if ref is not None:
+                        ref.after_step()
                 nt += 1
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
+            if ref is not None:
+                info["online_refreshes"] = ref.refreshes
+                ref.close()
+            w = max(1, min(100, args.steps))
+            info["mean_iters_per_%d_steps_%s" % (w, name)] = [
+                round(float(np.mean(its[i:i + w])), 2)
+                for i in range(0, len(its), w)]
             finals[name] = s.get_state()["X"]
             res[name] = (el, its)
             # one more step, profiled (outside the timed region), in the

@@ -132,6 +132,7 @@ PROTOS = {
     "fs_mlp_destroy": [P],
     "fs_mlp_num_params": [P, IP],
     "fs_mlp_set_norm": [P, FP, FP, FP],
+    "fs_mlp_set_skip": [P, FP],
     "fs_mlp_forward": [P, I, P, P, P],
     "fs_mlp_train_step": [P, I, P, P, C.c_float, FP, P],
     "fs_mlp_gradients": [P, I, P, P, FP, I, FP],

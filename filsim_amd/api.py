@@ -439,6 +439,16 @@ class Mlp:
         abi.check(self.lib.fs_mlp_set_norm(self.h, abi.fptr(mu),
                                            abi.fptr(sigma), abi.fptr(tau)))
 
+    def set_skip(self, S):
+        """Fixed linear term y_n += x_n S^T (S: n_out x n_in); None
+        removes it (fs_mlp_set_skip)."""
+        if S is None:
+            abi.check(self.lib.fs_mlp_set_skip(self.h, None))
+            return
+        S = np.ascontiguousarray(S, dtype=np.float32)
+        assert S.shape == (self.n_out, self.n_in)
+        abi.check(self.lib.fs_mlp_set_skip(self.h, abi.fptr(S)))
+
     def forward(self, X, Y, stream=None):
         abi.check(self.lib.fs_mlp_forward(self.h, X.shape[0], _ptr(X),
                                           _ptr(Y), _stream(stream)))

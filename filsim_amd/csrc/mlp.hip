@@ -8,7 +8,9 @@
 // Numerics (mirrored by the CPU oracle oracle/mlp_oracle.py):
 //   forward   x_n = (x - mu)/sigma rounded to BF16; weights rounded to BF16;
 //             z_l = h_{l-1} W_l^T + b_l in FP32; h_l = BF16(relu(z_l));
-//             y = tau * (h_3 W_3^T + b_3)
+//             y = tau * (h_3 W_3^T + b_3 + x_n S^T) with S an optional
+//             fixed linear map set by the host (a ridge fit), BF16
+//             (fs_mlp_set_skip); the layers then learn the residual
 //   loss      mean over batch and outputs of (y_n - t/tau)^2, y_n the
 //             normalised output
 //   backward  g = BF16(dL/dz) feeds both MFMA products (dW = g^T h,
@@ -36,7 +38,7 @@ __device__ __forceinline__ float bf2f(uint16_t b) {
   return __uint_as_float(((uint32_t)b) << 16);
 }
 
-enum { EPI_STORE = 0, EPI_HIDDEN = 1, EPI_OUT = 2, EPI_DRELU = 3 };
+enum { EPI_STORE = 0, EPI_HIDDEN = 1, EPI_OUT = 2, EPI_DRELU = 3, EPI_ADD = 4 };
 
 // C(m,n) = sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) =
 // B[k*sbk + n*sbn] (BF16). One wavefront per 32x32 output tile (2x2 MFMA
@@ -47,6 +49,7 @@ enum { EPI_STORE = 0, EPI_HIDDEN = 1, EPI_OUT = 2, EPI_DRELU = 3 };
 //            HIDDEN Zf = acc + bias, Hb = BF16(relu(Zf))
 //            OUT    Cf = acc + bias
 //            DRELU  g = acc * (Zf > 0): Cf = g, Hb = BF16(g)
+//            ADD    Cf += acc
 __global__ void __launch_bounds__(256)
     k_gemm_bf16(int M, int N, int K, const uint16_t *__restrict__ A, long sam,
                 long sak, const uint16_t *__restrict__ B, long sbk, long sbn,
@@ -107,6 +110,8 @@ __global__ void __launch_bounds__(256)
           Hb[(long)m * ldh + n] = f2bf(v > 0.f ? v : 0.f);
         } else if (epi == EPI_OUT) {
           Cf[(long)m * ldc + n] = v + bias[n];
+        } else if (epi == EPI_ADD) {
+          Cf[(long)m * ldc + n] += v;
         } else if (epi == EPI_DRELU) {
           float g = Zin[(long)m * ldz + n] > 0.f ? v : 0.f;
           Cf[(long)m * ldc + n] = g;
@@ -209,6 +214,8 @@ struct MlpImpl {
   long nparams = 0;
   dbuf params, grads, adam_m, adam_v, wbf;  // fp32 master + bf16 copies
   dbuf norm;                                 // mu[nin], sigma[nin], tau[nout]
+  dbuf skip;                                 // BF16 S[nout][nin] (optional)
+  bool has_skip = false;
   dbuf act;                                  // activations / scratch
   long act_rows = 0;
   long step = 0;
@@ -278,6 +285,7 @@ struct MlpImpl {
     adam_v.release();
     wbf.release();
     norm.release();
+    skip.release();
     act.release();
   }
   int requantise() {
@@ -289,18 +297,21 @@ struct MlpImpl {
   int ensure(long rows) {
     if (rows <= act_rows) return FS_OK;
     long w = width, mx = std::max<long>(std::max<long>(nin, nout), width);
+    // every piece starts on a 256-byte boundary: the allocation is the sum
+    // of the rounded sizes
+    auto r256 = [](size_t b) { return (b + 255) & ~size_t(255); };
     size_t bytes = 0;
-    bytes += 4 * rows * w * sizeof(float);         // Zf
-    bytes += 4 * rows * mx * sizeof(uint16_t);     // Hb (Hb[0] = x_n)
-    bytes += 3 * rows * mx * sizeof(float);        // Y, G, Gtmp
-    bytes += 2 * rows * mx * sizeof(uint16_t);     // Gb, Gb2
-    bytes += (rows * mx / 256 + 8) * sizeof(float);  // partial + loss
-    bytes += 64 * 16;
+    bytes += 4 * r256(rows * w * sizeof(float));      // Zf
+    bytes += 4 * r256(rows * mx * sizeof(uint16_t));  // Hb (Hb[0] = x_n)
+    bytes += 3 * r256(rows * mx * sizeof(float));     // Y, G, Gtmp
+    bytes += 2 * r256(rows * mx * sizeof(uint16_t));  // Gb, Gb2
+    bytes += r256((rows * mx / 256 + 4) * sizeof(float));  // partial
+    bytes += r256(16);                                      // loss
     FS_TRY(act.alloc(bytes));
     char *p = (char *)act.p;
     auto take = [&](size_t b) {
       char *r = p;
-      p += (b + 255) & ~size_t(255);
+      p += r256(b);
       return r;
     };
     for (int l = 0; l < 4; ++l) Zf[l] = (float *)take(rows * w * sizeof(float));
@@ -345,6 +356,9 @@ struct MlpImpl {
     }
     gemm(B, nout, width, h, width, 1, W + woff[3], 1, width, EPI_OUT,
          P + boff[3], Y, nout, nullptr, 0, nullptr, 0);
+    if (has_skip)  // Y += x_n S^T: B(k=in, n=out) = S[out][in]
+      gemm(B, nout, nin, Hb[0], nin, 1, skip.as<uint16_t>(), 1, nin, EPI_ADD,
+           nullptr, Y, nout, nullptr, 0, nullptr, 0);
     FS_LAUNCHED();
     return FS_OK;
   }
@@ -509,6 +523,36 @@ extern "C" int fs_mlp_set_params(fs_mlp *h, const float *p, int n) {
   FS_TRY(h->impl.requantise());
   FS_HIP(hipDeviceSynchronize());
   return FS_OK;
+}
+
+extern "C" int fs_mlp_set_skip(fs_mlp *h, const float *S) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  MlpImpl &m = h->impl;
+  if (!S) {
+    m.has_skip = false;
+    return FS_OK;
+  }
+  const long n = (long)m.nout * m.nin;
+  fs::dbuf tmp;
+  FS_TRY(tmp.alloc(sizeof(float) * n));
+  FS_TRY(m.skip.alloc(sizeof(uint16_t) * n));
+  int rc = FS_OK;
+  if (hipMemcpy(tmp.p, S, sizeof(float) * n, hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    fs::set_error("copy of the skip map failed");
+    rc = FS_ERR_HIP;
+  }
+  if (rc == FS_OK) {
+    hipLaunchKernelGGL(fs::k_to_bf16, dim3(fs::ceil_div(n, 256)), dim3(256), 0,
+                       nullptr, tmp.as<float>(), n, m.skip.as<uint16_t>());
+    if (hipDeviceSynchronize() != hipSuccess) {
+      fs::set_error("conversion of the skip map failed");
+      rc = FS_ERR_HIP;
+    }
+  }
+  tmp.release();
+  if (rc == FS_OK) m.has_skip = true;
+  return rc;
 }
 
 extern "C" int fs_mlp_set_norm(fs_mlp *h, const float *mu, const float *sigma,

@@ -328,6 +328,11 @@ int fs_mlp_create(int n_in, int width, int n_out, unsigned long long seed,
                   fs_mlp **out);
 int fs_mlp_destroy(fs_mlp *h);
 int fs_mlp_num_params(fs_mlp *h, int *n);
+/* Optional fixed linear term: y_n += x_n S^T with S (host, n_out x n_in
+ * row-major, FP32, stored in BF16), e.g. a ridge regression of the
+ * normalised targets on the normalised features; the layers are then
+ * trained on what it leaves. NULL removes it. */
+int fs_mlp_set_skip(fs_mlp *h, const float *S);
 /* Feature/target normalisation: x_n = (x - mu)/sigma, y = tau * y_n. */
 int fs_mlp_set_norm(fs_mlp *h, const float *mu, const float *sigma,
                     const float *tau);

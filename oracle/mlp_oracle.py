@@ -33,10 +33,16 @@ def unpack(params, dims):
     return out
 
 
-def forward(params, dims, X, mu, sigma, tau, rounding=True):
+def forward(params, dims, X, mu, sigma, tau, rounding=True, skip=None):
+    """Normalised output y_n (and the activations); skip: the optional fixed
+    linear map S (n_out x n_in), y_n += x_n S^T."""
     r = bf16 if rounding else (lambda a: np.asarray(a, dtype=np.float64))
     layers = unpack(params, dims)
-    h = r((np.asarray(X, np.float64) - mu) / sigma)
+    if rounding:  # the GPU normalises in FP32, then rounds to BF16
+        h = bf16((np.asarray(X, np.float32) - np.asarray(mu, np.float32)) /
+                 np.asarray(sigma, np.float32))
+    else:
+        h = (np.asarray(X, np.float64) - mu) / sigma
     zs, hs = [], [h]
     for l in range(3):
         W, b = layers[l]
@@ -46,13 +52,18 @@ def forward(params, dims, X, mu, sigma, tau, rounding=True):
         hs.append(h)
     W, b = layers[3]
     yn = h @ r(W).T + b
+    if skip is not None:
+        yn = yn + hs[0] @ r(skip).T
     return yn, zs, hs, layers
 
 
-def loss_and_grad(params, dims, X, T, mu, sigma, tau, rounding=True):
-    """Mean squared normalised error and its gradient (packed like params)."""
+def loss_and_grad(params, dims, X, T, mu, sigma, tau, rounding=True,
+                  skip=None):
+    """Mean squared normalised error and its gradient (packed like params);
+    the fixed skip map has no gradient."""
     r = bf16 if rounding else (lambda a: np.asarray(a, dtype=np.float64))
-    yn, zs, hs, layers = forward(params, dims, X, mu, sigma, tau, rounding)
+    yn, zs, hs, layers = forward(params, dims, X, mu, sigma, tau, rounding,
+                                 skip)
     tn = np.asarray(T, np.float64) / tau
     d = yn - tn
     n = d.size

@@ -42,7 +42,19 @@ def test_forward_matches_oracle(cuda):
     Yg = Y.cpu().numpy()
     err = np.linalg.norm(Yg - Yo) / np.linalg.norm(Yo)
     print("forward rel err", err)
-    assert err < 1e-2
+    # same BF16 rounding points; the rest is FP32 vs FP64 accumulation and
+    # the rare hidden value that rounds to the other BF16 neighbour
+    assert err < 2e-3
+    # with the fixed linear term y_n += x_n S^T
+    S = np.random.default_rng(5).standard_normal((180, X.shape[1])) * 0.05
+    mlp.set_skip(S)
+    mlp.forward(dX, Y)
+    yn2, _, _, _ = M.forward(p, mlp.dims, X, mu, sig, tau, skip=S)
+    Yo2 = yn2 * tau
+    err2 = np.linalg.norm(Y.cpu().numpy() - Yo2) / np.linalg.norm(Yo2)
+    print("forward with linear term rel err", err2)
+    assert err2 < 2e-3
+    assert np.linalg.norm(Yo2 - Yo) > 0.1 * np.linalg.norm(Yo)
 
 
 def test_gradients_match_oracle(cuda):
@@ -54,14 +66,14 @@ def test_gradients_match_oracle(cuda):
     err = np.linalg.norm(g - go) / np.linalg.norm(go)
     cos = g @ go / np.linalg.norm(g) / np.linalg.norm(go)
     print("loss", loss, lo, "grad rel err", err, "cos", cos)
-    assert abs(loss - lo) < 1e-3 * lo
-    assert err < 3e-2 and cos > 0.999
+    assert abs(loss - lo) < 1e-5 * lo
+    assert err < 1e-4 and cos > 0.99999
     # per-layer: every weight block agrees
     o = 0
     for l in range(4):
         n = mlp.dims[l] * mlp.dims[l + 1] + mlp.dims[l + 1]
         e = np.linalg.norm(g[o:o + n] - go[o:o + n]) / np.linalg.norm(go[o:o + n])
-        assert e < 5e-2, (l, e)
+        assert e < 1e-3, (l, e)
         o += n
 
 
@@ -110,30 +122,35 @@ def test_fallback_is_exact_plain_path(cuda):
 
 
 def test_trained_warmstart_gain(cuda):
-    """Train on the plain solver's own trajectory, then compare Broyden
-    iterations per step on the continuation with and without the network
-    (same states, same tolerance)."""
+    """Train on the plain solver's own trajectory (the network with its ridge
+    linear term), then continue from the recorded state with and without
+    it for 30 steps (same states, same tolerance). The recording system
+    itself continues with the network, so its velocity history is the
+    trajectory's own. The network must save at least 2 Broyden iterations
+    per step on average (measured: about 5 of 11), and its holdout loss
+    must not exceed the ridge fit it starts from."""
     torch = cuda
     sp, nws, props, st = _swimmers()
     rec = api.System(sp, nws, props)
     rec.set_state(**st)
-    X, T, its_rec = warmstart.record(rec, 60, torch=torch)
-    mlp, hist = warmstart.train(X, T, torch, epochs=30, batch=128, lr=1e-3)
+    X, T, its_rec = warmstart.record(rec, 60, torch=torch, skip=5)
+    mlp, hist = warmstart.train(X, T, torch, epochs=40, batch=128, lr=1e-3,
+                                linear=True)
     state = rec.get_state()
     keep = {k: state[k] for k in ("X", "Xt", "U", "Ut", "q", "qt", "lam",
                                   "lam1", "lam2")}
     keep.update(Xs=state["X"], Us=state["U"], qs=state["q"])
     base = api.System(sp, nws, props)
     base.set_state(**keep)
-    learn = api.System(sp, nws, props)
-    learn.set_state(**keep)
-    api.system_set_warmstart(learn, mlp)
-    # warm the velocity history of both with the same two plain steps
+    api.system_set_warmstart(rec, mlp)
     ib, il, used = [], [], 0
-    for nt in range(61, 71):
+    for nt in range(66, 96):
         ib.append(base.timestep(nt))
-        il.append(learn.timestep(nt))
-        used += api.system_last_step_info(learn)[0]
+        il.append(rec.timestep(nt))
+        used += api.system_last_step_info(rec)[0]
     print("plain iters", ib, "learned", il, "mlp used", used,
-          "train losses", [h[2] for h in hist[::10]])
-    assert np.mean(il) <= np.mean(ib) + 0.5
+          "holdout network %.3f ridge %.3f extrapolation %.3f" %
+          (hist[-1][2], mlp.ridge_holdout, mlp.zero_holdout))
+    assert hist[-1][2] <= mlp.ridge_holdout_net
+    assert mlp.ridge_holdout < 0.5 * mlp.zero_holdout
+    assert np.mean(il) <= np.mean(ib) - 2.0
