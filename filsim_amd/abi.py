@@ -166,6 +166,20 @@ PROTOS = {
     "fs_system_read_bak": [P, C.c_char_p, IP],
     "fs_system_write_checkpoint": [P, C.c_char_p, C.c_int],
     "fs_system_read_checkpoint": [P, C.c_char_p, IP],
+    "fs_system_barrier_force": [P, I, P],
+    "fs_system_copy_filaments": [P, P, I],
+    "fs_system_com": [P, I, _DP],
+    "fs_system_mean_velocity": [P, I, _DP],
+    "fs_io_write_swimmer_velocity_names": [C.c_char_p, I],
+    "fs_io_write_swimmer_collision_names": [C.c_char_p, I],
+    "fs_io_write_force_names": [C.c_char_p, I],
+    "fs_io_write_swimmer_velocity": [C.c_char_p, I, I, _DP, _DP, _DP, _DP,
+                                     _DP],
+    "fs_io_write_swimmer_collision": [C.c_char_p, I, I, _DP],
+    "fs_io_write_forces": [C.c_char_p, I, I, IP, _DP, _DP],
+    "fs_system_write_swimmer_velocity": [P, C.c_char_p, I],
+    "fs_system_write_swimmer_collision": [P, C.c_char_p, I],
+    "fs_system_write_forces": [P, C.c_char_p, I],
     "fs_batch_create": [C.POINTER(P), C.c_int, C.POINTER(P)],
     "fs_batch_destroy": [P],
     "fs_batch_size": [P, IP],

@@ -323,6 +323,42 @@ class System:
                                              C.byref(md)))
         return md.value
 
+    def barrier_force(self, nt):
+        """linked_list.F_total of the last step (fs_system_barrier_force):
+        the barrier force alone, (np, 3) on the host."""
+        import torch
+        out = torch.zeros(3 * self.np, dtype=torch.float64, device="cuda")
+        abi.check(self.lib.fs_system_barrier_force(self.h, nt, _ptr(out)))
+        return out.cpu().numpy().reshape(-1, 3)
+
+    def copy_filaments(self, src, first=0):
+        """This system's filaments take the state of src's filaments first,
+        first + 1, ... (fs_system_copy_filaments)."""
+        abi.check(self.lib.fs_system_copy_filaments(self.h, src.h, first))
+
+    def com(self, fil=0):
+        """Filament::getCOM of filament fil."""
+        c = np.zeros(3)
+        abi.check(self.lib.fs_system_com(self.h, fil, abi.dptr(c)))
+        return c
+
+    def mean_velocity(self, fil=0):
+        """Mean V over filament fil's segments, accumulated in float."""
+        v = np.zeros(3)
+        abi.check(self.lib.fs_system_mean_velocity(self.h, fil, abi.dptr(v)))
+        return v
+
+    def write_swimmer_velocity(self, path, nt):
+        abi.check(self.lib.fs_system_write_swimmer_velocity(
+            self.h, path.encode(), nt))
+
+    def write_swimmer_collision(self, path, nt):
+        abi.check(self.lib.fs_system_write_swimmer_collision(
+            self.h, path.encode(), nt))
+
+    def write_forces(self, path, nt):
+        abi.check(self.lib.fs_system_write_forces(self.h, path.encode(), nt))
+
     def autotune(self, reps=3):
         """Autotune the system's FCM launch shapes (fs_system_autotune)."""
         t = abi.FcmTuning()
@@ -563,6 +599,43 @@ def io_write_dat(path, nt, nws, X, q):
     X, q = _f64(X), _f64(q)
     abi.check(lib.fs_io_write_dat(path.encode(), nt, len(nw), abi.iptr(nw),
                                   abi.dptr(X), abi.dptr(q)))
+
+
+def io_write_names(path, kind, nw0):
+    """Column names of the swimmer velocity ("velocity"), swimmer
+    collision ("collision") or force ("forces") file."""
+    lib = abi.load()
+    fn = {"velocity": lib.fs_io_write_swimmer_velocity_names,
+          "collision": lib.fs_io_write_swimmer_collision_names,
+          "forces": lib.fs_io_write_force_names}[kind]
+    abi.check(fn(path.encode(), nw0))
+
+
+def _opt(a):
+    return None if a is None else abi.dptr(a)
+
+
+def io_write_swimmer_velocity(path, nt, nw0, Xs, qs, V=None, W=None, F=None):
+    lib = abi.load()
+    arrs = [_f64(a) if a is not None else None for a in (Xs, qs, V, W, F)]
+    abi.check(lib.fs_io_write_swimmer_velocity(path.encode(), nt, nw0,
+                                               *[_opt(a) for a in arrs]))
+
+
+def io_write_swimmer_collision(path, nt, nw0, F_total=None):
+    lib = abi.load()
+    a = _f64(F_total) if F_total is not None else None
+    abi.check(lib.fs_io_write_swimmer_collision(path.encode(), nt, nw0,
+                                                _opt(a)))
+
+
+def io_write_forces(path, nt, nws, F=None, T=None):
+    lib = abi.load()
+    nw = np.ascontiguousarray(nws, dtype=np.int32)
+    F = _f64(F) if F is not None else None
+    T = _f64(T) if T is not None else None
+    abi.check(lib.fs_io_write_forces(path.encode(), nt, len(nw),
+                                     abi.iptr(nw), _opt(F), _opt(T)))
 
 
 def io_write_par(path, nws, sp, kap=1.0, c=1.0, time_steps=0, plot_steps=1,

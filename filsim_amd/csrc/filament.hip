@@ -1853,6 +1853,31 @@ int SystemImpl::barrier(int nt, int link, double *min_dist2) {
   return cb.apply(d.Ya, nt, d.Fa, min_dist2, st);
 }
 
+// linked_list.F_total (CollisionBarrierFilament.cpp:681-686): the barrier
+// force alone, which the step added to F. Recomputed into out (3 Np, device)
+// from the current Y with the cell list linked at the step's start, each
+// segment's pair sum added to zero as the step added it to the other forces;
+// zero when the barrier is off.
+int SystemImpl::barrier_force(int nt, double *out) {
+  FS_CHECK(out || np == 0, FS_ERR_ARG, "null output");
+  const size_t n3 = 3 * (size_t)np;
+  if (n3 == 0) return FS_OK;
+  hipLaunchKernelGGL(k_fill_f64, dim3(ceil_div((long)n3, 256)), dim3(256), 0,
+                     st, out, n3, 0.0);
+  FS_LAUNCHED();
+  if (p.barrier_on && !barrier_off && cb.linked) {
+    // a batch's activity flag may hold 0 for a converged system: apply
+    // unconditionally here
+    const int *act = cb.act;
+    cb.act = nullptr;
+    int rc = cb.apply(d.Ya, nt, out, nullptr, st);
+    cb.act = act;
+    FS_TRY(rc);
+  }
+  FS_HIP(hipStreamSynchronize(st));
+  return FS_OK;
+}
+
 // main.cpp:284-754, with the two Jacobian solves of each Broyden iteration
 // (main.cpp:518, 650) fused into one (k_broyden_fused).
 int SystemImpl::timestep(int nt, int *iters) {

@@ -17,11 +17,15 @@
 //         count (3rd value of line 2) and the bead numbers (line 5)
 // The binary checkpoint (.fsck) stores every state array bit for bit, which
 // the 16-decimal text cannot.
+// The swimmer files (velocity, collision) and the force file follow
+// save_swimmer_velocity_data_to_file / save_swimmer_collision_data_to_file /
+// save_force_data_to_file and their column-name writers.
 #include <cerrno>
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -74,6 +78,122 @@ extern "C" int fs_io_write_dat(const char *path, int nt, int nf, const int *nw,
       for (int d = 0; d < 3; ++d) put(o.f, X[3 * b + d]);
       for (int c = 0; c < 4; ++c) put(o.f, q[4 * b + c]);
     }
+  std::fprintf(o.f, "\n");
+  FS_CHECK(!std::ferror(o.f), FS_ERR_ARG, "write to %s failed", path);
+  return FS_OK;
+}
+
+// ---- swimmer and force output (file_functions.cpp:102-145, 164-208;
+// Filament.hpp:509-650). The column names go to a new file; each data call
+// appends "nt " and one row. At nt = 0 the reference writes the integer 0
+// (no decimals) in place of every velocity and force.
+namespace {
+
+int write_names(const char *path, int nw0,
+                std::initializer_list<std::initializer_list<const char *>> g) {
+  FS_CHECK(path, FS_ERR_ARG, "null path");
+  FS_CHECK(nw0 >= 1, FS_ERR_ARG, "bad segment count %d", nw0);
+  File o(path, "w");
+  FS_CHECK(o.f, FS_ERR_ARG, "cannot open %s", path);
+  std::fprintf(o.f, "it ");
+  for (const auto &names : g)
+    for (int i = 0; i < nw0; ++i)
+      for (const char *n : names) std::fprintf(o.f, "%s ", n);
+  std::fprintf(o.f, "\n");
+  FS_CHECK(!std::ferror(o.f), FS_ERR_ARG, "write to %s failed", path);
+  return FS_OK;
+}
+
+void zeros(FILE *f, size_t n) {
+  for (size_t k = 0; k < n; ++k) std::fprintf(f, "0 ");
+}
+
+}  // namespace
+
+extern "C" int fs_io_write_swimmer_velocity_names(const char *path, int nw0) {
+  return write_names(path, nw0,
+                     {{"X", "Y", "Z", "qr", "qi", "qj", "qk"},
+                      {"Vx", "Vy", "Vz", "Wx", "Wy", "Wz", "Fx", "Fy", "Fz"}});
+}
+
+extern "C" int fs_io_write_swimmer_collision_names(const char *path,
+                                                   int nw0) {
+  return write_names(path, nw0, {{"Fx", "Fy", "Fz"}});
+}
+
+extern "C" int fs_io_write_force_names(const char *path, int nw0) {
+  return write_names(path, nw0, {{"Fx", "Fy", "Fz", "Tx", "Ty", "Tz"}});
+}
+
+extern "C" int fs_io_write_swimmer_velocity(const char *path, int nt, int nw0,
+                                            const double *Xs, const double *qs,
+                                            const double *V, const double *W,
+                                            const double *F) {
+  FS_CHECK(path && Xs && qs, FS_ERR_ARG, "null argument");
+  FS_CHECK(nt == 0 || (V && W && F), FS_ERR_ARG, "null V, W or F");
+  FS_CHECK(nw0 >= 1, FS_ERR_ARG, "bad segment count %d", nw0);
+  File o(path, "a");
+  FS_CHECK(o.f, FS_ERR_ARG, "cannot open %s", path);
+  std::fprintf(o.f, "%d ", nt);
+  for (int i = 0; i < nw0; ++i) {
+    for (int d = 0; d < 3; ++d) put(o.f, Xs[3 * i + d]);
+    for (int c = 0; c < 4; ++c) put(o.f, qs[4 * i + c]);
+  }
+  if (nt != 0) {
+    for (int i = 0; i < nw0; ++i)
+      for (const double *a : {V, W, F})
+        for (int d = 0; d < 3; ++d) put(o.f, a[3 * i + d]);
+  } else {
+    zeros(o.f, 9 * (size_t)nw0);
+  }
+  std::fprintf(o.f, "\n");
+  FS_CHECK(!std::ferror(o.f), FS_ERR_ARG, "write to %s failed", path);
+  return FS_OK;
+}
+
+extern "C" int fs_io_write_swimmer_collision(const char *path, int nt, int nw0,
+                                             const double *F_total) {
+  FS_CHECK(path && (nt == 0 || F_total), FS_ERR_ARG, "null argument");
+  FS_CHECK(nw0 >= 1, FS_ERR_ARG, "bad segment count %d", nw0);
+  File o(path, "a");
+  FS_CHECK(o.f, FS_ERR_ARG, "cannot open %s", path);
+  std::fprintf(o.f, "%d ", nt);
+  if (nt != 0) {
+    for (int i = 0; i < 3 * nw0; ++i) put(o.f, F_total[i]);
+  } else {
+    zeros(o.f, 3 * (size_t)nw0);
+  }
+  std::fprintf(o.f, "\n");
+  FS_CHECK(!std::ferror(o.f), FS_ERR_ARG, "write to %s failed", path);
+  return FS_OK;
+}
+
+// Filament::writeForceData reads rows forceLengthFilled / 6 + i, where
+// forceLengthFilled is the filament's first segment (Filament.cpp:3-5,
+// filament_initialisation_functions.cpp:52): filament 0 gets its own rows,
+// filament f > 0 the nw[f] rows from (its first segment) / 6 on, as the
+// reference writes them.
+extern "C" int fs_io_write_forces(const char *path, int nt, int nf,
+                                  const int *nw, const double *F,
+                                  const double *T) {
+  FS_CHECK(path && (nt == 0 || (F && T)), FS_ERR_ARG, "null argument");
+  FS_TRY(check_layout(nf, nw));
+  File o(path, "a");
+  FS_CHECK(o.f, FS_ERR_ARG, "cannot open %s", path);
+  std::fprintf(o.f, "%d ", nt);
+  size_t off = 0;
+  for (int f = 0; f < nf; ++f) {
+    if (nt != 0) {
+      const size_t r0 = off / 6;
+      for (int i = 0; i < nw[f]; ++i) {
+        for (int d = 0; d < 3; ++d) put(o.f, F[3 * (r0 + i) + d]);
+        for (int d = 0; d < 3; ++d) put(o.f, T[3 * (r0 + i) + d]);
+      }
+    } else {
+      zeros(o.f, 6 * (size_t)nw[f]);
+    }
+    off += nw[f];
+  }
   std::fprintf(o.f, "\n");
   FS_CHECK(!std::ferror(o.f), FS_ERR_ARG, "write to %s failed", path);
   return FS_OK;

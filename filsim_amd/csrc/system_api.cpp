@@ -121,6 +121,88 @@ extern "C" int fs_system_fcm_arrays(fs_system *h, double **Y, double **F,
   return FS_OK;
 }
 
+extern "C" int fs_system_barrier_force(fs_system *h, int nt, double *F_total) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  return h->impl.barrier_force(nt, F_total);
+}
+
+// filaments_copy = filaments; filaments.resize(1) (main_gaitresetandremove.
+// cpp:331-334), as a device-to-device copy of every state array of source
+// filaments first .. first + dst's count - 1.
+extern "C" int fs_system_copy_filaments(fs_system *dst, fs_system *src,
+                                        int first) {
+  FS_CHECK(dst && src, FS_ERR_ARG, "null handle");
+  FS_CHECK(dst != src, FS_ERR_ARG, "source and destination are one system");
+  SystemImpl &D = dst->impl, &S = src->impl;
+  FS_CHECK(first >= 0 && first + D.nf <= S.nf, FS_ERR_ARG,
+           "filaments %d..%d are outside the source's %d", first,
+           first + D.nf - 1, S.nf);
+  for (int f = 0; f < D.nf; ++f)
+    FS_CHECK(D.hnw[f] == S.hnw[first + f], FS_ERR_ARG,
+             "filament %d has %d segments, source filament %d has %d", f,
+             D.hnw[f], first + f, S.hnw[first + f]);
+  FS_HIP(hipStreamSynchronize(S.st));
+  FS_HIP(hipStreamSynchronize(D.st));
+  const size_t b = S.hboff[first], l = S.hloff[first];
+  const size_t n3 = 3 * (size_t)D.np, n4 = 4 * (size_t)D.np,
+               l3 = 3 * (size_t)D.nlam;
+  auto &d = D.d;
+  auto &s = S.d;
+  auto K = hipMemcpyDeviceToDevice;
+  FS_TRY(copy_field(d.X, s.X + 3 * b, n3, K));
+  FS_TRY(copy_field(d.Xs, s.Xs + 3 * b, n3, K));
+  FS_TRY(copy_field(d.Xt, s.Xt + 3 * b, n3, K));
+  FS_TRY(copy_field(d.U, s.U + 3 * b, n3, K));
+  FS_TRY(copy_field(d.Us, s.Us + 3 * b, n3, K));
+  FS_TRY(copy_field(d.Ut, s.Ut + 3 * b, n3, K));
+  FS_TRY(copy_field(d.q, s.q + 4 * b, n4, K));
+  FS_TRY(copy_field(d.qs, s.qs + 4 * b, n4, K));
+  FS_TRY(copy_field(d.qt, s.qt + 4 * b, n4, K));
+  FS_TRY(copy_field(d.lam, s.lam + 3 * l, l3, K));
+  FS_TRY(copy_field(d.lam1, s.lam1 + 3 * l, l3, K));
+  FS_TRY(copy_field(d.lam2, s.lam2 + 3 * l, l3, K));
+  FS_TRY(copy_field(d.F, s.F + 3 * b, n3, K));
+  FS_TRY(copy_field(d.TAU, s.TAU + 3 * b, n3, K));
+  return FS_OK;
+}
+
+// Filament::getCOM (Filament.hpp:248-255): the mean of the filament's bead
+// positions X, summed in bead order and divided by the bead count.
+extern "C" int fs_system_com(fs_system *h, int fil, double com[3]) {
+  FS_CHECK(h && com, FS_ERR_ARG, "null argument");
+  SystemImpl &S = h->impl;
+  FS_CHECK(fil >= 0 && fil < S.nf, FS_ERR_ARG, "no filament %d", fil);
+  const int nw = S.hnw[fil];
+  std::vector<double> x(3 * (size_t)nw);
+  FS_HIP(hipStreamSynchronize(S.st));
+  FS_TRY(copy_field(x.data(), S.d.X + 3 * (size_t)S.hboff[fil], x.size(),
+                    hipMemcpyDeviceToHost));
+  double c[3] = {0, 0, 0};
+  for (int n = 0; n < nw; ++n)
+    for (int k = 0; k < 3; ++k) c[k] += x[3 * n + k];
+  for (int k = 0; k < 3; ++k) com[k] = c[k] / nw;
+  return FS_OK;
+}
+
+// The swimmer's mean segment velocity of main_gaitresetandremove.cpp:781-
+// 787: float v += V[s] / n over the filament's n segments of the last
+// mobility solve (each term in double, the sum rounded to float).
+extern "C" int fs_system_mean_velocity(fs_system *h, int fil, double v[3]) {
+  FS_CHECK(h && v, FS_ERR_ARG, "null argument");
+  SystemImpl &S = h->impl;
+  FS_CHECK(fil >= 0 && fil < S.nf, FS_ERR_ARG, "no filament %d", fil);
+  const int nw = S.hnw[fil];
+  std::vector<double> x(3 * (size_t)nw);
+  FS_HIP(hipStreamSynchronize(S.st));
+  FS_TRY(copy_field(x.data(), S.d.V + 3 * (size_t)S.hboff[fil], x.size(),
+                    hipMemcpyDeviceToHost));
+  float a[3] = {0.f, 0.f, 0.f};
+  for (int n = 0; n < nw; ++n)
+    for (int k = 0; k < 3; ++k) a[k] = (float)(a[k] + x[3 * n + k] / nw);
+  for (int k = 0; k < 3; ++k) v[k] = a[k];
+  return FS_OK;
+}
+
 namespace {
 
 // One timestep (commit = 1) or gait-reset trial pass (commit = 0), with the
@@ -247,6 +329,49 @@ extern "C" int fs_system_write_dat(fs_system *h, const char *path, int nt) {
   FS_TRY(fs_system_get_state(h, &hs.p));
   return fs_io_write_dat(path, nt, h->impl.nf, h->impl.hnw.data(), hs.p.X,
                          hs.p.q);
+}
+
+// The swimmer (filament 0) and force rows of the system's last step or
+// trial pass: Xs, qs of the iterate, V, W and F of the last mobility solve.
+extern "C" int fs_system_write_swimmer_velocity(fs_system *h, const char *path,
+                                                int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  SystemImpl &S = h->impl;
+  const size_t n = S.hnw[0];
+  std::vector<double> xs(3 * n), qs(4 * n), v(3 * n), w(3 * n), f(3 * n);
+  FS_HIP(hipStreamSynchronize(S.st));
+  auto K = hipMemcpyDeviceToHost;
+  FS_TRY(copy_field(xs.data(), S.d.Xs, 3 * n, K));
+  FS_TRY(copy_field(qs.data(), S.d.qs, 4 * n, K));
+  FS_TRY(copy_field(v.data(), S.d.V, 3 * n, K));
+  FS_TRY(copy_field(w.data(), S.d.W, 3 * n, K));
+  FS_TRY(copy_field(f.data(), S.d.Fa, 3 * n, K));
+  return fs_io_write_swimmer_velocity(path, nt, (int)n, xs.data(), qs.data(),
+                                      v.data(), w.data(), f.data());
+}
+
+extern "C" int fs_system_write_swimmer_collision(fs_system *h,
+                                                 const char *path, int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  SystemImpl &S = h->impl;
+  fs::dbuf d;
+  FS_TRY(d.alloc(sizeof(double) * 3 * (size_t)S.np));
+  FS_TRY(S.barrier_force(nt, d.as<double>()));
+  std::vector<double> ft(3 * (size_t)S.hnw[0]);
+  FS_TRY(copy_field(ft.data(), d.as<double>(), ft.size(),
+                    hipMemcpyDeviceToHost));
+  return fs_io_write_swimmer_collision(path, nt, S.hnw[0], ft.data());
+}
+
+extern "C" int fs_system_write_forces(fs_system *h, const char *path, int nt) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  SystemImpl &S = h->impl;
+  const size_t n3 = 3 * (size_t)S.np;
+  std::vector<double> f(n3), t(n3);
+  FS_HIP(hipStreamSynchronize(S.st));
+  FS_TRY(copy_field(f.data(), S.d.Fa, n3, hipMemcpyDeviceToHost));
+  FS_TRY(copy_field(t.data(), S.d.Ta, n3, hipMemcpyDeviceToHost));
+  return fs_io_write_forces(path, nt, S.nf, S.hnw.data(), f.data(), t.data());
 }
 
 extern "C" int fs_system_write_bak(fs_system *h, const char *path, int nt) {

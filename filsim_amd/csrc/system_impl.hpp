@@ -67,6 +67,7 @@ struct SystemImpl {
   int step(int nt);
   int mobility(int nt);
   int barrier(int nt, int link, double *min_dist2);
+  int barrier_force(int nt, double *out);
   // barrier of the gait-reset passes: skipped while barrier_off is set
   bool barrier_off = false;
   int set_barrier(int on, int mask);

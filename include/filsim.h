@@ -230,6 +230,26 @@ int fs_system_get_state(fs_system *h, fs_state_ptrs *s);
 /* Device pointers of the FCM arrays the system drives (np x 3 each). */
 int fs_system_fcm_arrays(fs_system *h, double **Y, double **F, double **T,
                          double **V, double **W);
+/* linked_list.F_total (CollisionBarrierFilament.hpp:32, filled at
+ * CollisionBarrierFilament.cpp:681-686): the collision-barrier force alone,
+ * which the last step added to F, into F_total (device, np x 3): the barrier
+ * recomputed on the current Y with the cell list linked at the step's start
+ * (each segment's pair sum is the one the step added). Zero when the
+ * barrier is off. */
+int fs_system_barrier_force(fs_system *h, int nt, double *F_total);
+/* filaments_copy = filaments; filaments.resize(1) of the gait-reset-and-
+ * remove driver (main_gaitresetandremove.cpp:331-337): copies every state
+ * array of src's filaments first .. first + (dst's filament count) - 1 into
+ * dst, device to device. Segment counts must match; the two systems should
+ * share their parameters (dst is typically a one-filament system of src's
+ * filament 0 with the same FCM grid and box). */
+int fs_system_copy_filaments(fs_system *dst, fs_system *src, int first);
+/* Filament::getCOM (Filament.hpp:248-255) of filament fil: the mean of its
+ * bead positions X (main_gaitreset.cpp:760-772 prints its change over dt). */
+int fs_system_com(fs_system *h, int fil, double com[3]);
+/* The swimmer speed printout of main_gaitresetandremove.cpp:781-787: the
+ * mean of V over filament fil's segments, accumulated in float. */
+int fs_system_mean_velocity(fs_system *h, int fil, double v[3]);
 
 /* One whole implicit timestep nt (main.cpp:284-754) on the device: initial
  * guess, forces, Jacobian, link, barrier, mobility, Broyden loop, step.
@@ -429,6 +449,35 @@ int fs_system_write_bak(fs_system *h, const char *path, int nt);
 int fs_system_read_bak(fs_system *h, const char *path, int *nt);
 int fs_system_write_checkpoint(fs_system *h, const char *path, int nt);
 int fs_system_read_checkpoint(fs_system *h, const char *path, int *nt);
+
+/* ---- Swimmer and force output (SaveExtendedSwimmerData /
+ * SaveExtendedForcesData, the gait-reset files; file_functions.cpp:102-145,
+ * 164-208, Filament.hpp:509-650), in the text layout above:
+ *   swimmer velocity  "nt " + filament 0 per bead Xs(3) qs(4), then per
+ *                     segment V(3) W(3) F(3)
+ *   swimmer collision "nt " + filament 0 per segment F_total(3)
+ *   forces            "nt " + per filament per segment F(3) T(3), filament
+ *                     f reading rows (first segment of f) / 6 + i as
+ *                     Filament::writeForceData does (filament 0: its own)
+ * At nt = 0 every velocity and force is written as the integer 0, and the
+ * arrays may be NULL. The name writers start a new file with "it " and the
+ * column names of filament 0 (nw0 segments). Host arrays; the fs_system_
+ * versions write the system's last step or trial pass (F_total from
+ * fs_system_barrier_force). */
+int fs_io_write_swimmer_velocity_names(const char *path, int nw0);
+int fs_io_write_swimmer_collision_names(const char *path, int nw0);
+int fs_io_write_force_names(const char *path, int nw0);
+int fs_io_write_swimmer_velocity(const char *path, int nt, int nw0,
+                                 const double *Xs, const double *qs,
+                                 const double *V, const double *W,
+                                 const double *F);
+int fs_io_write_swimmer_collision(const char *path, int nt, int nw0,
+                                  const double *F_total);
+int fs_io_write_forces(const char *path, int nt, int nf, const int *nw,
+                       const double *F, const double *T);
+int fs_system_write_swimmer_velocity(fs_system *h, const char *path, int nt);
+int fs_system_write_swimmer_collision(fs_system *h, const char *path, int nt);
+int fs_system_write_forces(fs_system *h, const char *path, int nt);
 
 /* ---- Batched sweep (no reference counterpart: the reference runs one
  * process per sweep point, batch_run.py:60-229). Independent systems of any

@@ -10,6 +10,8 @@
 //                        Filament.hpp), batched over every filament
 //   filsim::ErrorCheckFCM (MobilityConstraints.cpp:447)
 //   filsim::spring_links struct spring_links (spring_link_functions.hpp:18-39)
+//   filsim::save_*_to_file  the trajectory, backup, swimmer and force
+//                        writers of file_functions.cpp
 //
 // Differences from the reference, all forced by running on one GPU:
 //   - sizes are runtime parameters (fs_fcm_params / fs_sim_params) instead
@@ -208,12 +210,14 @@ struct LinkedList {
 struct Filaments {
   fs_system *h = nullptr;
   int nfil = 0, Np = 0, Nlam = 0;
+  std::vector<int> nw;  // segments per filament
 
-  void initialise(const fs_sim_params &p, const std::vector<int> &nw,
+  void initialise(const fs_sim_params &p, const std::vector<int> &nw_,
                   const std::vector<fs_fil_props> &props) {
-    check(fs_system_create(&p, (int)nw.size(), nw.data(), props.data(), &h),
+    check(fs_system_create(&p, (int)nw_.size(), nw_.data(), props.data(), &h),
           "fs_system_create");
     check(fs_system_sizes(h, &nfil, &Np, &Nlam), "fs_system_sizes");
+    nw = nw_;
   }
   void set_state(const fs_state_ptrs &s) {
     check(fs_system_set_state(h, &s), "fs_system_set_state");
@@ -273,11 +277,94 @@ struct Filaments {
       check(fs_system_timestep(h, nt, &it), "timestep");
     return it;
   }
+  // main_gaitresetandremove.cpp:328-339, 458-480, 775-820: pass
+  // gait_reset_i of timestep nt. 0: every filament, no barrier, trial.
+  // 1: filament 0 alone, no barrier, trial, on `swimmer` (a one-filament
+  // system of filament 0's segments and properties on the same grid, made
+  // with initialise) after it takes filament 0's state (filaments_copy =
+  // filaments; filaments.resize(1); Np = its segment count, where the
+  // reference writes 15). 2: every filament, full barrier, committed step.
+  // This system is not touched by pass 1, so pass 2 starts from the state
+  // pass 0 left, as the reference's filaments = filaments_copy restores it.
+  // Spring links of a network stay on this system (the reference's pass 1
+  // would apply them to filaments it removed).
+  int gaitResetAndRemovePass(int nt, int gait_reset_i, Filaments &swimmer) {
+    int it = 0;
+    if (gait_reset_i == 1) {
+      check(fs_system_copy_filaments(swimmer.h, h, 0), "copy_filaments");
+      check(fs_system_set_barrier(swimmer.h, 0, FS_BARRIER_ALL),
+            "set_barrier");
+      check(fs_system_timestep_trial(swimmer.h, nt, &it), "timestep_trial");
+      return it;
+    }
+    check(fs_system_set_barrier(h, gait_reset_i == 2 ? 1 : 0, FS_BARRIER_ALL),
+          "set_barrier");
+    if (gait_reset_i == 0)
+      check(fs_system_timestep_trial(h, nt, &it), "timestep_trial");
+    else
+      check(fs_system_timestep(h, nt, &it), "timestep");
+    return it;
+  }
+  // filaments[fil].getCOM() (Filament.hpp:248-255)
+  std::vector<double> getCOM(int fil) {
+    std::vector<double> c(3);
+    check(fs_system_com(h, fil, c.data()), "getCOM");
+    return c;
+  }
+  // average_v of main_gaitresetandremove.cpp:781-787 (filament fil's mean
+  // segment velocity, float accumulation)
+  std::vector<double> meanVelocity(int fil) {
+    std::vector<double> v(3);
+    check(fs_system_mean_velocity(h, fil, v.data()), "meanVelocity");
+    return v;
+  }
   void free_memory() {
     if (h) fs_system_destroy(h);
     h = nullptr;
   }
 };
+
+// file_functions.cpp writers over a system's device state (the reference's
+// V, W, F, T and linked_list.F_total arguments are the system's own arrays
+// of its last step or trial pass; at nt = 0 only the positions are read).
+inline void save_data_to_file(const std::string &name, Filaments &f, int nt) {
+  check(fs_system_write_dat(f.h, name.c_str(), nt), "save_data_to_file");
+}
+inline void save_backup_data_to_file(const std::string &name, Filaments &f,
+                                     int nt) {
+  check(fs_system_write_bak(f.h, name.c_str(), nt),
+        "save_backup_data_to_file");
+}
+inline void save_data_column_names_to_swimmer_velocity_file(
+    const std::string &name, Filaments &f) {
+  check(fs_io_write_swimmer_velocity_names(name.c_str(), f.nw.at(0)),
+        "save_data_column_names_to_swimmer_velocity_file");
+}
+inline void save_data_column_names_to_swimmer_collision_file(
+    const std::string &name, Filaments &f) {
+  check(fs_io_write_swimmer_collision_names(name.c_str(), f.nw.at(0)),
+        "save_data_column_names_to_swimmer_collision_file");
+}
+inline void save_data_column_names_to_forces_file(const std::string &name,
+                                                  Filaments &f) {
+  check(fs_io_write_force_names(name.c_str(), f.nw.at(0)),
+        "save_data_column_names_to_forces_file");
+}
+inline void save_swimmer_velocity_data_to_file(const std::string &name,
+                                               Filaments &f, int nt) {
+  check(fs_system_write_swimmer_velocity(f.h, name.c_str(), nt),
+        "save_swimmer_velocity_data_to_file");
+}
+inline void save_swimmer_collision_data_to_file(const std::string &name,
+                                                Filaments &f, int nt) {
+  check(fs_system_write_swimmer_collision(f.h, name.c_str(), nt),
+        "save_swimmer_collision_data_to_file");
+}
+inline void save_force_data_to_file(const std::string &name, Filaments &f,
+                                    int nt) {
+  check(fs_system_write_forces(f.h, name.c_str(), nt),
+        "save_force_data_to_file");
+}
 
 // struct spring_links: the link table {A, B, C, D} (filament A bead B to
 // filament C bead D) and its "-springlinks.dat" file. The reference calls

@@ -36,6 +36,7 @@ struct or_colbar {
   int ncell;
   int *map, *head, *list;
   int *sfil, *sseg;
+  double *Ftot; /* F_total of the last apply (zero before the first) */
 };
 
 /* icell (CollisionBarrierFilament.cpp:307-315) */
@@ -81,6 +82,7 @@ or_colbar *or_colbar_create(const fs_colbar_params *p, int np,
   h->sseg = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
   memcpy(h->sfil, seg_fil, sizeof(int) * np);
   memcpy(h->sseg, seg_idx, sizeof(int) * np);
+  h->Ftot = (double *)calloc(3 * (size_t)(np > 0 ? np : 1), sizeof(double));
   return h;
 }
 
@@ -91,6 +93,7 @@ void or_colbar_destroy(or_colbar *h) {
   free(h->list);
   free(h->sfil);
   free(h->sseg);
+  free(h->Ftot);
   free(h);
 }
 
@@ -235,9 +238,18 @@ double or_colbar_apply(or_colbar *h, const double *Y, int nt, double *F) {
       i = h->list[i];
     }
   }
-  for (int i = 0; i < 3 * h->np; ++i) F[i] += Fn[i];
+  /* F_total of CollisionBarrierFilament.cpp:681-686, kept as the reference
+   * keeps linked_list.F_total, then added to the FCM forces */
+  for (int i = 0; i < 3 * h->np; ++i) {
+    h->Ftot[i] = Fn[i];
+    F[i] += Fn[i];
+  }
   free(Fn);
   return min_dist2;
+}
+
+void or_colbar_last_force(or_colbar *h, double *out) {
+  memcpy(out, h->Ftot, sizeof(double) * 3 * (size_t)h->np);
 }
 
 static int cmp_pair(const void *a, const void *b) {

@@ -1214,6 +1214,13 @@ double or_sim_barrier(or_sim *h, int nt, int link) {
   return or_colbar_apply(h->cb, h->Y, nt, h->Fa);
 }
 
+void or_sim_barrier_force(or_sim *h, double *out) {
+  if (h->cb)
+    or_colbar_last_force(h->cb, out);
+  else
+    memset(out, 0, sizeof(double) * 3 * (size_t)h->np);
+}
+
 void or_sim_set_barrier(or_sim *h, int on, int mask) {
   h->barrier_off = !on;
   or_colbar_set_mask(h->cb, mask);

@@ -57,6 +57,8 @@ void or_colbar_link(or_colbar *h, const double *Xs);
 void or_colbar_get_lists(or_colbar *h, int *head, int *list);
 /* F += barrier forces; returns min_dist2. */
 double or_colbar_apply(or_colbar *h, const double *Y, int nt, double *F);
+/* linked_list.F_total: the barrier force of the last apply (np x 3). */
+void or_colbar_last_force(or_colbar *h, double *out);
 /* Pair mask (FS_BARRIER_*) of later apply / pairs calls. */
 void or_colbar_set_mask(or_colbar *h, int mask);
 /* Pairs within cutoff via the cell list (i<j sorted); returns count
@@ -101,6 +103,8 @@ int or_sim_residual(or_sim *h, int nt, const double *V, const double *W,
 void or_sim_step(or_sim *h, int nt);
 void or_sim_mobility(or_sim *h, int nt);
 double or_sim_barrier(or_sim *h, int nt, int link);
+/* F_total of the sim's last barrier application (zero if none). */
+void or_sim_barrier_force(or_sim *h, double *out);
 
 /* ---- Small closed-form helpers exposed for tests ---- */
 void or_quat_expm(double *q, const double *v);        /* q <- exp(v) q   */

@@ -64,6 +64,7 @@ def lib():
         "or_sim_timestep": (C.c_int, [P, C.c_int]),
         "or_sim_timestep_trial": (C.c_int, [P, C.c_int]),
         "or_sim_set_barrier": (None, [P, C.c_int, C.c_int]),
+        "or_sim_barrier_force": (None, [P, _DP]),
         "or_sim_initial_guess": (None, [P]),
         "or_sim_forces": (None, [P, C.c_int]),
         "or_sim_set_spring_links": (C.c_int, [P, C.c_int, _IP, C.c_double,
@@ -358,3 +359,9 @@ class Sim:
 
     def barrier(self, nt, link):
         return lib().or_sim_barrier(self.h, nt, link)
+
+    def barrier_force(self):
+        """linked_list.F_total of the last barrier application."""
+        out = np.zeros((self.np, 3))
+        lib().or_sim_barrier_force(self.h, dp(out))
+        return out

@@ -131,3 +131,63 @@ def test_spring_link_file(tmp_path):
     bad.write_text("header\n2\n0 1 2 3\n")
     with pytest.raises(abi.FsError):
         api.io_read_spring_links(str(bad))
+
+
+def test_swimmer_and_force_file_layouts(tmp_path):
+    """The extended output files (file_functions.cpp:102-145, 164-208;
+    Filament.hpp:509-650): column names of filament 0 after "it"; at nt = 0
+    the positions of filament 0 and the integer 0 for every velocity and
+    force; later rows in 16-decimal fixed notation."""
+    nws = [4, 3, 13]
+    npt = sum(nws)
+    rng = np.random.default_rng(5)
+    Xs, qs = rng.normal(0, 9, (npt, 3)), rng.normal(0, 1, (npt, 4))
+    V, W, F, T, Ft = (rng.normal(0, 1, (npt, 3)) for _ in range(5))
+    pv, pc, pf = (str(tmp_path / n) for n in ("v.dat", "c.dat", "f.dat"))
+    api.io_write_names(pv, "velocity", nws[0])
+    api.io_write_names(pc, "collision", nws[0])
+    api.io_write_names(pf, "forces", nws[0])
+    api.io_write_swimmer_velocity(pv, 0, nws[0], Xs, qs)
+    api.io_write_swimmer_collision(pc, 0, nws[0])
+    api.io_write_forces(pf, 0, nws)
+    api.io_write_swimmer_velocity(pv, 7, nws[0], Xs, qs, V, W, F)
+    api.io_write_swimmer_collision(pc, 7, nws[0], Ft)
+    api.io_write_forces(pf, 7, nws, F, T)
+    lv, lc, lf = (open(p).read().split("\n") for p in (pv, pc, pf))
+    for lines in (lv, lc, lf):
+        assert len(lines) == 4 and lines[-1] == ""
+    assert lv[0] == "it " + "X Y Z qr qi qj qk " * 4 + \
+        "Vx Vy Vz Wx Wy Wz Fx Fy Fz " * 4
+    assert lc[0] == "it " + "Fx Fy Fz " * 4
+    assert lf[0] == "it " + "Fx Fy Fz Tx Ty Tz " * 4
+    # nt = 0: positions, then integer zeros
+    t = lv[1].split(" ")[:-1]
+    assert t[0] == "0" and len(t) == 1 + 7 * 4 + 9 * 4
+    assert all(NUM.match(x) for x in t[1:29]) and t[29:] == ["0"] * 36
+    assert lc[1] == "0 " + "0 " * 12
+    assert lf[1] == "0 " + "0 " * (6 * npt)
+    # nt = 7: filament 0's Xs qs then V W F per segment
+    t = [float(x) for x in lv[2].split(" ")[1:-1]]
+    assert lv[2].startswith("7 ")
+    pos = np.hstack([Xs[:4], qs[:4]]).ravel()
+    vel = np.hstack([V[:4], W[:4], F[:4]]).ravel()
+    assert np.abs(np.array(t) - np.concatenate([pos, vel])).max() <= 1e-14
+    t = [float(x) for x in lc[2].split(" ")[1:-1]]
+    assert np.abs(np.array(t) - Ft[:4].ravel()).max() <= 1e-15
+    # forces: filament f reads rows (first segment of f) // 6 + i, so
+    # filament 1 (first segment 4) rows 0..2 and filament 2 (first segment
+    # 7) rows 1..13 -- the reference's forceLengthFilled / 6
+    t = np.array([float(x) for x in lf[2].split(" ")[1:-1]]).reshape(-1, 6)
+    rows = list(range(4)) + list(range(0, 3)) + list(range(1, 14))
+    assert np.abs(t - np.hstack([F[rows], T[rows]])).max() <= 1e-15
+
+
+def test_swimmer_files_reject_missing_arrays(tmp_path):
+    p = str(tmp_path / "v.dat")
+    with pytest.raises(abi.FsError):
+        api.io_write_swimmer_velocity(p, 3, 2, np.zeros((2, 3)),
+                                      np.zeros((2, 4)))
+    with pytest.raises(abi.FsError):
+        api.io_write_forces(p, 3, [2, 2])
+    with pytest.raises(abi.FsError):
+        api.io_write_names(p, "velocity", 0)
