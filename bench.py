@@ -45,8 +45,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_system(precision, nfil=1000, nw=30, grid=256, seed=20261202, kap=1000.0,
-                 tol=1e-6, dt=0.005):
+def workload(precision, nfil=1000, nw=30, grid=256, seed=20261202,
+             kap=1000.0, tol=1e-6, dt=0.005):
+    """The swimming workload's parameters, filament properties and initial
+    positions/orientations."""
     import numpy as np
     from filsim_amd import api, params
     fp = params.fcm_params(grid)
@@ -59,7 +61,13 @@ def build_system(precision, nfil=1000, nw=30, grid=256, seed=20261202, kap=1000.
                                                           nfil)))
     X, q = api.make_filaments(params.box(fp), [nw] * nfil, dL=2.2,
                               min_sep=2.2, seed=seed)
-    s = api.System(sp, [nw] * nfil, props)
+    return fp, sp, [nw] * nfil, props, X, q
+
+
+def build_system(precision, **kw):
+    from filsim_amd import api
+    fp, sp, nws, props, X, q = workload(precision, **kw)
+    s = api.System(sp, nws, props)
     s.set_state(X=X, Xs=X, Xt=X, q=q, qs=q, qt=q)
     return s, fp
 
@@ -116,25 +124,105 @@ def mobility_bench(precision, reps, torch, np_=30000, grid=256, seed=7):
     return t, stages, fp
 
 
-def cpu_baseline(fp_iters):
-    """FP64 oracle (CPU restatement, one thread) on one 256^3 / 30,000-
-    segment mobility solve, scaled to timesteps/s with the GPU FP64 path's
-    mean Broyden iterations per step (each step = iterations + 1 solves)."""
+CPU_BUCKETS = ("assign_F", "build_Jinv", "collision_barrier", "FCM",
+               "error_check", "solve_Jinv", "timestep_total", "FCM_spread",
+               "FCM_fft_forward", "FCM_flow_solve", "FCM_fft_backward",
+               "FCM_particle_velocities_rotations")
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(state, nt0, max_steps=3, budget_s=25.0, wl=None):
+    """The oracle (the FP64 CPU restatement of main.cpp:284-754, its FFT,
+    spread, interpolation and per-filament loops on OpenMP threads) steps
+    the workload from `state` at timestep nt0 -- the state the GPU paths
+    start from -- for max_steps real timesteps, or fewer once budget_s of
+    wall time has passed. Returns timesteps/s, the iterations per step,
+    the profiler buckets (profilers.hpp:24-55) in seconds per step, the
+    thread count and the CPU model."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
-    from filsim_amd import params
-    fp = params.fcm_params(256)
-    rng = np.random.default_rng(7)
-    L = np.array(params.box(fp))
-    Y = rng.uniform(0, 1, (30000, 3)) * L
-    F = rng.standard_normal((30000, 3))
-    T = rng.standard_normal((30000, 3))
+    from filsim_amd import abi
+    wl = wl or {}
+    fp, sp, nws, props, _, _ = workload(abi.FS_FP64, **wl)
+    o = O.Sim(sp, nws, props)
+    o.set_state(**state)
+    its, tb = [], np.zeros(len(CPU_BUCKETS))
+    nt = nt0
     t0 = time.perf_counter()
-    O.mobility(fp, Y, F, T)
-    t_solve = time.perf_counter() - t0
-    solves_per_step = fp_iters + 1
-    return t_solve, 1.0 / (t_solve * solves_per_step), solves_per_step
+    while len(its) < max_steps:
+        its.append(o.timestep(nt))
+        tb += o.bucket_times()
+        nt += 1
+        log("cpu oracle step %d: %d iterations, %.1f s so far" %
+            (nt - 1, its[-1], time.perf_counter() - t0))
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    n = len(its)
+    return {"timesteps_per_s": n / el, "steps": n, "first_step": nt0,
+            "iterations": its, "seconds": el,
+            "buckets_s_per_step": {k: round(float(v) / n, 4)
+                                   for k, v in zip(CPU_BUCKETS, tb)},
+            "fcm_solve_s_mean": float(tb[3]) / (sum(its) + n),
+            "threads": O.num_threads(), "cpu_model": cpu_model()}
+
+
+def c1_cpu_gpu(args, torch):
+    """The CPU-runnable case in full (16 filaments x 15 segments, 64^3,
+    gravity, barrier on, TOL 1e-4, dt 0.01, 100 steps, FP64) on the oracle
+    and on the GPU from the same state: timesteps/s of each, iterations
+    per step and the oracle's profiler buckets."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    from filsim_amd import abi, api, params
+    fp = params.fcm_params(64)
+    sp = params.sim_params(fp, dt=0.01, tol=1e-4, maxiter=100,
+                           precision=abi.FS_FP64, weight=(0.0, 0.0, -1.0),
+                           barrier_on=1)
+    nws = [15] * 16
+    props = params.fil_props(16, kap=1.0)
+    X, q = api.make_filaments(params.box(fp), nws, dL=2.2, min_sep=2.2,
+                              seed=20261202)
+    st = dict(X=X, Xs=X, Xt=X, q=q, qs=q, qt=q)
+    out = {"cpu_model": cpu_model(), "threads": O.num_threads()}
+    o = O.Sim(sp, nws, props)
+    o.set_state(**st)
+    its, tb = [], np.zeros(len(CPU_BUCKETS))
+    t0 = time.perf_counter()
+    for nt in range(1, 101):
+        its.append(o.timestep(nt))
+        tb += o.bucket_times()
+        if nt % 20 == 0:
+            log("c1 oracle step %d (%.1f s)" % (nt, time.perf_counter() - t0))
+    el = time.perf_counter() - t0
+    out["c1_cpu"] = {"timesteps_per_s": 100 / el, "iterations": its,
+                     "buckets_s_per_step": {
+                         k: round(float(v) / 100, 5)
+                         for k, v in zip(CPU_BUCKETS, tb)}}
+    g = api.System(sp, nws, props)
+    g.set_state(**st)
+    gits = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for nt in range(1, 101):
+        gits.append(g.timestep(nt))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    g.close()
+    out["c1_gpu_fp64"] = {"timesteps_per_s": 100 / el, "iterations": gits}
+    return out
 
 
 def batch_bench(args, torch, n=64, seed=4000):
@@ -391,7 +479,7 @@ def main():
     ap.add_argument("--skip", type=int, default=10,
                     help="start-up steps run before recording")
     ap.add_argument("--only", choices=["all", "mixed", "fp64", "mobility",
-                                       "copy", "batch", "c5", "c3"],
+                                       "copy", "batch", "c5", "c3", "cpu"],
                     default="all")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -414,6 +502,9 @@ def main():
         return
     if args.only == "c3":
         print(json.dumps(c3_bench(args, torch)), flush=True)
+        return
+    if args.only == "cpu":
+        print(json.dumps(c1_cpu_gpu(args, torch)), flush=True)
         return
     if args.only == "copy":
         # calibration for HBM byte counters: a plain 1 GiB FP32 scale-by-one
@@ -451,6 +542,7 @@ def main():
 
     res = {}
     info = {}
+    cpu_state, nt0 = None, 1
     if args.only in ("all", "fp64", "mixed"):
         from filsim_amd import api, warmstart
         # 1) the mixed-precision solver generates the training trajectory
@@ -471,6 +563,7 @@ def main():
         keep = {k: state[k] for k in ("X", "Xt", "U", "Ut", "q", "qt", "lam",
                                       "lam1", "lam2")}
         keep.update(Xs=state["X"], Us=state["U"], qs=state["q"])
+        cpu_state = keep
         api.system_set_warmstart(s_l, mlp)
         info["train"] = {"samples": args.record * 1000, "skip": args.skip,
                          "record_s": t_rec,
@@ -493,11 +586,12 @@ def main():
             else:
                 s, _ = build_system(prec)
                 s.set_state(**keep)
-            its, used = [], 0
+            its, used, wits = [], 0, []
             nt = nt0
             for _ in range(args.warmup):
-                s.timestep(nt)
+                wits.append(s.timestep(nt))
                 nt += 1
+            info["warmup_iters_" + name] = wits
             ref = None
             if name == "learned" and args.refresh > 0:
                 ref = warmstart.OnlineRefresh(s, mlp, torch,
@@ -554,16 +648,23 @@ def main():
     out.update(info)
 
     cpu = None
-    if not args.no_cpu:
-        iters = res["fp64"][1] if "fp64" in res else [10]
-        mean_it = sum(iters) / len(iters)
-        t_solve, tsps, sps = cpu_baseline(mean_it)
-        cpu = {"value": tsps, "unit": "timesteps/s", "cores": 1,
-               "kind": "port",
-               "sample": ("one FP64 256^3 / 30,000-segment mobility solve "
-                          "by the oracle (%.1f s), scaled by %.1f solves per "
-                          "step (GPU FP64 mean Broyden iterations + 1)" %
-                          (t_solve, sps))}
+    if not args.no_cpu and cpu_state is not None:
+        cb = cpu_baseline(cpu_state, nt0, max_steps=2, budget_s=30.0)
+        cpu = {"value": cb["timesteps_per_s"], "unit": "timesteps/s",
+               "cores": cb["threads"], "kind": "port",
+               "cpu_model": cb["cpu_model"],
+               "sample": ("the oracle's FP64 timesteps %d-%d of this "
+                          "workload (%s Broyden iterations; the GPU FP64 "
+                          "path's first steps from the same state took %s) "
+                          "on %d OpenMP threads, %.1f s" %
+                          (cb["first_step"], cb["first_step"] + cb["steps"]
+                           - 1, cb["iterations"],
+                           (info.get("warmup_iters_fp64", []) +
+                            res["fp64"][1])[:cb["steps"]]
+                           if "fp64" in res else "-",
+                           cb["threads"], cb["seconds"])),
+               "buckets_s_per_step": cb["buckets_s_per_step"],
+               "fcm_solve_s_mean": cb["fcm_solve_s_mean"]}
         log("cpu baseline", cpu)
 
     ms = 1e3 * res["learned"][0] / args.steps if "learned" in res else None

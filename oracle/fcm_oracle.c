@@ -30,6 +30,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define OR_PI 3.14159265358979323846264338327950288
 
@@ -42,6 +45,15 @@ static double now_s(void) {
 }
 
 void or_fcm_last_times(double *t5) { memcpy(t5, g_times, sizeof g_times); }
+
+/* threads the OpenMP loops use (1 without OpenMP) */
+int or_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 /* ------------------------------------------------------------------ */
 /* FFT: recursive mixed radix, out-of-place, twiddles from a table.    */
@@ -134,62 +146,80 @@ void or_fft1d(int n, double *data, int sign) {
 /* Transform along x and y of a half spectrum c[nx][ny][nzh]. */
 static void fft_xy(int nx, int ny, int nzh, double *c, int sign) {
   int nmax = nx > ny ? nx : ny;
-  double *line = (double *)malloc(sizeof(double) * 2 * nmax);
-  double *s = (double *)malloc(sizeof(double) * 2 * nmax);
   fft_plan py, px;
   plan_init(&py, ny, sign);
   plan_init(&px, nx, sign);
-  for (int x = 0; x < nx; ++x)
-    for (int k = 0; k < nzh; ++k) {
-      for (int y = 0; y < ny; ++y) {
-        size_t o = 2 * (((size_t)x * ny + y) * nzh + k);
-        line[2 * y] = c[o];
-        line[2 * y + 1] = c[o + 1];
+  /* each x plane [y][k] (and each y plane [x][k]) is copied into nzh
+   * contiguous lines, transformed line by line and copied back: the same
+   * per-line transforms as a strided walk, with cache-friendly access */
+#pragma omp parallel
+  {
+    double *blk = (double *)malloc(sizeof(double) * 2 * (size_t)nmax * nzh);
+    double *s = (double *)malloc(sizeof(double) * 2 * nmax);
+#pragma omp for schedule(static)
+    for (int x = 0; x < nx; ++x) {
+      double *P = c + 2 * (size_t)x * ny * nzh;
+      for (int y = 0; y < ny; ++y)
+        for (int k = 0; k < nzh; ++k) {
+          blk[2 * ((size_t)k * ny + y)] = P[2 * ((size_t)y * nzh + k)];
+          blk[2 * ((size_t)k * ny + y) + 1] = P[2 * ((size_t)y * nzh + k) + 1];
+        }
+      for (int k = 0; k < nzh; ++k) plan_exec(&py, blk + 2 * (size_t)k * ny, s);
+      for (int y = 0; y < ny; ++y)
+        for (int k = 0; k < nzh; ++k) {
+          P[2 * ((size_t)y * nzh + k)] = blk[2 * ((size_t)k * ny + y)];
+          P[2 * ((size_t)y * nzh + k) + 1] = blk[2 * ((size_t)k * ny + y) + 1];
+        }
+    }
+#pragma omp for schedule(static)
+    for (int y = 0; y < ny; ++y) {
+      for (int x = 0; x < nx; ++x) {
+        const double *P = c + 2 * ((size_t)x * ny + y) * nzh;
+        for (int k = 0; k < nzh; ++k) {
+          blk[2 * ((size_t)k * nx + x)] = P[2 * k];
+          blk[2 * ((size_t)k * nx + x) + 1] = P[2 * k + 1];
+        }
       }
-      plan_exec(&py, line, s);
-      for (int y = 0; y < ny; ++y) {
-        size_t o = 2 * (((size_t)x * ny + y) * nzh + k);
-        c[o] = line[2 * y];
-        c[o + 1] = line[2 * y + 1];
+      for (int k = 0; k < nzh; ++k) plan_exec(&px, blk + 2 * (size_t)k * nx, s);
+      for (int x = 0; x < nx; ++x) {
+        double *P = c + 2 * ((size_t)x * ny + y) * nzh;
+        for (int k = 0; k < nzh; ++k) {
+          P[2 * k] = blk[2 * ((size_t)k * nx + x)];
+          P[2 * k + 1] = blk[2 * ((size_t)k * nx + x) + 1];
+        }
       }
     }
-  for (int y = 0; y < ny; ++y)
-    for (int k = 0; k < nzh; ++k) {
-      for (int x = 0; x < nx; ++x) {
-        size_t o = 2 * (((size_t)x * ny + y) * nzh + k);
-        line[2 * x] = c[o];
-        line[2 * x + 1] = c[o + 1];
-      }
-      plan_exec(&px, line, s);
-      for (int x = 0; x < nx; ++x) {
-        size_t o = 2 * (((size_t)x * ny + y) * nzh + k);
-        c[o] = line[2 * x];
-        c[o + 1] = line[2 * x + 1];
-      }
-    }
+    free(blk);
+    free(s);
+  }
   plan_free(&py);
   plan_free(&px);
-  free(line);
-  free(s);
 }
 
+/* The OpenMP loops below split independent lines, planes or segments
+ * between threads; every output value is computed by one thread in the
+ * serial order, so results do not depend on the thread count. */
 void or_rfft3d(int nx, int ny, int nz, const double *r, double *c) {
   int nzh = nz / 2 + 1;
-  double *line = (double *)malloc(sizeof(double) * 2 * nz);
-  double *s = (double *)malloc(sizeof(double) * 2 * nz);
   fft_plan pz;
   plan_init(&pz, nz, -1);
-  for (size_t row = 0; row < (size_t)nx * ny; ++row) {
-    for (int z = 0; z < nz; ++z) {
-      line[2 * z] = r[row * nz + z];
-      line[2 * z + 1] = 0.0;
+#pragma omp parallel
+  {
+    double *line = (double *)malloc(sizeof(double) * 2 * nz);
+    double *s = (double *)malloc(sizeof(double) * 2 * nz);
+#pragma omp for schedule(static)
+    for (long row = 0; row < (long)nx * ny; ++row) {
+      for (int z = 0; z < nz; ++z) {
+        line[2 * z] = r[row * nz + z];
+        line[2 * z + 1] = 0.0;
+      }
+      plan_exec(&pz, line, s);
+      memcpy(c + 2 * row * nzh, line, sizeof(double) * 2 * nzh);
     }
-    plan_exec(&pz, line, s);
-    memcpy(c + 2 * row * nzh, line, sizeof(double) * 2 * nzh);
+    free(line);
+    free(s);
   }
   plan_free(&pz);
-  free(line);
-  free(s);
   fft_xy(nx, ny, nzh, c, -1);
 }
 
@@ -199,29 +229,33 @@ void or_irfft3d(int nx, int ny, int nz, const double *cin, double *r) {
   double *c = (double *)malloc(sizeof(double) * csz);
   memcpy(c, cin, sizeof(double) * csz);
   fft_xy(nx, ny, nzh, c, +1);
-  double *line = (double *)malloc(sizeof(double) * 2 * nz);
-  double *s = (double *)malloc(sizeof(double) * 2 * nz);
   fft_plan pz;
   plan_init(&pz, nz, +1);
-  for (size_t row = 0; row < (size_t)nx * ny; ++row) {
-    const double *h = c + 2 * row * nzh;
-    for (int k = 0; k < nzh; ++k) {
-      line[2 * k] = h[2 * k];
-      line[2 * k + 1] = h[2 * k + 1];
+#pragma omp parallel
+  {
+    double *line = (double *)malloc(sizeof(double) * 2 * nz);
+    double *s = (double *)malloc(sizeof(double) * 2 * nz);
+#pragma omp for schedule(static)
+    for (long row = 0; row < (long)nx * ny; ++row) {
+      const double *h = c + 2 * row * nzh;
+      for (int k = 0; k < nzh; ++k) {
+        line[2 * k] = h[2 * k];
+        line[2 * k + 1] = h[2 * k + 1];
+      }
+      /* a real 1D signal's DC and Nyquist coefficients are real */
+      line[1] = 0.0;
+      if (nz % 2 == 0) line[2 * (nz / 2) + 1] = 0.0;
+      for (int k = nzh; k < nz; ++k) { /* Hermitian completion */
+        line[2 * k] = h[2 * (nz - k)];
+        line[2 * k + 1] = -h[2 * (nz - k) + 1];
+      }
+      plan_exec(&pz, line, s);
+      for (int z = 0; z < nz; ++z) r[row * nz + z] = line[2 * z];
     }
-    /* a real 1D signal's DC and Nyquist coefficients are real */
-    line[1] = 0.0;
-    if (nz % 2 == 0) line[2 * (nz / 2) + 1] = 0.0;
-    for (int k = nzh; k < nz; ++k) { /* Hermitian completion */
-      line[2 * k] = h[2 * (nz - k)];
-      line[2 * k + 1] = -h[2 * (nz - k) + 1];
-    }
-    plan_exec(&pz, line, s);
-    for (int z = 0; z < nz; ++z) r[row * nz + z] = line[2 * z];
+    free(line);
+    free(s);
   }
   plan_free(&pz);
-  free(line);
-  free(s);
   free(c);
 }
 
@@ -300,9 +334,26 @@ void or_fcm_spread(const fs_fcm_params *p, int np, const double *Y,
   size_t N = (size_t)p->nx * p->ny * p->nz;
   memset(f3, 0, sizeof(double) * 3 * N);
   double *fx = f3, *fy = f3 + N, *fz = f3 + 2 * N;
+  /* each thread owns the x planes [x0, x1) and adds every segment's terms
+   * on them in segment order, as the serial loop does */
+#pragma omp parallel
+  {
+#ifdef _OPENMP
+  const int nth = omp_get_num_threads(), tid = omp_get_thread_num();
+#else
+  const int nth = 1, tid = 0;
+#endif
+  const int x0 = (int)((long)p->nx * tid / nth);
+  const int x1 = (int)((long)p->nx * (tid + 1) / nth);
   seg_tables t;
   tables_alloc(&t, g.ngd);
   for (int s = 0; s < np; ++s) {
+    axis_tables(&g, Y[3 * s], p->dx, p->nx, t.ind[0], t.ga[0], t.gd[0],
+                t.gr[0]);
+    int hit = 0;
+    for (int i = 0; i < g.ngd; ++i)
+      hit |= t.ind[0][i] >= x0 && t.ind[0][i] < x1;
+    if (!hit) continue;
     tables_fill(p, &g, Y + 3 * s, &t);
     double GA3 = 0.5 * T[3 * s + 2], GA4 = -0.5 * T[3 * s + 1],
            GA5 = 0.5 * T[3 * s + 0];
@@ -311,6 +362,7 @@ void or_fcm_spread(const fs_fcm_params *p, int np, const double *Y,
            g32 = -GA5;
     for (int i = 0; i < g.ngd; ++i) {
       int ii = t.ind[0][i];
+      if (ii < x0 || ii >= x1) continue;
       double xx = t.gr[0][i];
       for (int j = 0; j < g.ngd; ++j) {
         int jj = t.ind[1][j];
@@ -332,6 +384,7 @@ void or_fcm_spread(const fs_fcm_params *p, int np, const double *Y,
     }
   }
   tables_free(&t);
+  }
 }
 
 /* Wavenumbers (FCMfunctions.cpp:129-162): Nyquist index gets +N/2. */
@@ -351,6 +404,7 @@ void or_fcm_project(const fs_fcm_params *p, double *c3) {
   double Lx = nx * p->dx, Ly = ny * p->dx, Lz = nz * p->dx;
   /* k = 0: zero the mean force (FCMfunctions.cpp:659-666) */
   cx[0] = cx[1] = cy[0] = cy[1] = cz[0] = cz[1] = 0.0;
+#pragma omp parallel for schedule(static)
   for (int x = 0; x < nx; ++x) {
     double q1 = wavenumber(x, nx, Lx);
     for (int y = 0; y < ny; ++y) {
@@ -380,8 +434,11 @@ void or_fcm_interp(const fs_fcm_params *p, int np, const double *Y,
   size_t N = (size_t)p->nx * p->ny * p->nz;
   const double *ux = u3, *uy = u3 + N, *uz = u3 + 2 * N;
   double norm = p->dx * p->dx * p->dx;
+#pragma omp parallel
+  {
   seg_tables t;
   tables_alloc(&t, g.ngd);
+#pragma omp for schedule(static)
   for (int s = 0; s < np; ++s) {
     tables_fill(p, &g, Y + 3 * s, &t);
     double v0 = 0, v1 = 0, v2 = 0;
@@ -425,6 +482,7 @@ void or_fcm_interp(const fs_fcm_params *p, int np, const double *Y,
     W[3 * s + 2] = w2;
   }
   tables_free(&t);
+  }
 }
 
 void or_fcm_mobility(const fs_fcm_params *p, int np, const double *Y,

@@ -49,6 +49,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define OR_PI 3.14159265358979323846264338327950288
 
@@ -288,7 +289,23 @@ struct or_sim {
   int nspring;
   int *spring; /* 2 per link */
   double spring_k, spring_len;
+  /* seconds spent in the last timestep per profiler bucket (profilers.hpp:
+   * 24-55): assign F, build Jinv, collision barrier, FCM, error check,
+   * solve Jinv, total, then the FCM's spread (make_zero + gaussian_setup +
+   * force_distribution), FFT forward, flow solve, FFT backward and
+   * particle velocities/rotations */
+  double tb[OR_NBUCKETS];
 };
+
+static double wall_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+void or_sim_bucket_times(or_sim *h, double *out) {
+  memcpy(out, h->tb, sizeof h->tb);
+}
 
 static double *dz(size_t n) { return (double *)calloc(n ? n : 1, 8); }
 
@@ -1051,6 +1068,8 @@ static void build_jacobian_fil(or_sim *h, int f, int nt) {
 void or_sim_build_jacobian(or_sim *h, int nt) {
   memset(h->Cmat, 0, sizeof(double) * 6 * (size_t)h->np * h->p.broyden_maxiter);
   memset(h->Dmat, 0, sizeof(double) * 6 * (size_t)h->np * h->p.broyden_maxiter);
+  /* filaments are independent: one thread per filament at a time */
+#pragma omp parallel for schedule(dynamic, 4)
   for (int f = 0; f < h->nf; ++f) build_jacobian_fil(h, f, nt);
 }
 
@@ -1068,6 +1087,7 @@ void or_sim_get_jinv(or_sim *h, int f, double *out) {
 void or_sim_jsolve(or_sim *h, const double *b, double *x, int iter) {
   size_t N6 = 6 * (size_t)h->np;
   double *xo = (double *)malloc(sizeof(double) * N6);
+#pragma omp parallel for schedule(dynamic, 8)
   for (int f = 0; f < h->nf; ++f) {
     int n = 6 * h->nw[f];
     size_t so = 6 * (size_t)h->boff[f];
@@ -1228,28 +1248,59 @@ void or_sim_set_barrier(or_sim *h, int on, int mask) {
 
 /* main.cpp:284-754; commit = 0: a trial pass of main_gaitreset.cpp
  * (gait_reset_i < 2), which skips Filament::step (:775-781) */
+/* the FCM solve, its time and its stage times into the buckets */
+static void timed_mobility(or_sim *h, int nt) {
+  double t0 = wall_s();
+  or_sim_mobility(h, nt);
+  h->tb[3] += wall_s() - t0;
+  if (nt > h->p.initial_friction_steps) {
+    double st[5];
+    or_fcm_last_times(st);
+    for (int k = 0; k < 5; ++k) h->tb[7 + k] += st[k];
+  }
+}
+
 static int timestep_impl(or_sim *h, int nt, int commit) {
   size_t N6 = 6 * (size_t)h->np;
   const int bar = h->p.barrier_on && !h->barrier_off;
+  memset(h->tb, 0, sizeof h->tb);
+  double t0 = wall_s(), t;
   or_sim_initial_guess(h);
   or_sim_forces(h, nt);
+  h->tb[0] += (t = wall_s()) - t0;
   or_sim_build_jacobian(h, nt);
+  h->tb[1] += wall_s() - t;
+  t = wall_s();
   if (bar) or_sim_barrier(h, nt, 1);
-  or_sim_mobility(h, nt);
+  h->tb[2] += wall_s() - t;
+  timed_mobility(h, nt);
+  t = wall_s();
   int check = or_sim_residual(h, nt, h->V, h->W, h->E, NULL);
+  h->tb[4] += wall_s() - t;
   int iter = 0;
   while (check && iter < h->p.broyden_maxiter) {
+    t = wall_s();
     or_sim_jsolve(h, h->E, h->Upd, iter);
+    h->tb[5] += wall_s() - t;
+    t = wall_s();
     or_sim_update(h, h->Upd);
     or_sim_forces(h, nt);
+    h->tb[0] += wall_s() - t;
+    t = wall_s();
     if (bar) or_sim_barrier(h, nt, 0);
-    or_sim_mobility(h, nt);
+    h->tb[2] += wall_s() - t;
+    timed_mobility(h, nt);
+    t = wall_s();
     check = or_sim_residual(h, nt, h->V, h->W, h->NewE, NULL);
+    h->tb[4] += wall_s() - t;
+    t = wall_s();
     or_sim_broyden_update(h, h->E, h->NewE, h->Upd, iter);
     memcpy(h->E, h->NewE, sizeof(double) * N6);
+    h->tb[5] += wall_s() - t;
     ++iter;
   }
   if (commit) or_sim_step(h, nt);
+  h->tb[6] = wall_s() - t0;
   return iter;
 }
 

@@ -46,6 +46,8 @@ void or_fcm_friction(const fs_fcm_params *p, int np, const double *F,
 /* Per-stage wall time (seconds) of the last or_fcm_mobility call:
  * gauss+spread, fft forward, flow solve, fft backward, interpolation. */
 void or_fcm_last_times(double *t5);
+/* Threads of the OpenMP loops (OMP_NUM_THREADS or every core). */
+int or_num_threads(void);
 
 /* ---- Collision barrier ---- */
 typedef struct or_colbar or_colbar;
@@ -72,6 +74,13 @@ int or_colbar_pairs_brute(const fs_colbar_params *p, int np, const double *Y,
 
 /* ---- Filament system ---- */
 typedef struct or_sim or_sim;
+#define OR_NBUCKETS 12
+/* Seconds of the last or_sim_timestep(_trial) per profiler bucket
+ * (profilers.hpp:24-55): assign F, build Jinv, collision barrier, FCM,
+ * error check, solve Jinv, timestep total, FCM spread (make_zero +
+ * gaussian_setup + force_distribution), FFT forward, flow solve, FFT
+ * backward, particle velocities/rotations. */
+void or_sim_bucket_times(or_sim *h, double *out);
 or_sim *or_sim_create(const fs_sim_params *p, int nfil, const int *nw,
                       const fs_fil_props *props);
 void or_sim_destroy(or_sim *h);

@@ -65,6 +65,8 @@ def lib():
         "or_sim_timestep_trial": (C.c_int, [P, C.c_int]),
         "or_sim_set_barrier": (None, [P, C.c_int, C.c_int]),
         "or_sim_barrier_force": (None, [P, _DP]),
+        "or_sim_bucket_times": (None, [P, _DP]),
+        "or_num_threads": (C.c_int, []),
         "or_sim_initial_guess": (None, [P]),
         "or_sim_forces": (None, [P, C.c_int]),
         "or_sim_set_spring_links": (C.c_int, [P, C.c_int, _IP, C.c_double,
@@ -129,6 +131,11 @@ def irfft3d(c, nz):
     r = np.empty((nx, ny, nz))
     lib().or_irfft3d(nx, ny, nz, dp(buf), dp(r))
     return r
+
+
+def num_threads():
+    """Threads of the oracle's OpenMP loops."""
+    return lib().or_num_threads()
 
 
 def mobility(fp, Y, F, T):
@@ -359,6 +366,12 @@ class Sim:
 
     def barrier(self, nt, link):
         return lib().or_sim_barrier(self.h, nt, link)
+
+    def bucket_times(self):
+        """Seconds of the last timestep per profiler bucket (oracle.h)."""
+        out = np.zeros(12)
+        lib().or_sim_bucket_times(self.h, dp(out))
+        return out
 
     def barrier_force(self):
         """linked_list.F_total of the last barrier application."""
