@@ -4,7 +4,9 @@
 //   prepare   per-segment 1D Gaussian tables + stencil bin   (gaussian_setup,
 //             FCMfunctions.cpp:549-593; GA_setup :536-547)
 //   bin       counting sort of segments by the grid tile holding their
-//             stencil origin (deterministic order inside a bin)
+//             stencil origin (deterministic order inside a bin); on the
+//             matrix-core path each tile then gets the records of the
+//             segments whose stencil meets it (k_tile_lists)
 //   spread    gather form of mono_and_dipole_force_distribution
 //             (:596-646): one workgroup per grid tile sums, in a fixed
 //             order, every segment whose stencil covers the tile, then
@@ -19,7 +21,9 @@
 //   fft_y     inverse along y
 //   fft_z_c2r half-complex->real along z
 //   interp    particle_velocities + particle_rotations (:740-808) fused:
-//             one wavefront per segment, segments visited in bin order
+//             on the matrix cores per tile (k_interp_mfma: the transpose
+//             of the tile spread, partial sums added per segment in a
+//             fixed order), else one wavefront per segment in bin order
 // Grids: real R[3][nx][ny][nz], half spectrum C[3][nx][ny][nz/2+1] (complex).
 // Precision: FS_FP64 runs every stage in double; FS_FP32 runs them in float
 // with float accumulation (stencil offsets are formed from the FP64
@@ -51,7 +55,26 @@ constexpr int kCoef = 8;  // spread coefficients per segment (6 used)
 struct GaussConst {
   double dx, anorm, anorm2, anormdip, anormdip2, sigmadipsq;
   int ngd, ngdh, nx, ny, nz;
+  // FP32 weights: g = anorm 2^(-xx^2 e2), gD = anormdip 2^(-xx^2 ed2),
+  // e2 = log2(e) / anorm2, ed2 = log2(e) / anormdip2
+  double e2, ed2;
+  double nrsd;  // -1 / sigmadipsq
 };
+
+// What a tile needs of one segment: per axis the FP64 offset xx0 = (grid
+// coordinate of stencil point 0) * dx - y (point i lies at xx0 + i dx), its
+// id, stencil origin (10 bits per axis, x lowest) and spread coefficients
+// (force, half torque: GA_setup).
+template <typename T>
+struct alignas(16) SegRec {
+  double xx0[3];
+  int id, org;
+  T cf[6];
+};
+constexpr int kTileCap = 32;  // records in a tile's slab; the rest: pool
+inline size_t rec_bytes(int tsz) {
+  return tsz == 8 ? sizeof(SegRec<double>) : sizeof(SegRec<float>);
+}
 
 // Grid cell holding coordinate y: truncation of y/dx, as
 // FCMfunctions.cpp:565-567. A coordinate that is not finite or lies more
@@ -140,12 +163,19 @@ __global__ void k_scatter(int np, const int *__restrict__ bin,
 }
 
 // Ascending segment order inside each bin (atomic slots are unordered);
-// idorg (may be null) receives (segment, stencil origin packed 10 bits per
-// axis) in the same order, so a tile's candidates come with one load.
-__global__ void k_sort_bins(int nbins, const int *__restrict__ start,
+// srec and sorg (the matrix-core path; may be null) receive each segment's
+// record and packed stencil origin in the same order.
+template <typename T>
+__global__ void k_sort_bins(const int *__restrict__ act, int nbins,
+                            const int *__restrict__ start,
                             int *__restrict__ ids,
                             const int *__restrict__ origin,
-                            int2 *__restrict__ idorg) {
+                            const double *__restrict__ Y,
+                            const double *__restrict__ F,
+                            const double *__restrict__ Tq, double dx,
+                            int ngdh, SegRec<T> *__restrict__ srec,
+                            int *__restrict__ sorg) {
+  if (act && !*act) return;  // batched sweep: suspension converged
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbins) return;
   int lo = start[b], hi = start[b + 1];
@@ -158,11 +188,21 @@ __global__ void k_sort_bins(int nbins, const int *__restrict__ start,
     }
     ids[j + 1] = v;
   }
-  if (idorg)
+  if (srec)
     for (int i = lo; i < hi; ++i) {
       const int sid = ids[i];
-      idorg[i] = make_int2(sid, origin[3 * sid] | (origin[3 * sid + 1] << 10) |
-                                    (origin[3 * sid + 2] << 20));
+      SegRec<T> r;
+      for (int d = 0; d < 3; ++d) {
+        const double y = Y[3 * sid + d];
+        r.xx0[d] = ((double)(stencil_cell(y, dx, sid) - ngdh + 1)) * dx - y;
+        r.cf[d] = (T)F[3 * sid + d];
+        r.cf[3 + d] = (T)(0.5 * Tq[3 * sid + d]);
+      }
+      r.id = sid;
+      r.org = origin[3 * sid] | (origin[3 * sid + 1] << 10) |
+              (origin[3 * sid + 2] << 20);
+      srec[i] = r;
+      sorg[i] = r.org;
     }
 }
 
@@ -329,6 +369,222 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------
+// Matrix-core spread and interpolation work on 8 x 8 x 16 grid tiles.
+// Per solve, k_tile_lists gives every tile the records of the segments
+// whose stencil meets it; both kernels then load a tile's records in one
+// round trip and form the Gaussian weights of those segments at the tile's
+// own 8 + 8 + 16 axis points in LDS (no per-segment tables).
+// ---------------------------------------------------------------------
+
+// Whether a stencil of ngd points starting at o overlaps the te points
+// starting at p0 on a periodic axis of n points (o, p0 in [0, n)).
+__device__ __forceinline__ bool stencil_meets(int o, int p0, int te, int ngd,
+                                             int n) {
+  int d = o - p0;
+  if (d < 0) d += n;
+  return d < te || d > n - ngd;
+}
+
+// Block id -> tile (bx, by, bz): consecutive block ids on one XCD (blocks
+// are dealt to the 8 XCDs round-robin) take a contiguous run of tiles, and
+// z-neighbouring tiles, which share 128-byte lines of the grid, are next to
+// each other in it when nbz is even.
+__device__ __forceinline__ void tile_of_block(const SpreadGeom &G, int bid,
+                                              int *bx, int *by, int *bz) {
+  const int ntiles = G.nbx * G.nby * G.nbz;
+  if ((ntiles & 7) == 0) bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
+  if ((G.nbz & 1) == 0) {
+    const int zl = bid & 1, r = bid >> 1;
+    *bx = r % G.nbx;
+    *by = (r / G.nbx) % G.nby;
+    *bz = 2 * (r / (G.nbx * G.nby)) + zl;
+  } else {
+    *bx = bid % G.nbx;
+    *by = (bid / G.nbx) % G.nby;
+    *bz = bid / (G.nbx * G.nby);
+  }
+}
+
+// One wavefront per tile (tile = 4 * block id + wave, x fastest): the
+// segments of the bins that can reach the tile whose stencil meets it, in
+// (neighbouring bin, position in bin) order; the test reads the packed
+// origins (sorg), only kept segments' records are copied. The first
+// kTileCap records go to the tile's slab, the rest to a block of the pool
+// reserved with one atomic per tile (where the block lies varies between
+// runs, the order inside it does not); tcnt = how many, tovf = the pool
+// block. The pool holds np times the tiles one stencil can meet, a bound on
+// the records of all tiles.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_tile_lists(const int *__restrict__ act, SpreadGeom G,
+                 const int *__restrict__ start, const int *__restrict__ sorg,
+                 const SegRec<T> *__restrict__ srec, SegRec<T> *__restrict__ slab,
+                 int *__restrict__ tcnt, int *__restrict__ tovf,
+                 SegRec<T> *__restrict__ pool, int *__restrict__ pool_top) {
+  if (act && !*act) return;  // batched sweep: suspension converged
+  __shared__ int s_pre[4][64], s_first[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ntiles = G.nbx * G.nby * G.nbz;
+  const int tile = blockIdx.x * 4 + wave;
+  if (tile >= ntiles) return;  // whole wave; no workgroup barrier below
+  int *pre = s_pre[wave], *first = s_first[wave];
+  const int bx = tile % G.nbx, by = (tile / G.nbx) % G.nby,
+            bz = tile / (G.nbx * G.nby);
+  const int cx = min(G.hx + 1, G.nbx), cy = min(G.hy + 1, G.nby),
+            cz = min(G.hz + 1, G.nbz);
+  const int nbins = cx * cy * cz;  // <= 64
+  int n = 0, f = 0;
+  if (lane < nbins) {
+    const int dz = lane / (cx * cy), dy = (lane / cx) % cy, dx = lane % cx;
+    const int ix = ((bx - G.hx + dx) % G.nbx + G.nbx) % G.nbx;
+    const int iy = ((by - G.hy + dy) % G.nby + G.nby) % G.nby;
+    const int iz = ((bz - G.hz + dz) % G.nbz + G.nbz) % G.nbz;
+    const int b = spread_bin(ix, iy, iz, G.nbx, G.nby, G.nbz);
+    f = start[b];
+    n = start[b + 1] - f;
+  }
+  int incl = n;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  const int total = __shfl(incl, 63, 64);
+  pre[lane] = incl - n;
+  first[lane] = f;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int px0 = bx * 8, py0 = by * 8, pz0 = bz * 16;
+  int kept = 0, base = 0;
+  // pass 0 counts and fills the slab; pass 1 (a tile with more than
+  // kTileCap records) fills its pool block
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1) {
+      if (kept <= kTileCap) break;
+      int b0 = 0;
+      if (lane == 0) b0 = atomicAdd(pool_top, kept - kTileCap);
+      base = __shfl(b0, 0, 64);
+      kept = 0;
+    }
+    for (int j0 = 0; j0 < total; j0 += 64) {
+      const int j = j0 + lane;
+      bool keep = false;
+      int pos = 0;
+      if (j < total) {
+        // the last bin whose first list position is <= j holds entry j
+        int lo = 0, hi = nbins - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (pre[mid] <= j)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        pos = first[lo] + j - pre[lo];
+        const int org = sorg[pos];
+        keep = stencil_meets(org & 1023, px0, 8, G.ngd, G.nx) &&
+               stencil_meets((org >> 10) & 1023, py0, 8, G.ngd, G.ny) &&
+               stencil_meets(org >> 20, pz0, 16, G.ngd, G.nz);
+      }
+      const unsigned long long kb = __ballot(keep);
+      const int k = kept + __popcll(kb & ((1ull << lane) - 1ull));
+      if (keep) {
+        if (pass == 0 && k < kTileCap)
+          slab[(size_t)tile * kTileCap + k] = srec[pos];
+        else if (pass == 1 && k >= kTileCap)
+          pool[base + k - kTileCap] = srec[pos];
+      }
+      kept += __popcll(kb);
+    }
+  }
+  if (lane == 0) {
+    tcnt[tile] = kept;
+    tovf[tile] = base;
+  }
+}
+
+// Stages records [r0, r0 + n) of tile `tile` into s_rec: r0 = 0 copies the
+// whole slab (issued before the count is known; entries past the count are
+// never used), r0 >= kTileCap the tile's pool block. 256 threads.
+template <typename T>
+__device__ __forceinline__ void stage_tile_records(
+    const SegRec<T> *__restrict__ slab, const SegRec<T> *__restrict__ pool,
+    int tile, int ovf, int r0, int n, SegRec<T> *s_rec) {
+  constexpr int W = sizeof(SegRec<T>) / 16;
+  static_assert(kTileCap * W <= 256, "one 16-byte word per thread");
+  const int4 *src =
+      r0 == 0 ? reinterpret_cast<const int4 *>(slab + (size_t)tile * kTileCap)
+              : reinterpret_cast<const int4 *>(pool + ovf + r0 - kTileCap);
+  const int nw = (r0 == 0 ? kTileCap : n) * W;
+  if ((int)threadIdx.x < nw)
+    reinterpret_cast<int4 *>(s_rec)[threadIdx.x] = src[threadIdx.x];
+}
+
+// Gaussian weights of one segment at a tile's axis points. Point a: x
+// (a < 8), y (8 <= a < 16), z (16 <= a < 32) of the tile starting at
+// (px0, py0, pz0). w[a] = g (monopole), w[32 + a] = gD (dipole), w[64 + a]
+// = the gradient factor -xx / sigma_dip^2, zero off the stencil; xx is
+// formed in FP64. FP64 uses the expressions of k_prepare_tab; FP32 rounds
+// the exponent, formed in FP64 in base 2, before the hardware exp2.
+constexpr int kTileW = 104;  // weights (96) + coefficients (8) per record
+template <typename T>
+__device__ __forceinline__ void tile_weights(const GaussConst &g,
+                                             const SegRec<T> &r, int a, int px0,
+                                             int py0, int pz0, T *w) {
+  const int d = a < 8 ? 0 : (a < 16 ? 1 : 2);
+  const int p = d == 0 ? px0 + a : (d == 1 ? py0 + a - 8 : pz0 + a - 16);
+  const int n = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+  int i = p - ((r.org >> (10 * d)) & 1023);
+  if (i < 0) i += n;
+  T ga = 0, gd = 0, gr = 0;
+  if (i < g.ngd) {
+    const double x0 = d == 0 ? r.xx0[0] : (d == 1 ? r.xx0[1] : r.xx0[2]);
+    const double xx = x0 + (double)i * g.dx;
+    if constexpr (sizeof(T) == 8) {
+      ga = g.anorm * exp(-xx * xx / g.anorm2);
+      gd = g.anormdip * exp(-xx * xx / g.anormdip2);
+      gr = -xx / g.sigmadipsq;
+    } else {
+      const double x2 = xx * xx;
+      ga = (float)g.anorm * __builtin_amdgcn_exp2f((float)(-x2 * g.e2));
+      gd = (float)g.anormdip * __builtin_amdgcn_exp2f((float)(-x2 * g.ed2));
+      gr = (float)(xx * g.nrsd);
+    }
+  }
+  w[a] = ga;
+  w[32 + a] = gd;
+  w[64 + a] = gr;
+}
+
+// Weights (and, with CF, coefficients) of records s_rec[c0 .. c0 + nc) into
+// s_w[0 .. KC); rows nc .. KC are zero. 256 threads.
+template <typename T, int KC, bool CF>
+__device__ __forceinline__ void tile_chunk_weights(const GaussConst &g,
+                                                   const SegRec<T> *s_rec,
+                                                   int c0, int nc, int px0,
+                                                   int py0, int pz0,
+                                                   T (*s_w)[kTileW]) {
+#pragma unroll
+  for (int e = threadIdx.x; e < KC * 32; e += 256) {
+    const int c = e >> 5, a = e & 31;
+    if (c < nc) {
+      tile_weights<T>(g, s_rec[c0 + c], a, px0, py0, pz0, s_w[c]);
+    } else {
+      s_w[c][a] = 0;
+      s_w[c][32 + a] = 0;
+      s_w[c][64 + a] = 0;
+    }
+  }
+  if constexpr (CF) {
+    for (int e = threadIdx.x; e < KC * 8; e += 256) {
+      const int c = e >> 3, k = e & 7;
+      s_w[c][96 + k] = (c < nc && k < 6) ? s_rec[c0 + c].cf[k] : (T)0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
 // spread on the matrix cores: one workgroup per 8 x 8 x 16 tile,
 // four wavefronts of 16 (x, y) points each. Per component the tile is
 //   R[xy][z] = sum over candidates c and terms t of A_t[xy][c] * B_t[c][z]
@@ -339,10 +595,11 @@ __global__ void __launch_bounds__(256)
 // half torque: the same sum as k_spread). Each MFMA 16x16x4 step takes four
 // candidates of one term; lane l supplies A[xy = l % 16][c = l / 16] and
 // B[c = l / 16][z = l % 16] and holds R[xy = Mfma<T>::row(l, r)][z = l % 16].
-// Candidates outside a point's stencil contribute exact zeros (selects,
-// never products with stale LDS).
+// Weights are zero off a candidate's stencil and for the unused rows of a
+// round. Candidates are summed in tile-list order: no atomics, no zeroing
+// pass, every grid point written once.
 // ---------------------------------------------------------------------
-// Candidates staged per round: LDS for 8 workgroups per CU at ngd = 12.
+// Candidates per round (FP32 32, FP64 16: the same 13 KB of weights).
 template <typename T>
 constexpr int kMfmaChunk = sizeof(T) == 8 ? 16 : 32;
 
@@ -372,192 +629,63 @@ struct Mfma<double> {
   }
 };
 
-// Whether a stencil of ngd points starting at o overlaps the te points
-// starting at p0 on a periodic axis of n points (o, p0 in [0, n)).
-__device__ __forceinline__ bool stencil_meets(int o, int p0, int te, int ngd,
-                                             int n) {
-  int d = o - p0;
-  if (d < 0) d += n;
-  return d < te || d > n - ngd;
-}
-
-// Candidate lists of the 8 x 8 x 16 matrix-core tiles. tile_bins (wave 0):
-// the bins whose segments' stencil origins can reach tile (bx, by, bz)
-// (nbins <= 64), their first list entries and sizes; returns the largest
-// bin size. The whole workgroup calls it (one barrier).
-__device__ __forceinline__ int tile_bins(const SpreadGeom &G, int bx, int by,
-                                         int bz, const int *__restrict__ start,
-                                         int *s_bfirst, int *s_bcnt,
-                                         int *s_maxc) {
-  if (threadIdx.x < 64) {
-    const int cx = min(G.hx + 1, G.nbx), cy = min(G.hy + 1, G.nby),
-              cz = min(G.hz + 1, G.nbz);
-    const int t = threadIdx.x;
-    int n = 0;
-    if (t < cx * cy * cz) {
-      int dz = t / (cx * cy), dy = (t / cx) % cy, dx = t % cx;
-      int ix = ((bx - G.hx + dx) % G.nbx + G.nbx) % G.nbx;
-      int iy = ((by - G.hy + dy) % G.nby + G.nby) % G.nby;
-      int iz = ((bz - G.hz + dz) % G.nbz + G.nbz) % G.nbz;
-      int b = spread_bin(ix, iy, iz, G.nbx, G.nby, G.nbz);
-      const int f = start[b];
-      n = start[b + 1] - f;
-      s_bfirst[t] = f;
-      s_bcnt[t] = n;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) n = max(n, __shfl_xor(n, off, 64));
-    if (t == 0) *s_maxc = n;
-  }
-  __syncthreads();
-  return *s_maxc;
-}
-
-// One pass of the tile's candidates: thread t takes entry k0 + t % K of bin
-// t / K (K = 256 / nbins slots per bin and pass), so the candidate order
-// is (pass, bin, position in bin). Keeps the segments whose stencil meets
-// the tile starting at (px0, py0, pz0), in that order (s_kid: segment,
-// s_korg: origin packed 10 bits per axis), and returns how many. The bins
-// over-include; the kept list skips only segments that contribute exact
-// zeros to the tile.
-__device__ __forceinline__ int tile_filter_pass(
-    const SpreadGeom &G, int px0, int py0, int pz0, int k0, int K, int nbins,
-    const int2 *__restrict__ idorg, const int *s_bfirst, const int *s_bcnt,
-    int *s_wsum, int *s_kid, int *s_korg) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int b = threadIdx.x / K, k = k0 + threadIdx.x - b * K;
-  bool keep = false;
-  int2 e = make_int2(0, 0);
-  if (b < nbins && k < s_bcnt[b]) {
-    e = idorg[s_bfirst[b] + k];
-    keep = stencil_meets(e.y & 1023, px0, 8, G.ngd, G.nx) &&
-           stencil_meets((e.y >> 10) & 1023, py0, 8, G.ngd, G.ny) &&
-           stencil_meets(e.y >> 20, pz0, 16, G.ngd, G.nz);
-  }
-  const unsigned long long kb = __ballot(keep);
-  const int below = __popcll(kb & ((1ull << lane) - 1ull));
-  if (lane == 0) s_wsum[wave] = __popcll(kb);
-  __syncthreads();
-  int off = 0;
-  for (int w = 0; w < wave; ++w) off += s_wsum[w];
-  const int nk = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
-  if (keep) {
-    s_kid[off + below] = e.x;
-    s_korg[off + below] = e.y;
-  }
-  __syncthreads();
-  return nk;
-}
-
-// Copies the records (WORDS units of U each) of segments s_kid[c0 ..
-// c0 + n) to dst[0 .. n) with independent loads; 256 threads.
-template <typename U, int WORDS>
-__device__ __forceinline__ void stage_records(const U *__restrict__ src,
-                                              const int *s_kid, int c0, int n,
-                                              U *dst) {
-  for (int e = threadIdx.x; e < n * WORDS; e += 256) {
-    const int c = e / WORDS, w = e - c * WORDS;
-    dst[e] = src[(size_t)s_kid[c0 + c] * WORDS + w];
-  }
-}
-
-// Record of R reals staged as 16-byte words when it is made of whole words.
-template <typename T, int R>
-__device__ __forceinline__ void stage_reals(const T *__restrict__ src,
-                                            const int *s_kid, int c0, int n,
-                                            T *dst) {
-  if constexpr ((R * sizeof(T)) % 16 == 0)
-    stage_records<int4, R * sizeof(T) / 16>(
-        reinterpret_cast<const int4 *>(src), s_kid, c0, n,
-        reinterpret_cast<int4 *>(dst));
-  else
-    stage_records<T, R>(src, s_kid, c0, n, dst);
-}
-
-template <typename T, int NGD>
+template <typename T>
 __global__ void __launch_bounds__(256)
-    k_spread_mfma(const int *__restrict__ act, SpreadGeom G,
-                  const T *__restrict__ tab, const T *__restrict__ coef,
-                  const int *__restrict__ start,
-                  const int2 *__restrict__ idorg, T *__restrict__ R) {
+    k_spread_mfma(const int *__restrict__ act, SpreadGeom G, GaussConst g,
+                  const SegRec<T> *__restrict__ slab,
+                  const SegRec<T> *__restrict__ pool,
+                  const int *__restrict__ tcnt, const int *__restrict__ tovf,
+                  T *__restrict__ R) {
   typedef typename Mfma<T>::v4 v4;
-  constexpr int kMfmaChunk = fs::kMfmaChunk<T>;
-  constexpr int ngd = NGD;
-  constexpr int kTab = 9 * NGD;  // table record [9][ngd]
+  constexpr int KC = kMfmaChunk<T>;
   if (act && !*act) return;  // batched sweep: suspension converged
-  __shared__ int s_bcnt[kMaxBinsPerTile];
-  __shared__ int s_bfirst[kMaxBinsPerTile];
-  __shared__ int s_maxc;
-  __shared__ int s_wsum[4];
-  // candidates of one pass that meet the tile, in candidate order
-  __shared__ int s_kid[256], s_korg[256];
-  __shared__ __align__(16) T s_tab[kMfmaChunk][kTab];
-  __shared__ __align__(16) T s_coef[kMfmaChunk][kCoef];
+  __shared__ SegRec<T> s_rec[kTileCap];
+  __shared__ T s_w[KC][kTileW];
 
-  const int ntiles = G.nbx * G.nby * G.nbz;
-  int bid = blockIdx.x;  // XCD-aware order, as k_spread
-  if ((ntiles & 7) == 0) bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
-  const int bx = bid % G.nbx;
-  const int by = (bid / G.nbx) % G.nby;
-  const int bz = bid / (G.nbx * G.nby);
-  const int maxc = tile_bins(G, bx, by, bz, start, s_bfirst, s_bcnt, &s_maxc);
-  const int nbins = min(G.hx + 1, G.nbx) * min(G.hy + 1, G.nby) *
-                    min(G.hz + 1, G.nbz);
-  const int K = 256 / nbins;
+  int bx, by, bz;
+  tile_of_block(G, blockIdx.x, &bx, &by, &bz);
+  const int tile = bx + G.nbx * (by + G.nby * bz);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int q = lane >> 4, i16 = lane & 15;
   const int xy = wave * 16 + i16;  // this lane's A row
+  const int lx = xy >> 3, ly = xy & 7, lz = i16;
   const int px0 = bx * 8, py0 = by * 8, pz0 = bz * 16;
-  const int px = px0 + (xy >> 3), py = py0 + (xy & 7), pz = pz0 + i16;
+  // the slab is copied while the count is still on its way
+  stage_tile_records<T>(slab, pool, tile, 0, 0, 0, s_rec);
+  const int cnt = tcnt[tile];
+  const int ovf = cnt > kTileCap ? tovf[tile] : 0;
   v4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
 
-  for (int k0 = 0; k0 < maxc; k0 += K) {
-    const int nk = tile_filter_pass(G, px0, py0, pz0, k0, K, nbins, idorg,
-                                    s_bfirst, s_bcnt, s_wsum, s_kid, s_korg);
-    for (int c0 = 0; c0 < nk; c0 += kMfmaChunk) {
-      const int nc = min(kMfmaChunk, nk - c0);
-      // one level of 16-byte loads: table and coefficient records
-      stage_reals<T, kTab>(tab, s_kid, c0, nc, &s_tab[0][0]);
-      stage_reals<T, kCoef>(coef, s_kid, c0, nc, &s_coef[0][0]);
+  for (int r0 = 0; r0 < cnt; r0 += kTileCap) {
+    const int nb = min(kTileCap, cnt - r0);
+    if (r0 > 0) {
+      __syncthreads();  // the last round still reads s_rec
+      stage_tile_records<T>(slab, pool, tile, ovf, r0, nb, s_rec);
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < nb; c0 += KC) {
+      const int nc = min(KC, nb - c0);
+      tile_chunk_weights<T, KC, true>(g, s_rec, c0, nc, px0, py0, pz0, s_w);
       __syncthreads();
       for (int c4 = 0; c4 < nc; c4 += 4) {
-        // lane group q takes candidate c4 + q; every read is in range
-        // (clamped indices) and the selects zero what lies outside
-        const int c = c4 + q;
-        const bool live = c < nc;
-        const int cs = live ? c : 0;
-        const int og = s_korg[c0 + cs];
-        int ix = px - (og & 1023);
-        if (ix < 0) ix += G.nx;
-        int iy = py - ((og >> 10) & 1023);
-        if (iy < 0) iy += G.ny;
-        int iz = pz - (og >> 20);
-        if (iz < 0) iz += G.nz;
-        const bool in = live && ix < ngd && iy < ngd;
-        const bool zin = live && iz < ngd;
-        const int ixc = ix < ngd ? ix : 0, iyc = iy < ngd ? iy : 0,
-                  izc = iz < ngd ? iz : 0;
-        const T *tb = s_tab[cs];
-        const T *cf = s_coef[cs];
-        const T gxy = tb[ixc] * tb[ngd + iyc];
-        const T dxy = tb[3 * ngd + ixc] * tb[4 * ngd + iyc];
-        const T xx = tb[6 * ngd + ixc], yy = tb[7 * ngd + iyc];
-        const T gz = tb[2 * ngd + izc], gdz = tb[5 * ngd + izc],
-                grz = tb[8 * ngd + izc];
+        // lane group q takes candidate c4 + q (rows past nc are zero)
+        const T *w = s_w[c4 + q];
+        const T *cf = w + 96;
+        const T gxy = w[lx] * w[8 + ly], dxy = w[32 + lx] * w[40 + ly];
+        const T xx = w[64 + lx], yy = w[72 + ly];
+        const T gz = w[16 + lz], gdz = w[48 + lz], grz = w[80 + lz];
         const T ax = cf[5] * yy, ay = -cf[5] * xx, az = cf[4] * xx - cf[3] * yy;
-        const T zero = 0;
-        const T b0 = zin ? gz : zero, b1 = zin ? gdz : zero,
-                b2 = zin ? gdz * grz : zero;
-        acc0 = Mfma<T>::step(in ? cf[0] * gxy : zero, b0, acc0);
-        acc1 = Mfma<T>::step(in ? cf[1] * gxy : zero, b0, acc1);
-        acc2 = Mfma<T>::step(in ? cf[2] * gxy : zero, b0, acc2);
-        acc0 = Mfma<T>::step(in ? ax * dxy : zero, b1, acc0);
-        acc1 = Mfma<T>::step(in ? ay * dxy : zero, b1, acc1);
-        acc2 = Mfma<T>::step(in ? az * dxy : zero, b1, acc2);
-        acc0 = Mfma<T>::step(in ? -cf[4] * dxy : zero, b2, acc0);
-        acc1 = Mfma<T>::step(in ? cf[3] * dxy : zero, b2, acc1);
+        const T b2 = gdz * grz;
+        acc0 = Mfma<T>::step(cf[0] * gxy, gz, acc0);
+        acc1 = Mfma<T>::step(cf[1] * gxy, gz, acc1);
+        acc2 = Mfma<T>::step(cf[2] * gxy, gz, acc2);
+        acc0 = Mfma<T>::step(ax * dxy, gdz, acc0);
+        acc1 = Mfma<T>::step(ay * dxy, gdz, acc1);
+        acc2 = Mfma<T>::step(az * dxy, gdz, acc2);
+        acc0 = Mfma<T>::step(-cf[4] * dxy, b2, acc0);
+        acc1 = Mfma<T>::step(cf[3] * dxy, b2, acc1);
       }
-      __syncthreads();  // the next round overwrites the staged candidates
+      __syncthreads();  // the next round overwrites the weights
     }
   }
   // lane holds R[xy = wave*16 + Mfma<T>::row(lane, r)][z = i16]: 16 lanes
@@ -566,14 +694,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int oxy = wave * 16 + Mfma<T>::row(lane, r);
-    const size_t o = ((size_t)(bx * 8 + (oxy >> 3)) * G.ny + by * 8 + (oxy & 7)) *
+    const size_t o = ((size_t)(px0 + (oxy >> 3)) * G.ny + py0 + (oxy & 7)) *
                          G.nz + pz0 + i16;
     R[o] = acc0[r];
     R[N + o] = acc1[r];
     R[2 * N + o] = acc2[r];
   }
 }
-
 // ---------------------------------------------------------------------
 // FFT passes. Half spectrum layout C[c][kz][y][x] (x fastest): the z pass
 // writes it transposed so the x pass reads contiguous lines and the y pass
@@ -1044,6 +1171,207 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---------------------------------------------------------------------
+// interpolation on the matrix cores: the transpose of k_spread_mfma. One
+// workgroup per 8 x 8 x 16 tile reads the tile's three velocity grids once
+// (each lane four consecutive z of one (x, y) column per component) and,
+// for the segments of its tile list (16 per round), contracts them with
+// each segment's weights:
+//   P_m[xy][c] = sum_z u_d[xy][z] b_m,c[z]   (per wave 16 xy rows x 16
+//   segments, v_mfma 16x16x4 over z; 8 products m: u_0..2 with g_z,
+//   u_0..2 with gD_z, u_0..1 with gD_z grad_z)
+// then weights each (x, y) row with the segment's x and y factors, sums
+// the lane's rows, the four lane groups and the four waves in a fixed
+// order, and stores the tile's 6 partial sums per segment in
+// part[segment][component][slot] (slot = the tile's offset from the
+// segment's origin tile). k_interp_sum adds a segment's slots in slot
+// order: no atomics, results independent of scheduling.
+// ---------------------------------------------------------------------
+// Tiles of te points that a stencil of ngd points can meet along an axis.
+__host__ __device__ constexpr int stencil_tiles(int ngd, int te) {
+  return (ngd + te - 2) / te + 1;
+}
+template <int NGD>
+constexpr int kInterpSlots = stencil_tiles(NGD, 8) * stencil_tiles(NGD, 8) *
+                             stencil_tiles(NGD, 16);
+
+// Four consecutive reals at p (16-byte aligned).
+template <typename T>
+__device__ __forceinline__ void ld4(const T *p, T *out) {
+  ld16(p, out);
+  if constexpr (sizeof(T) == 8) ld16(p + 2, out + 2);
+}
+
+template <typename T, int NGD>
+__global__ void __launch_bounds__(256)
+    k_interp_mfma(const int *__restrict__ act, SpreadGeom G, GaussConst g,
+                  const SegRec<T> *__restrict__ slab,
+                  const SegRec<T> *__restrict__ pool,
+                  const int *__restrict__ tcnt, const int *__restrict__ tovf,
+                  const T *__restrict__ R, T *__restrict__ part) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int KC = 16;
+  constexpr int SX = stencil_tiles(NGD, 8), SZ = stencil_tiles(NGD, 16);
+  constexpr int NSLOT = kInterpSlots<NGD>;
+  if (act && !*act) return;  // batched sweep: suspension converged
+  __shared__ SegRec<T> s_rec[kTileCap];
+  __shared__ T s_w[KC][kTileW];
+  __shared__ T s_red[4][KC][6];
+
+  int bx, by, bz;
+  tile_of_block(G, blockIdx.x, &bx, &by, &bz);
+  const int tile = bx + G.nbx * (by + G.nby * bz);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = lane >> 4, i16 = lane & 15;
+  const int px0 = bx * 8, py0 = by * 8, pz0 = bz * 16;
+  // A operands: u_d[xy = wave*16 + i16][z = pz0 + 4q + k] for K step k
+  // (lane group q supplies z 4q .. 4q + 3)
+  const size_t N = (size_t)G.nx * G.ny * G.nz;
+  T u[3][4];
+  {
+    const int xy = wave * 16 + i16;
+    const size_t o = ((size_t)(px0 + (xy >> 3)) * G.ny + py0 + (xy & 7)) * G.nz +
+                     pz0 + 4 * q;
+    ld4(R + o, u[0]);
+    ld4(R + N + o, u[1]);
+    ld4(R + 2 * N + o, u[2]);
+  }
+  stage_tile_records<T>(slab, pool, tile, 0, 0, 0, s_rec);
+  const int cnt = tcnt[tile];
+  const int ovf = cnt > kTileCap ? tovf[tile] : 0;
+
+  for (int r0 = 0; r0 < cnt; r0 += kTileCap) {
+    const int nb = min(kTileCap, cnt - r0);
+    if (r0 > 0) {
+      __syncthreads();  // the last round still reads s_rec
+      stage_tile_records<T>(slab, pool, tile, ovf, r0, nb, s_rec);
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < nb; c0 += KC) {
+      const int nc = min(KC, nb - c0);
+      tile_chunk_weights<T, KC, false>(g, s_rec, c0, nc, px0, py0, pz0, s_w);
+      __syncthreads();
+      // B operands: this lane's column is candidate i16 (zero weights past nc)
+      const T *w = s_w[i16];
+      const v4 z4 = {0, 0, 0, 0};
+      v4 pg0 = z4, pg1 = z4, pg2 = z4, pd0 = z4, pd1 = z4, pd2 = z4,
+         pr0 = z4, pr1 = z4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int z = 4 * q + k;
+        const T gz = w[16 + z], gdz = w[48 + z];
+        const T grz = gdz * w[80 + z];
+        pg0 = Mfma<T>::step(u[0][k], gz, pg0);
+        pg1 = Mfma<T>::step(u[1][k], gz, pg1);
+        pg2 = Mfma<T>::step(u[2][k], gz, pg2);
+        pd0 = Mfma<T>::step(u[0][k], gdz, pd0);
+        pd1 = Mfma<T>::step(u[1][k], gdz, pd1);
+        pd2 = Mfma<T>::step(u[2][k], gdz, pd2);
+        pr0 = Mfma<T>::step(u[0][k], grz, pr0);
+        pr1 = Mfma<T>::step(u[1][k], grz, pr1);
+      }
+      // lane holds P[xy = wave*16 + row(lane, r)][candidate i16]: weight
+      // each row with the candidate's x/y factors
+      T v0 = 0, v1 = 0, v2 = 0, w0 = 0, w1 = 0, w2 = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rxy = wave * 16 + Mfma<T>::row(lane, r);
+        const int x = rxy >> 3, y = rxy & 7;
+        const T gxy = w[x] * w[8 + y], dxy = w[32 + x] * w[40 + y];
+        const T xx = w[64 + x], yy = w[72 + y];
+        v0 += gxy * pg0[r];
+        v1 += gxy * pg1[r];
+        v2 += gxy * pg2[r];
+        w0 += dxy * (yy * pd2[r] - pr1[r]);
+        w1 += dxy * (pr0[r] - xx * pd2[r]);
+        w2 += dxy * (xx * pd1[r] - yy * pd0[r]);
+      }
+      // the four lane groups (rows) of the wave, then the four waves
+#pragma unroll
+      for (int off = 16; off < 64; off <<= 1) {
+        v0 += __shfl_xor(v0, off, 64);
+        v1 += __shfl_xor(v1, off, 64);
+        v2 += __shfl_xor(v2, off, 64);
+        w0 += __shfl_xor(w0, off, 64);
+        w1 += __shfl_xor(w1, off, 64);
+        w2 += __shfl_xor(w2, off, 64);
+      }
+      if (q == 0) {
+        T *sr = s_red[wave][i16];
+        sr[0] = v0;
+        sr[1] = v1;
+        sr[2] = v2;
+        sr[3] = w0;
+        sr[4] = w1;
+        sr[5] = w2;
+      }
+      __syncthreads();
+      if (threadIdx.x < KC * 6) {
+        const int c = threadIdx.x / 6, m = threadIdx.x - 6 * c;
+        if (c < nc) {
+          const T sum = ((s_red[0][c][m] + s_red[1][c][m]) + s_red[2][c][m]) +
+                        s_red[3][c][m];
+          const SegRec<T> &rc = s_rec[c0 + c];
+          int sx = bx - (rc.org & 1023) / 8, sy = by - ((rc.org >> 10) & 1023) / 8,
+              sz = bz - (rc.org >> 20) / 16;
+          if (sx < 0) sx += G.nbx;
+          if (sy < 0) sy += G.nby;
+          if (sz < 0) sz += G.nbz;
+          part[((size_t)rc.id * 6 + m) * NSLOT + (sx * SX + sy) * SZ + sz] = sum;
+        }
+      }
+      __syncthreads();  // the next round overwrites weights and sums
+    }
+  }
+}
+
+// One thread per (segment, component): adds the segment's tile partials in
+// slot order (only the tiles its stencil meets, the ones k_interp_mfma
+// wrote) and scales them: V = dx^3 sum, Omega = -dx^3 / 2 sum
+// (particle_velocities / particle_rotations, FCMfunctions.cpp:740-808).
+template <typename T, int NGD>
+__global__ void k_interp_sum(const int *__restrict__ act, int np, SpreadGeom G,
+                             double dx3, const int *__restrict__ origin,
+                             const T *__restrict__ part,
+                             double *__restrict__ V, double *__restrict__ W) {
+  constexpr int SX = stencil_tiles(NGD, 8), SZ = stencil_tiles(NGD, 16);
+  constexpr int NSLOT = kInterpSlots<NGD>;
+  if (act && !*act) return;  // batched sweep: suspension converged
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = e / 6, m = e - 6 * s;
+  if (s >= np) return;
+  const int ox = origin[3 * s], oy = origin[3 * s + 1], oz = origin[3 * s + 2];
+  // every slot is read (in range); the ones no tile wrote are not added
+  T v[NSLOT];
+  const T *pp = part + ((size_t)s * 6 + m) * NSLOT;
+#pragma unroll
+  for (int k = 0; k < NSLOT; ++k) v[k] = pp[k];
+  double acc = 0;
+#pragma unroll
+  for (int sx = 0; sx < SX; ++sx) {
+    int tx = ox / 8 + sx;
+    if (tx >= G.nbx) tx -= G.nbx;
+    const bool mx = sx < G.nbx && stencil_meets(ox, tx * 8, 8, NGD, G.nx);
+#pragma unroll
+    for (int sy = 0; sy < SX; ++sy) {
+      int ty = oy / 8 + sy;
+      if (ty >= G.nby) ty -= G.nby;
+      const bool my = sy < G.nby && stencil_meets(oy, ty * 8, 8, NGD, G.ny);
+#pragma unroll
+      for (int sz = 0; sz < SZ; ++sz) {
+        int tz = oz / 16 + sz;
+        if (tz >= G.nbz) tz -= G.nbz;
+        const bool mz = sz < G.nbz && stencil_meets(oz, tz * 16, 16, NGD, G.nz);
+        if (mx && my && mz) acc += (double)v[(sx * SX + sy) * SZ + sz];
+      }
+    }
+  }
+  if (m < 3)
+    V[3 * s + m] = dx3 * acc;
+  else
+    W[3 * s + m - 3] = -0.5 * dx3 * acc;
+}
+
 __global__ void k_friction(const int *__restrict__ act, int n3, const double *__restrict__ F,
                            const double *__restrict__ Tq, double a1, double a2,
                            double *__restrict__ V, double *__restrict__ W) {
@@ -1239,7 +1567,8 @@ int FcmImpl::init(const fs_fcm_params &pp, int maxnp, int prec) {
   FS_TRY(tab.alloc((size_t)maxnp * 9 * p.ngd * tsz));
   FS_TRY(origin.alloc((size_t)maxnp * 3 * sizeof(int)));
   FS_TRY(coef.alloc((size_t)maxnp * kCoef * tsz));
-  FS_TRY(idorg.alloc((size_t)std::max(maxnp, 1) * sizeof(int2)));
+  FS_TRY(srec.alloc((size_t)std::max(maxnp, 1) * rec_bytes(tsz)));
+  FS_TRY(sorg.alloc((size_t)std::max(maxnp, 1) * sizeof(int)));
   FS_TRY(bin.alloc((size_t)maxnp * sizeof(int)));
   FS_TRY(slot.alloc((size_t)maxnp * sizeof(int)));
   FS_TRY(ids.alloc((size_t)maxnp * sizeof(int)));
@@ -1317,7 +1646,17 @@ int FcmImpl::set_geometry() {
   geom.hy = (p.ngd - 1 + ty - 1) / ty;
   geom.hz = (p.ngd - 1 + tz - 1) / tz;
   nbins = geom.nbx * geom.nby * geom.nbz;
-  FS_TRY(bin_count.alloc((size_t)nbins * sizeof(int)));
+  if (mfma_spread) {
+    const size_t nslot = (size_t)stencil_tiles(p.ngd, 8) *
+                         stencil_tiles(p.ngd, 8) * stencil_tiles(p.ngd, 16);
+    FS_TRY(ipart.alloc((size_t)max_np * 6 * tsz * nslot));
+    FS_TRY(tslab.alloc((size_t)nbins * kTileCap * rec_bytes(tsz)));
+    FS_TRY(tcnt.alloc((size_t)nbins * sizeof(int)));
+    FS_TRY(tovf.alloc((size_t)nbins * sizeof(int)));
+    FS_TRY(tpool.alloc((size_t)max_np * nslot * rec_bytes(tsz)));
+  }
+  // one more counter after the bins: the tile lists' pool top
+  FS_TRY(bin_count.alloc((size_t)(nbins + 1) * sizeof(int)));
   FS_TRY(bin_start.alloc((size_t)(nbins + 1) * sizeof(int)));
   FS_TRY(scan_tmp.alloc((size_t)(nbins / kScanBlock + 4) * sizeof(int)));
   // FFT pass geometry: z rows per workgroup (even divisor of nx), y-pass run
@@ -1353,7 +1692,13 @@ void FcmImpl::release() {
   bin.release();
   slot.release();
   ids.release();
-  idorg.release();
+  srec.release();
+  sorg.release();
+  tslab.release();
+  tcnt.release();
+  tovf.release();
+  tpool.release();
+  ipart.release();
   bin_count.release();
   bin_start.release();
   scan_tmp.release();
@@ -1378,6 +1723,9 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   g.anorm2 = 2.0 * sigmasq;
   g.ngd = p.ngd;
   g.ngdh = p.ngd / 2;
+  g.e2 = 1.4426950408889634074 / g.anorm2;
+  g.ed2 = 1.4426950408889634074 / g.anormdip2;
+  g.nrsd = -1.0 / sigmadipsq;
   g.nx = p.nx;
   g.ny = p.ny;
   g.nz = p.nz;
@@ -1398,10 +1746,11 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   };
   mark(0);
   // bin counting sort
-  hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbins, 256)), dim3(256), 0, st, act,
-                     bin_count.as<int>(), nbins, 0);
-  hipLaunchKernelGGL(k_prepare_tab<T>, dim3(ceil_div((long)np * 3 * p.ngd, 256)),
-                     dim3(256), 0, st, act, np, Y, g, tab.as<T>());
+  hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbins + 1, 256)), dim3(256), 0, st,
+                     act, bin_count.as<int>(), nbins + 1, 0);
+  if (!mfma_spread)  // the matrix-core kernels form their weights per tile
+    hipLaunchKernelGGL(k_prepare_tab<T>, dim3(ceil_div((long)np * 3 * p.ngd, 256)),
+                       dim3(256), 0, st, act, np, Y, g, tab.as<T>());
   hipLaunchKernelGGL(k_prepare<T>, dim3(ceil_div(np, 64)), dim3(64), 0, st, act, np,
                      Y, F, Tq, g, geom.tx, geom.ty, geom.tz, geom.nbx, geom.nby,
                      geom.nbz, origin.as<int>(), coef.as<T>(), bin.as<int>(),
@@ -1411,20 +1760,24 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   hipLaunchKernelGGL(k_scatter, dim3(ceil_div(np, 256)), dim3(256), 0, st, np,
                      bin.as<int>(), slot.as<int>(), bin_start.as<int>(),
                      ids.as<int>());
-  hipLaunchKernelGGL(k_sort_bins, dim3(ceil_div(nbins, 64)), dim3(64), 0, st,
-                     nbins, bin_start.as<int>(), ids.as<int>(),
-                     (const int *)origin.as<int>(),
-                     mfma_spread ? idorg.as<int2>() : (int2 *)nullptr);
-  mark(1);
+  hipLaunchKernelGGL(k_sort_bins<T>, dim3(ceil_div(nbins, 64)), dim3(64), 0, st,
+                     act, nbins, bin_start.as<int>(), ids.as<int>(),
+                     (const int *)origin.as<int>(), Y, F, Tq, p.dx, p.ngd / 2,
+                     mfma_spread ? srec.as<SegRec<T>>() : (SegRec<T> *)nullptr,
+                     mfma_spread ? sorg.as<int>() : (int *)nullptr);
   const int ntiles = nbins;
+  if (mfma_spread)
+    hipLaunchKernelGGL(k_tile_lists<T>, dim3(ceil_div(ntiles, 4)), dim3(256), 0, st,
+                       act, geom, bin_start.as<int>(), (const int *)sorg.as<int>(),
+                       srec.as<SegRec<T>>(),
+                       tslab.as<SegRec<T>>(), tcnt.as<int>(), tovf.as<int>(),
+                       tpool.as<SegRec<T>>(), bin_count.as<int>() + nbins);
+  mark(1);
   bool okz = true;
   if (mfma_spread)
-    okz = with_NGD(p.ngd, [&](auto g) {
-      constexpr int NG = decltype(g)::value;
-      hipLaunchKernelGGL((k_spread_mfma<T, NG>), dim3(ntiles), dim3(256), 0, st,
-                         act, geom, tab.as<T>(), coef.as<T>(),
-                         bin_start.as<int>(), idorg.as<int2>(), R.as<T>());
-    });
+    hipLaunchKernelGGL(k_spread_mfma<T>, dim3(ntiles), dim3(256), 0, st, act, geom,
+                       g, tslab.as<SegRec<T>>(), tpool.as<SegRec<T>>(),
+                       tcnt.as<int>(), tovf.as<int>(), R.as<T>());
   if (!mfma_spread)
     okz = with_TZ(geom.tz, [&](auto z) {
       constexpr int TZ = decltype(z)::value;
@@ -1494,12 +1847,22 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   FS_LAUNCHED();
   mark(5);
   double dx3 = p.dx * p.dx * p.dx;
-  bool okn = with_NGD(p.ngd, [&](auto g) {
-    constexpr int NG = decltype(g)::value;
-    hipLaunchKernelGGL((k_interp<T, NG>),
-                       dim3(8 * ceil_div(ceil_div(np, 4), 8)), dim3(256), 0,
-                       st, act, np, geom, dx3, tab.as<T>(), origin.as<int>(),
-                       ids.as<int>(), R.as<T>(), V, W);
+  bool okn = with_NGD(p.ngd, [&](auto ng) {
+    constexpr int NG = decltype(ng)::value;
+    if (mfma_spread) {
+      hipLaunchKernelGGL((k_interp_mfma<T, NG>), dim3(ntiles), dim3(256), 0,
+                         st, act, geom, g, tslab.as<SegRec<T>>(),
+                         tpool.as<SegRec<T>>(), tcnt.as<int>(), tovf.as<int>(),
+                         R.as<T>(), ipart.as<T>());
+      hipLaunchKernelGGL((k_interp_sum<T, NG>), dim3(ceil_div(6L * np, 256)),
+                         dim3(256), 0, st, act, np, geom, dx3,
+                         origin.as<int>(), ipart.as<T>(), V, W);
+    } else {
+      hipLaunchKernelGGL((k_interp<T, NG>),
+                         dim3(8 * ceil_div(ceil_div(np, 4), 8)), dim3(256), 0,
+                         st, act, np, geom, dx3, tab.as<T>(), origin.as<int>(),
+                         ids.as<int>(), R.as<T>(), V, W);
+    }
   });
   FS_CHECK(okn, FS_ERR_ARG, "interp dispatch failed");
   FS_LAUNCHED();

@@ -42,7 +42,11 @@ struct FcmImpl {
   const int *act = nullptr;
   dbuf R, C, twx, twy, twz;
   dbuf tab, origin, coef, bin, slot, ids, bin_count, bin_start, scan_tmp;
-  dbuf idorg;  // (segment, packed stencil origin) in bin order
+  // matrix-core path: segment records and packed origins in bin order,
+  // per-tile record lists (slab of kTileCap per tile, count, pool block of
+  // the rest, the pool) and the interpolation's partial sums
+  // [segment][component][tile slot]
+  dbuf srec, sorg, tslab, tcnt, tovf, tpool, ipart;
   hipEvent_t ev[8] = {};
 
   int init(const fs_fcm_params &pp, int maxnp, int prec);

@@ -84,9 +84,10 @@ typedef struct fs_fcm_tuning {
   int spread_tile;  /* spread tile edge in x and y (4, 8 or 16)        */
   int spread_zrun;  /* largest spread tile extent in z (4..16)         */
   int fft_xlines;   /* x lines (per component) per workgroup, x pass  */
-  int spread_mfma;  /* 1 spreads with matrix-core products on 8 x 8 x 16
-                       tiles (grids with nx, ny multiples of 8, nz of 16,
-                       nz >= 16 + ngd), 0 with the vector gather. In
+  int spread_mfma;  /* 1 spreads and interpolates with matrix-core
+                       products on 8 x 8 x 16 tiles (grids with nx, ny
+                       multiples of 8, nz of 16, nz >= 16 + ngd), 0 with
+                       the vector gather and one wavefront per segment. In
                        fs_fcm_set_tuning a negative value keeps the current
                        setting, as do values <= 0 of the other fields. */
 } fs_fcm_tuning;

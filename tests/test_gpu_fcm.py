@@ -19,6 +19,11 @@ CASES = [
     # stencils wrapping onto both ends of a spread tile
     ((16, 16, 16), 12, None),
     ((24, 24, 24), 20, "ref"),
+    # matrix-core tiles: fewer tiles per axis than a stencil can meet (the
+    # stencil meets one tile at both ends), reference resolution (ngd 20)
+    ((16, 16, 32), 12, None),
+    ((24, 24, 48), 20, "ref"),
+    ((48, 48, 48), 20, "ref"),
     # 512-point lines (the 512^3 suspension's FFT length)
     ((512, 16, 24), 12, None),
     # 128/256/512-point lines take compile-time power-of-two plans
