@@ -50,8 +50,10 @@ void set_error(const char *fmt, ...);
     }                                                                      \
   } while (0)
 
+// Complex value, aligned to its size so that LDS and global accesses of one
+// value are single 8-byte (FP32) or 16-byte (FP64) instructions.
 template <typename T>
-struct cx {
+struct alignas(2 * sizeof(T)) cx {
   T x, y;
 };
 

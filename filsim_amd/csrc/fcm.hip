@@ -922,12 +922,12 @@ __global__ void __launch_bounds__(1024)
   if (vec) {
     for (int e = threadIdx.x * VC; e < B * n; e += blockDim.x * VC) {
       int m = dB.q(e), col = e - m * B;
-      ld16c(base + (size_t)m * nx + col, a + e);
+      ld16c(base + (size_t)m * nx + col, a + fft_rowpad(m) * B + col);
     }
   } else {
     for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
       int m = dB.q(e), col = e - m * B;
-      a[e] = base[(size_t)m * nx + col];
+      a[fft_rowpad(m) * B + col] = base[(size_t)m * nx + col];
     }
   }
   __syncthreads();
@@ -938,12 +938,12 @@ __global__ void __launch_bounds__(1024)
   if (vec) {
     for (int e = threadIdx.x * VC; e < B * n; e += blockDim.x * VC) {
       int m = dB.q(e), col = e - m * B;
-      st16c(base + (size_t)m * nx + col, a + e);
+      st16c(base + (size_t)m * nx + col, a + fft_rowpad(m) * B + col);
     }
   } else {
     for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
       int m = dB.q(e), col = e - m * B;
-      base[(size_t)m * nx + col] = a[e];
+      base[(size_t)m * nx + col] = a[fft_rowpad(m) * B + col];
     }
   }
 }
@@ -1801,7 +1801,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   ok &= with_EL(ply, [&](auto e, auto l) {
     constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
     hipLaunchKernelGGL((k_fft_y<T, EE, LL>), dim3(nh * (p.nx / yB), 3),
-                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st, act,
+                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * fft_rowpad_len(p.ny)), st, act,
                        C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 0,
                        make_div(yB));
   });
@@ -1831,7 +1831,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   ok = with_EL(ply, [&](auto e, auto l) {
     constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
     hipLaunchKernelGGL((k_fft_y<T, EE, LL>), dim3(nh * (p.nx / yB), 3),
-                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * p.ny), st, act,
+                       dim3(p.ny / EE * yB), csz * (p.ny + (size_t)yB * fft_rowpad_len(p.ny)), st, act,
                        C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 1,
                        make_div(yB));
   });
@@ -1946,9 +1946,9 @@ int FcmImpl::set_tuning(const fs_fcm_tuning &t) {
   FS_CHECK(nt.fft_batch >= 1 && nt.fft_batch <= 64 && nt.fft_xlines >= 1 &&
                nt.fft_xlines <= 16,
            FS_ERR_ARG, "fft_batch must be in [1,64], fft_xlines in [1,16]");
-  // LDS: y pass holds (B+1)*ny complex, x pass nx + 3Q*(padded nx)
+  // LDS: y pass holds ny + B*(row-padded ny) complex, x pass nx + 3Q*(padded nx)
   size_t cs = (size_t)tsz * 2;
-  size_t lds = std::max(cs * (p.ny + (size_t)nt.fft_batch * p.ny),
+  size_t lds = std::max(cs * (p.ny + (size_t)nt.fft_batch * fft_rowpad_len(p.ny)),
                         cs * (p.nx + 3 * (size_t)nt.fft_xlines *
                                          fft_padded_len(p.nx)));
   FS_CHECK(lds <= 160 * 1024, FS_ERR_ARG, "tuning needs %zu B of LDS", lds);

@@ -222,22 +222,50 @@ __device__ __forceinline__ void dftR(cx<T> *v, bool inv) {
 }
 
 // Padded LDS layout for contiguous sequences: element i of a sequence sits at
-// i + i/8. Early Stockham stages write with a lane stride of R elements
-// (d = j*R + r); unpadded, 16 (FP32) or 32 (FP64) lanes of a wave would hit
-// one bank. One pad element per 8 spreads those strided writes over the
-// banks, and contiguous reads stay conflict-free.
+// i + i / G (G = 2^FS_FFT_PAD_SHIFT elements per pad group). Early Stockham
+// stages write with a lane stride of R elements (d = j*R + r); unpadded,
+// 16 (FP32) or 32 (FP64) lanes of a wave would hit one bank. One pad
+// element per group spreads those strided writes over the banks, and
+// contiguous reads stay (nearly) conflict-free. Sequences follow each other
+// at fft_padded_len(n), rounded up to FS_FFT_STRIDE_MOD modulo 32 elements
+// when that is set, so that lanes reading the same position of different
+// sequences do not share banks.
+// The defaults were chosen with the LDS bank-conflict counter on gfx950
+// (64 banks of 4 bytes; an 8-byte access serves 32 lanes per cycle, a
+// 16-byte one 16): one pad per 16 elements and a stride of 18 modulo 32.
+#ifndef FS_FFT_PAD_SHIFT
+#define FS_FFT_PAD_SHIFT 4
+#endif
+#ifndef FS_FFT_STRIDE_MOD
+#define FS_FFT_STRIDE_MOD 18
+#endif
+constexpr int kFftPadGroup = 1 << FS_FFT_PAD_SHIFT;
 __host__ __device__ __forceinline__ constexpr int fft_pad(int i) {
-  return i + (i >> 3);
+  return i + (i >> FS_FFT_PAD_SHIFT);
 }
 __host__ __device__ __forceinline__ constexpr int fft_padded_len(int n) {
-  return fft_pad(n - 1) + 1;
+  int l = fft_pad(n - 1) + 1;
+  if (FS_FFT_STRIDE_MOD >= 0)
+    while (l % 32 != FS_FFT_STRIDE_MOD) ++l;
+  return l;
+}
+
+// Element-major layout (element m of every sequence in one row of sm): row
+// m sits at row m + m / 8, so rows 8 apart (the lanes of a radix-8 stage's
+// writes) fall on different banks.
+__host__ __device__ __forceinline__ constexpr int fft_rowpad(int m) {
+  return m + (m >> 3);
+}
+__host__ __device__ __forceinline__ constexpr int fft_rowpad_len(int n) {
+  return fft_rowpad(n - 1) + 1;
 }
 
 // LDS index of element i of sequence col: sequences contiguous and padded
-// (PAD, stride sc = padded length) or element-major (i * sm + col * sc)
+// (PAD, stride sc = padded length) or element-major
+// (fft_rowpad(i) * sm + col * sc)
 template <bool PAD>
 __device__ __forceinline__ int lds_idx(int i, int col, int sm, int sc) {
-  return PAD ? col * sc + fft_pad(i) : i * sm + col * sc;
+  return PAD ? col * sc + fft_pad(i) : fft_rowpad(i) * sm + col * sc;
 }
 
 // One in-place stage; blockDim.x * (E/R) == (n/R) * ncols exactly.
@@ -361,7 +389,7 @@ __device__ __forceinline__ void p2_stage(cx<T> *buf, int ncols, int sm,
     if constexpr (PAD) {  // sequences contiguous: butterflies fastest
       col = t >> LOGNB;
       j = t & (nb - 1);
-    } else {  // sequences interleaved (element m of column c at m*sm + c)
+    } else {  // sequences interleaved (element m of column c in row m)
       j = dcols.q(t);
       col = t - j * ncols;
     }
@@ -374,7 +402,7 @@ __device__ __forceinline__ void p2_stage(cx<T> *buf, int ncols, int sm,
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         int idx;
-        if constexpr (nb % 8 == 0)
+        if constexpr (nb % kFftPadGroup == 0)
           idx = pj + fft_pad(r * nb);
         else
           idx = fft_pad(j + r * nb);
@@ -382,7 +410,8 @@ __device__ __forceinline__ void p2_stage(cx<T> *buf, int ncols, int sm,
       }
     } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[m * R + r] = buf[(j + r * nb) * sm + col];
+      for (int r = 0; r < R; ++r)
+        v[m * R + r] = buf[fft_rowpad(j + r * nb) * sm + col];
     }
     if constexpr (LOGNS > 0) {
 #pragma unroll
@@ -405,7 +434,7 @@ __device__ __forceinline__ void p2_stage(cx<T> *buf, int ncols, int sm,
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         int idx;
-        if constexpr (ns % 8 == 0)
+        if constexpr (ns % kFftPadGroup == 0)
           idx = pd + fft_pad(r * ns);
         else
           idx = fft_pad(d + r * ns);
@@ -413,7 +442,8 @@ __device__ __forceinline__ void p2_stage(cx<T> *buf, int ncols, int sm,
       }
     } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[(d + r * ns) * sm + col] = v[m * R + r];
+      for (int r = 0; r < R; ++r)
+        buf[fft_rowpad(d + r * ns) * sm + col] = v[m * R + r];
     }
   }
   __syncthreads();
