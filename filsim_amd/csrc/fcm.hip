@@ -521,6 +521,26 @@ __device__ __forceinline__ void stage_tile_records(
     reinterpret_cast<int4 *>(s_rec)[threadIdx.x] = src[threadIdx.x];
 }
 
+// A tile's record count and this thread's 16-byte word of its slab, loaded
+// ahead of use: the persistent tile kernels fetch the next tile's while
+// they work on the current one.
+template <typename T>
+struct TilePrefetch {
+  static constexpr int W = sizeof(SegRec<T>) / 16;
+  int4 w;
+  int cnt;
+  __device__ __forceinline__ void load(const SegRec<T> *__restrict__ slab,
+                                       const int *__restrict__ tcnt, int tile) {
+    if ((int)threadIdx.x < kTileCap * W)
+      w = reinterpret_cast<const int4 *>(slab + (size_t)tile * kTileCap)[threadIdx.x];
+    cnt = tcnt[tile];
+  }
+  __device__ __forceinline__ void store(SegRec<T> *s_rec) const {
+    if ((int)threadIdx.x < kTileCap * W)
+      reinterpret_cast<int4 *>(s_rec)[threadIdx.x] = w;
+  }
+};
+
 // Gaussian weights of one segment at a tile's axis points. Point a: x
 // (a < 8), y (8 <= a < 16), z (16 <= a < 32) of the tile starting at
 // (px0, py0, pz0). w[a] = g (monopole), w[32 + a] = gD (dipole), w[64 + a]
@@ -640,7 +660,7 @@ __global__ void __launch_bounds__(256)
   constexpr int KC = kMfmaChunk<T>;
   if (act && !*act) return;  // batched sweep: suspension converged
   __shared__ SegRec<T> s_rec[kTileCap];
-  __shared__ T s_w[KC][kTileW];
+  __shared__ T s_w[KC + 1][kTileW];  // row KC stays zero
 
   int bx, by, bz;
   tile_of_block(G, blockIdx.x, &bx, &by, &bz);
@@ -652,6 +672,7 @@ __global__ void __launch_bounds__(256)
   const int px0 = bx * 8, py0 = by * 8, pz0 = bz * 16;
   // the slab is copied while the count is still on its way
   stage_tile_records<T>(slab, pool, tile, 0, 0, 0, s_rec);
+  for (int e = threadIdx.x; e < kTileW; e += 256) s_w[KC][e] = 0;
   const int cnt = tcnt[tile];
   const int ovf = cnt > kTileCap ? tovf[tile] : 0;
   v4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
@@ -667,9 +688,25 @@ __global__ void __launch_bounds__(256)
       const int nc = min(KC, nb - c0);
       tile_chunk_weights<T, KC, true>(g, s_rec, c0, nc, px0, py0, pz0, s_w);
       __syncthreads();
-      for (int c4 = 0; c4 < nc; c4 += 4) {
-        // lane group q takes candidate c4 + q (rows past nc are zero)
-        const T *w = s_w[c4 + q];
+      // this wave's rows lie in x columns 2 wave and 2 wave + 1: candidates
+      // with no stencil point there (zero monopole and dipole weights in
+      // both) add exact zeros to them and are skipped
+      const T *wl = s_w[lane < nc ? lane : KC];
+      unsigned long long m = __ballot(
+          lane < nc && (wl[2 * wave] != (T)0 || wl[2 * wave + 1] != (T)0 ||
+                        wl[32 + 2 * wave] != (T)0 || wl[33 + 2 * wave] != (T)0));
+      const int nw = __popcll(m);
+      for (int c4 = 0; c4 < nw; c4 += 4) {
+        // lane group q takes the (c4 + q)-th candidate of the wave's list
+        // (the zero row past its end)
+        int c = KC;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int ck = m ? (int)__builtin_ctzll(m) : KC;
+          if (k == q) c = ck;
+          m &= m - 1;
+        }
+        const T *w = s_w[c];
         const T *cf = w + 96;
         const T gxy = w[lx] * w[8 + ly], dxy = w[32 + lx] * w[40 + ly];
         const T xx = w[64 + lx], yy = w[72 + ly];
@@ -1175,8 +1212,9 @@ __global__ void __launch_bounds__(256)
 // interpolation on the matrix cores: the transpose of k_spread_mfma. One
 // workgroup per 8 x 8 x 16 tile reads the tile's three velocity grids once
 // (each lane four consecutive z of one (x, y) column per component) and,
-// for the segments of its tile list (16 per round), contracts them with
-// each segment's weights:
+// for the segments of its tile list (up to 32 per round; each wave takes
+// those with a stencil point in its rows, 16 per group), contracts them
+// with each segment's weights:
 //   P_m[xy][c] = sum_z u_d[xy][z] b_m,c[z]   (per wave 16 xy rows x 16
 //   segments, v_mfma 16x16x4 over z; 8 products m: u_0..2 with g_z,
 //   u_0..2 with gD_z, u_0..1 with gD_z grad_z)
@@ -1210,108 +1248,146 @@ __global__ void __launch_bounds__(256)
                   const int *__restrict__ tcnt, const int *__restrict__ tovf,
                   const T *__restrict__ R, T *__restrict__ part) {
   typedef typename Mfma<T>::v4 v4;
-  constexpr int KC = 16;
+  constexpr int KC = kTileCap;  // candidates per round
   constexpr int SX = stencil_tiles(NGD, 8), SZ = stencil_tiles(NGD, 16);
   constexpr int NSLOT = kInterpSlots<NGD>;
   if (act && !*act) return;  // batched sweep: suspension converged
   __shared__ SegRec<T> s_rec[kTileCap];
-  __shared__ T s_w[KC][kTileW];
+  __shared__ T s_w[KC + 1][kTileW];  // row KC stays zero
   __shared__ T s_red[4][KC][6];
+  __shared__ int s_wl[4][KC];  // each wave's candidate list
 
-  int bx, by, bz;
-  tile_of_block(G, blockIdx.x, &bx, &by, &bz);
-  const int tile = bx + G.nbx * (by + G.nby * bz);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int q = lane >> 4, i16 = lane & 15;
-  const int px0 = bx * 8, py0 = by * 8, pz0 = bz * 16;
-  // A operands: u_d[xy = wave*16 + i16][z = pz0 + 4q + k] for K step k
-  // (lane group q supplies z 4q .. 4q + 3)
+  const int ntiles = G.nbx * G.nby * G.nbz;
   const size_t N = (size_t)G.nx * G.ny * G.nz;
-  T u[3][4];
+  for (int e = threadIdx.x; e < kTileW; e += 256) s_w[KC][e] = 0;
+  // persistent: block b takes tiles b, b + gridDim.x, ... (gridDim.x <=
+  // ntiles, a multiple of 8 so each block's tiles stay on its XCD's run)
+  int v = blockIdx.x;
+  TilePrefetch<T> pf;
   {
-    const int xy = wave * 16 + i16;
-    const size_t o = ((size_t)(px0 + (xy >> 3)) * G.ny + py0 + (xy & 7)) * G.nz +
-                     pz0 + 4 * q;
-    ld4(R + o, u[0]);
-    ld4(R + N + o, u[1]);
-    ld4(R + 2 * N + o, u[2]);
+    int bx, by, bz;
+    tile_of_block(G, v, &bx, &by, &bz);
+    pf.load(slab, tcnt, bx + G.nbx * (by + G.nby * bz));
   }
-  stage_tile_records<T>(slab, pool, tile, 0, 0, 0, s_rec);
-  const int cnt = tcnt[tile];
-  const int ovf = cnt > kTileCap ? tovf[tile] : 0;
-
-  for (int r0 = 0; r0 < cnt; r0 += kTileCap) {
-    const int nb = min(kTileCap, cnt - r0);
-    if (r0 > 0) {
-      __syncthreads();  // the last round still reads s_rec
-      stage_tile_records<T>(slab, pool, tile, ovf, r0, nb, s_rec);
+  for (; v < ntiles; v += gridDim.x) {
+    int bx, by, bz;
+    tile_of_block(G, v, &bx, &by, &bz);
+    const int tile = bx + G.nbx * (by + G.nby * bz);
+    const int px0 = bx * 8, py0 = by * 8, pz0 = bz * 16;
+    // A operands: u_d[xy = wave*16 + i16][z = pz0 + 4q + k] for K step k
+    // (lane group q supplies z 4q .. 4q + 3); in flight during the weights
+    T u[3][4];
+    {
+      const int xy = wave * 16 + i16;
+      const size_t o = ((size_t)(px0 + (xy >> 3)) * G.ny + py0 + (xy & 7)) * G.nz +
+                       pz0 + 4 * q;
+      ld4(R + o, u[0]);
+      ld4(R + N + o, u[1]);
+      ld4(R + 2 * N + o, u[2]);
     }
-    __syncthreads();
-    for (int c0 = 0; c0 < nb; c0 += KC) {
-      const int nc = min(KC, nb - c0);
-      tile_chunk_weights<T, KC, false>(g, s_rec, c0, nc, px0, py0, pz0, s_w);
-      __syncthreads();
-      // B operands: this lane's column is candidate i16 (zero weights past nc)
-      const T *w = s_w[i16];
-      const v4 z4 = {0, 0, 0, 0};
-      v4 pg0 = z4, pg1 = z4, pg2 = z4, pd0 = z4, pd1 = z4, pd2 = z4,
-         pr0 = z4, pr1 = z4;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int z = 4 * q + k;
-        const T gz = w[16 + z], gdz = w[48 + z];
-        const T grz = gdz * w[80 + z];
-        pg0 = Mfma<T>::step(u[0][k], gz, pg0);
-        pg1 = Mfma<T>::step(u[1][k], gz, pg1);
-        pg2 = Mfma<T>::step(u[2][k], gz, pg2);
-        pd0 = Mfma<T>::step(u[0][k], gdz, pd0);
-        pd1 = Mfma<T>::step(u[1][k], gdz, pd1);
-        pd2 = Mfma<T>::step(u[2][k], gdz, pd2);
-        pr0 = Mfma<T>::step(u[0][k], grz, pr0);
-        pr1 = Mfma<T>::step(u[1][k], grz, pr1);
-      }
-      // lane holds P[xy = wave*16 + row(lane, r)][candidate i16]: weight
-      // each row with the candidate's x/y factors
-      T v0 = 0, v1 = 0, v2 = 0, w0 = 0, w1 = 0, w2 = 0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int rxy = wave * 16 + Mfma<T>::row(lane, r);
-        const int x = rxy >> 3, y = rxy & 7;
-        const T gxy = w[x] * w[8 + y], dxy = w[32 + x] * w[40 + y];
-        const T xx = w[64 + x], yy = w[72 + y];
-        v0 += gxy * pg0[r];
-        v1 += gxy * pg1[r];
-        v2 += gxy * pg2[r];
-        w0 += dxy * (yy * pd2[r] - pr1[r]);
-        w1 += dxy * (pr0[r] - xx * pd2[r]);
-        w2 += dxy * (xx * pd1[r] - yy * pd0[r]);
-      }
-      // the four lane groups (rows) of the wave, then the four waves
-#pragma unroll
-      for (int off = 16; off < 64; off <<= 1) {
-        v0 += __shfl_xor(v0, off, 64);
-        v1 += __shfl_xor(v1, off, 64);
-        v2 += __shfl_xor(v2, off, 64);
-        w0 += __shfl_xor(w0, off, 64);
-        w1 += __shfl_xor(w1, off, 64);
-        w2 += __shfl_xor(w2, off, 64);
-      }
-      if (q == 0) {
-        T *sr = s_red[wave][i16];
-        sr[0] = v0;
-        sr[1] = v1;
-        sr[2] = v2;
-        sr[3] = w0;
-        sr[4] = w1;
-        sr[5] = w2;
+    const int cnt = pf.cnt;
+    __syncthreads();  // the last tile is done with s_rec, s_w and s_red
+    pf.store(s_rec);
+    if (v + (int)gridDim.x < ntiles) {
+      int nx_, ny_, nz_;
+      tile_of_block(G, v + gridDim.x, &nx_, &ny_, &nz_);
+      pf.load(slab, tcnt, nx_ + G.nbx * (ny_ + G.nby * nz_));
+    }
+    const int ovf = cnt > kTileCap ? tovf[tile] : 0;
+
+    for (int r0 = 0; r0 < cnt; r0 += kTileCap) {
+      const int nc = min(kTileCap, cnt - r0);
+      if (r0 > 0) {
+        __syncthreads();  // the last round still reads s_rec
+        stage_tile_records<T>(slab, pool, tile, ovf, r0, nc, s_rec);
       }
       __syncthreads();
+      tile_chunk_weights<T, KC, false>(g, s_rec, 0, nc, px0, py0, pz0, s_w);
+      __syncthreads();
+      // this wave's rows lie in x columns 2 wave and 2 wave + 1: it handles
+      // only the candidates with a stencil point there (non-zero monopole or
+      // dipole weight), 16 per group, and leaves zeros for the others
+      {
+        const T *wl = s_w[lane < nc ? lane : KC];
+        const bool mine = lane < nc && (wl[2 * wave] != (T)0 ||
+                                        wl[2 * wave + 1] != (T)0 ||
+                                        wl[32 + 2 * wave] != (T)0 ||
+                                        wl[33 + 2 * wave] != (T)0);
+        const unsigned long long mm = __ballot(mine);
+        if (mine) s_wl[wave][__popcll(mm & ((1ull << lane) - 1ull))] = lane;
+        for (int e = lane; e < KC * 6; e += 64) (&s_red[wave][0][0])[e] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int nw = __popcll(mm);
+        for (int g0 = 0; g0 < nw; g0 += 16) {
+          // B operands: this lane's column is the wave's candidate g0 + i16
+          const int c = g0 + i16 < nw ? s_wl[wave][g0 + i16] : KC;
+          const T *w = s_w[c];
+          const v4 z4 = {0, 0, 0, 0};
+          v4 pg0 = z4, pg1 = z4, pg2 = z4, pd0 = z4, pd1 = z4, pd2 = z4,
+             pr0 = z4, pr1 = z4;
+  #pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int z = 4 * q + k;
+            const T gz = w[16 + z], gdz = w[48 + z];
+            const T grz = gdz * w[80 + z];
+            pg0 = Mfma<T>::step(u[0][k], gz, pg0);
+            pg1 = Mfma<T>::step(u[1][k], gz, pg1);
+            pg2 = Mfma<T>::step(u[2][k], gz, pg2);
+            pd0 = Mfma<T>::step(u[0][k], gdz, pd0);
+            pd1 = Mfma<T>::step(u[1][k], gdz, pd1);
+            pd2 = Mfma<T>::step(u[2][k], gdz, pd2);
+            pr0 = Mfma<T>::step(u[0][k], grz, pr0);
+            pr1 = Mfma<T>::step(u[1][k], grz, pr1);
+          }
+          // lane holds P[xy = wave*16 + row(lane, r)][candidate c]: weight
+          // each row with the candidate's x/y factors
+          T v0 = 0, v1 = 0, v2 = 0, w0 = 0, w1 = 0, w2 = 0;
+  #pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int rxy = wave * 16 + Mfma<T>::row(lane, r);
+            const int x = rxy >> 3, y = rxy & 7;
+            const T gxy = w[x] * w[8 + y], dxy = w[32 + x] * w[40 + y];
+            const T xx = w[64 + x], yy = w[72 + y];
+            v0 += gxy * pg0[r];
+            v1 += gxy * pg1[r];
+            v2 += gxy * pg2[r];
+            w0 += dxy * (yy * pd2[r] - pr1[r]);
+            w1 += dxy * (pr0[r] - xx * pd2[r]);
+            w2 += dxy * (xx * pd1[r] - yy * pd0[r]);
+          }
+          // the four lane groups (rows) of the wave
+  #pragma unroll
+          for (int off = 16; off < 64; off <<= 1) {
+            v0 += __shfl_xor(v0, off, 64);
+            v1 += __shfl_xor(v1, off, 64);
+            v2 += __shfl_xor(v2, off, 64);
+            w0 += __shfl_xor(w0, off, 64);
+            w1 += __shfl_xor(w1, off, 64);
+            w2 += __shfl_xor(w2, off, 64);
+          }
+          if (q == 0 && c < KC) {
+            T *sr = s_red[wave][c];
+            sr[0] = v0;
+            sr[1] = v1;
+            sr[2] = v2;
+            sr[3] = w0;
+            sr[4] = w1;
+            sr[5] = w2;
+          }
+        }
+      }
+      __syncthreads();
+      // the four waves in a fixed order
       if (threadIdx.x < KC * 6) {
         const int c = threadIdx.x / 6, m = threadIdx.x - 6 * c;
         if (c < nc) {
           const T sum = ((s_red[0][c][m] + s_red[1][c][m]) + s_red[2][c][m]) +
                         s_red[3][c][m];
-          const SegRec<T> &rc = s_rec[c0 + c];
+          const SegRec<T> &rc = s_rec[c];
           int sx = bx - (rc.org & 1023) / 8, sy = by - ((rc.org >> 10) & 1023) / 8,
               sz = bz - (rc.org >> 20) / 16;
           if (sx < 0) sx += G.nbx;
@@ -1320,7 +1396,6 @@ __global__ void __launch_bounds__(256)
           part[((size_t)rc.id * 6 + m) * NSLOT + (sx * SX + sy) * SZ + sz] = sum;
         }
       }
-      __syncthreads();  // the next round overwrites weights and sums
     }
   }
 }
@@ -1592,6 +1667,23 @@ static int largest_divisor_le(int n, int t) {
   return 1;
 }
 
+// Blocks of a persistent tile kernel (256 threads): as many as are resident
+// on the device at once, a multiple of 8, at most one per tile.
+template <typename K>
+static int persistent_blocks(K kernel, int ntiles) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+          hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) !=
+          hipSuccess ||
+      per < 1 || cus < 1)
+    return ntiles;
+  long b = (long)cus * per;
+  b = std::max(8L, b - b % 8);
+  return (int)std::min<long>(b, ntiles);
+}
+
 int FcmImpl::set_geometry() {
   int t = tune.spread_tile;
   FS_CHECK(t == 4 || t == 8 || t == 16, FS_ERR_ARG,
@@ -1654,6 +1746,11 @@ int FcmImpl::set_geometry() {
     FS_TRY(tcnt.alloc((size_t)nbins * sizeof(int)));
     FS_TRY(tovf.alloc((size_t)nbins * sizeof(int)));
     FS_TRY(tpool.alloc((size_t)max_np * nslot * rec_bytes(tsz)));
+    with_NGD(p.ngd, [&](auto ng) {
+      constexpr int NG = decltype(ng)::value;
+      grid_interp = tsz == 8 ? persistent_blocks(k_interp_mfma<double, NG>, nbins)
+                             : persistent_blocks(k_interp_mfma<float, NG>, nbins);
+    });
   }
   // one more counter after the bins: the tile lists' pool top
   FS_TRY(bin_count.alloc((size_t)(nbins + 1) * sizeof(int)));
@@ -1850,7 +1947,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   bool okn = with_NGD(p.ngd, [&](auto ng) {
     constexpr int NG = decltype(ng)::value;
     if (mfma_spread) {
-      hipLaunchKernelGGL((k_interp_mfma<T, NG>), dim3(ntiles), dim3(256), 0,
+      hipLaunchKernelGGL((k_interp_mfma<T, NG>), dim3(grid_interp), dim3(256), 0,
                          st, act, geom, g, tslab.as<SegRec<T>>(),
                          tpool.as<SegRec<T>>(), tcnt.as<int>(), tovf.as<int>(),
                          R.as<T>(), ipart.as<T>());

@@ -36,6 +36,7 @@ struct FcmImpl {
   SpreadGeom geom{};
   int nbins = 0;
   bool mfma_spread = false;  // spread on the matrix cores (8 x 8 x 16 tiles)
+  int grid_interp = 0;  // blocks of the persistent interpolation
   int zP = 1, yB = 1, xQ = 1;  // FFT pass workgroup shapes
   // device flag read by every kernel of a solve: 0 = skip (a converged
   // suspension of a batched sweep); nullptr = always run
