@@ -33,12 +33,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per dispatch of the FP32 spread (k_spread_mfma<float, 12>) at 256^3 /
-# 30,000 segments, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in
-# separate runs, each scaled by the same counter's reading for a 1 GiB copy
-# (profiles/fcm_mobility_256cubed_30k_hbm_bytes.txt): 51.1 MB read (ids,
-# origins, coefficients, tables) + 201.3 MB written
-SPREAD_FP32_TRAFFIC_BYTES = 252.4e6
+# HBM bytes per dispatch of the FP32 y pass (k_fft_y<float, 8, 8>, the
+# timestep's largest kernel by total time) at 256^3, measured with rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE in separate runs, each scaled by the same
+# counter's reading for a 1 GiB copy
+# (profiles/fcm_mobility_256cubed_30k_hbm_bytes.txt): 203.0 MB read + 202.9
+# MB written, the half spectrum once each way
+YPASS_FP32_TRAFFIC_BYTES = 405.9e6
+
+
+def api_kernels():
+    from filsim_amd import api
+    return api.Fcm.KERNELS
 
 
 def log(*a):
@@ -111,17 +117,21 @@ def mobility_bench(precision, reps, torch, np_=30000, grid=256, seed=7):
         fcm.mobility(Y, F, T, V, W)
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / reps
-    # per-stage device times (HIP events on the solver's stream)
+    # per-stage and per-kernel device times (HIP events on the solver's
+    # stream)
     fcm.set_timing(True)
     stages = np.zeros(7)
+    kernels = np.zeros(10)
     nrep = 10
     for _ in range(nrep):
         fcm.mobility(Y, F, T, V, W)
         stages += fcm.stage_times()
+        kernels += fcm.kernel_times()
     fcm.set_timing(False)
     stages /= nrep
+    kernels /= nrep
     fcm.close()
-    return t, stages, fp
+    return t, stages, fp, kernels
 
 
 CPU_BUCKETS = ("assign_F", "build_Jinv", "collision_barrier", "FCM",
@@ -519,12 +529,14 @@ def main():
         return
 
     # FCM mobility microbenchmark (256^3, 30,000 segments)
-    t64, st64, fp = mobility_bench(abi.FS_FP64, args.mob_reps, torch)
-    t32, st32, _ = mobility_bench(abi.FS_FP32, args.mob_reps, torch)
+    t64, st64, fp, kt64 = mobility_bench(abi.FS_FP64, args.mob_reps, torch)
+    t32, st32, _, kt32 = mobility_bench(abi.FS_FP32, args.mob_reps, torch)
     log("mobility fp64 %.3f ms (%.1f solves/s), fp32 %.3f ms (%.1f/s)" %
         (1e3 * t64, 1 / t64, 1e3 * t32, 1 / t32))
     log("fp64 stages ms", [round(float(x), 4) for x in st64])
     log("fp32 stages ms", [round(float(x), 4) for x in st32])
+    log("fp64 kernels ms", [round(float(x), 4) for x in kt64])
+    log("fp32 kernels ms", [round(float(x), 4) for x in kt32])
     N = 256 ** 3
     out["mobility"] = {
         "fp64_solves_per_s": 1 / t64, "fp32_solves_per_s": 1 / t32,
@@ -532,13 +544,17 @@ def main():
         "fp64_algorithmic_GBps": 24 * N * 8 / t64 / 1e9,
         "fp32_algorithmic_GBps": 24 * N * 4 / t32 / 1e9,
         "fp64_stage_ms": [float(x) for x in st64],
-        "fp32_stage_ms": [float(x) for x in st32]}
-    # dominant kernel of the timestep: the FP32 spread (stage bucket 2, HIP
-    # events on the solver's stream around its launch). Algorithmic bytes:
-    # one write of the 3 real grids (3 * 256^3 floats); the candidate
-    # tables it reads are ~2% of that and not counted.
-    xk_bytes = 3 * N * 4
-    xk_ms = float(st32[2])
+        "fp32_stage_ms": [float(x) for x in st32],
+        "kernels": list(api_kernels()),
+        "fp64_kernel_ms": [float(x) for x in kt64],
+        "fp32_kernel_ms": [float(x) for x in kt32]}
+    # dominant kernel of the timestep by total time: the FP32 y pass, two
+    # launches per solve (forward and inverse; HIP events on the solver's
+    # stream around each launch, averaged). Algorithmic bytes: one read and
+    # one write of the 3 half-spectrum components (3 * 256 * 256 * 129
+    # complex floats each way).
+    xk_bytes = 2 * 3 * 256 * 256 * 129 * 8
+    xk_ms = 0.5 * float(kt32[4] + kt32[6])
 
     res = {}
     info = {}
@@ -680,11 +696,11 @@ def main():
                                "KB=1000, dt=0.005, TOL=1e-6, barrier on",
                    "global_batch": 30000, "seq_len": 256,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_spread_mfma<float, 12>",
+        "roofline": {"bound": "hbm", "kernel": "k_fft_y<float, 8, 8>",
                      "achieved": xk_bytes / (xk_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": xk_bytes / (xk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": SPREAD_FP32_TRAFFIC_BYTES,
+                     "traffic": YPASS_FP32_TRAFFIC_BYTES,
                      "algorithmic_bytes": xk_bytes},
         "cpu_baseline": cpu,
         "extra": out,

@@ -87,6 +87,7 @@ PROTOS = {
     "fs_fcm_friction": [P, I, P, P, P, P, P],
     "fs_fcm_set_timing": [P, I],
     "fs_fcm_stage_times": [P, FP],
+    "fs_fcm_kernel_times": [P, FP],
     "fs_fcm_get_tuning": [P, C.POINTER(FcmTuning)],
     "fs_fcm_set_tuning": [P, C.POINTER(FcmTuning)],
     "fs_fcm_autotune": [P, C.c_int, C.c_int, C.POINTER(FcmTuning)],

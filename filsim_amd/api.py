@@ -86,6 +86,17 @@ class Fcm:
         abi.check(self.lib.fs_fcm_stage_times(self.h, abi.fptr(t)))
         return t
 
+    KERNELS = ("tables_bin_sort", "tile_lists", "spread", "fft_z_r2c",
+               "fft_y_forward", "fft_x_project", "fft_y_inverse",
+               "fft_z_c2r", "interp", "interp_sum")
+
+    def kernel_times(self):
+        """Per-kernel device times (ms) of the last timed solve, in the
+        order of KERNELS."""
+        t = np.zeros(10, dtype=np.float32)
+        abi.check(self.lib.fs_fcm_kernel_times(self.h, abi.fptr(t)))
+        return t
+
     def get_tuning(self):
         t = abi.FcmTuning()
         abi.check(self.lib.fs_fcm_get_tuning(self.h, C.byref(t)))

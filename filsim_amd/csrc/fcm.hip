@@ -1654,6 +1654,7 @@ int FcmImpl::init(const fs_fcm_params &pp, int maxnp, int prec) {
   tune.fft_xlines = (prec == FS_FP32) ? 4 : 2;
   FS_TRY(set_geometry());
   for (int i = 0; i < 8; ++i) FS_HIP(hipEventCreate(&ev[i]));
+  for (int i = 0; i < 11; ++i) FS_HIP(hipEventCreate(&kev[i]));
   if (prec == FS_FP64)
     FS_TRY(set_fft_lds_attr<double>(plx, ply, plz));
   else
@@ -1801,6 +1802,8 @@ void FcmImpl::release() {
   scan_tmp.release();
   for (int i = 0; i < 8; ++i)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
+  for (int i = 0; i < 11; ++i)
+    if (kev[i]) (void)hipEventDestroy(kev[i]);
   for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
   prof_ev.clear();
   prof_used = 0;
@@ -1841,7 +1844,17 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
     if (timing) (void)hipEventRecord(ev[i], st);
     if (pset) (void)hipEventRecord(pset[i], st);
   };
+  // kernel boundaries (timing only); a boundary not recorded in this solve
+  // marks a kernel the path does not have
+  for (int i = 0; i < 11; ++i) kev_used[i] = false;
+  auto kmark = [&](int i) {
+    if (timing) {
+      (void)hipEventRecord(kev[i], st);
+      kev_used[i] = true;
+    }
+  };
   mark(0);
+  kmark(0);
   // bin counting sort
   hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbins + 1, 256)), dim3(256), 0, st,
                      act, bin_count.as<int>(), nbins + 1, 0);
@@ -1863,6 +1876,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                      mfma_spread ? srec.as<SegRec<T>>() : (SegRec<T> *)nullptr,
                      mfma_spread ? sorg.as<int>() : (int *)nullptr);
   const int ntiles = nbins;
+  kmark(1);
   if (mfma_spread)
     hipLaunchKernelGGL(k_tile_lists<T>, dim3(ceil_div(ntiles, 4)), dim3(256), 0, st,
                        act, geom, bin_start.as<int>(), (const int *)sorg.as<int>(),
@@ -1870,6 +1884,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                        tslab.as<SegRec<T>>(), tcnt.as<int>(), tovf.as<int>(),
                        tpool.as<SegRec<T>>(), bin_count.as<int>() + nbins);
   mark(1);
+  if (mfma_spread) kmark(2);
   bool okz = true;
   if (mfma_spread)
     hipLaunchKernelGGL(k_spread_mfma<T>, dim3(ntiles), dim3(256), 0, st, act, geom,
@@ -1885,6 +1900,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   FS_CHECK(okz, FS_ERR_ARG, "spread dispatch failed");
   FS_LAUNCHED();
   mark(2);
+  kmark(3);
   const int nh = p.nz / 2 + 1;
   const size_t csz = sizeof(cx<T>);
   bool ok = with_EL(plz, [&](auto e, auto l) {
@@ -1895,6 +1911,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                        R.as<T>(), C.as<cx<T>>(), plz, twz.as<cx<T>>(), p.nx,
                        p.ny, zP, make_div(2 * zP), make_div(zP));
   });
+  kmark(4);
   ok &= with_EL(ply, [&](auto e, auto l) {
     constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
     hipLaunchKernelGGL((k_fft_y<T, EE, LL>), dim3(nh * (p.nx / yB), 3),
@@ -1905,6 +1922,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
   mark(3);
+  kmark(5);
   ProjConst pc;
   pc.nx = p.nx;
   pc.ny = p.ny;
@@ -1925,6 +1943,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
   mark(4);
+  kmark(6);
   ok = with_EL(ply, [&](auto e, auto l) {
     constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
     hipLaunchKernelGGL((k_fft_y<T, EE, LL>), dim3(nh * (p.nx / yB), 3),
@@ -1932,6 +1951,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                        C.as<cx<T>>(), ply, twy.as<cx<T>>(), p.nx, nh, yB, 1,
                        make_div(yB));
   });
+  kmark(7);
   ok &= with_EL(plz, [&](auto e, auto l) {
     constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
     hipLaunchKernelGGL((k_fft_z_c2r<T, EE, LL>), dim3(p.ny * (p.nx / (2 * zP)), 3),
@@ -1943,6 +1963,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   FS_CHECK(ok, FS_ERR_ARG, "FFT dispatch failed");
   FS_LAUNCHED();
   mark(5);
+  kmark(8);
   double dx3 = p.dx * p.dx * p.dx;
   bool okn = with_NGD(p.ngd, [&](auto ng) {
     constexpr int NG = decltype(ng)::value;
@@ -1951,6 +1972,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                          st, act, geom, g, tslab.as<SegRec<T>>(),
                          tpool.as<SegRec<T>>(), tcnt.as<int>(), tovf.as<int>(),
                          R.as<T>(), ipart.as<T>());
+      kmark(9);
       hipLaunchKernelGGL((k_interp_sum<T, NG>), dim3(ceil_div(6L * np, 256)),
                          dim3(256), 0, st, act, np, geom, dx3,
                          origin.as<int>(), ipart.as<T>(), V, W);
@@ -1964,6 +1986,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   FS_CHECK(okn, FS_ERR_ARG, "interp dispatch failed");
   FS_LAUNCHED();
   mark(6);
+  kmark(10);
   return FS_OK;
 }
 
@@ -2005,6 +2028,25 @@ int FcmImpl::friction(int np, const double *F, const double *Tq, double *V,
   hipLaunchKernelGGL(k_friction, dim3(ceil_div(3 * np, 256)), dim3(256), 0,
                      st, act, 3 * np, F, Tq, a1, a2, V, W);
   FS_LAUNCHED();
+  return FS_OK;
+}
+
+int FcmImpl::kernel_times(float *ms10) {
+  for (int i = 0; i < 10; ++i) ms10[i] = 0;
+  if (!timing || !kev_used[10]) return FS_OK;
+  FS_HIP(hipEventSynchronize(kev[10]));
+  auto span = [&](int a, int b, float *out) -> int {
+    FS_HIP(hipEventElapsedTime(out, kev[a], kev[b]));
+    return FS_OK;
+  };
+  const int sp0 = kev_used[2] ? 2 : 1;   // spread starts after the lists
+  const int ie = kev_used[9] ? 9 : 10;   // interpolation ends before its sums
+  FS_TRY(span(0, 1, &ms10[0]));
+  if (kev_used[2]) FS_TRY(span(1, 2, &ms10[1]));
+  FS_TRY(span(sp0, 3, &ms10[2]));
+  for (int k = 3; k < 8; ++k) FS_TRY(span(k, k + 1, &ms10[k]));
+  FS_TRY(span(8, ie, &ms10[8]));
+  if (kev_used[9]) FS_TRY(span(9, 10, &ms10[9]));
   return FS_OK;
 }
 
@@ -2222,6 +2264,11 @@ extern "C" int fs_fcm_set_timing(fs_fcm *h, int on) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
   h->impl.timing = on ? 1 : 0;
   return FS_OK;
+}
+
+extern "C" int fs_fcm_kernel_times(fs_fcm *h, float *ms10) {
+  FS_CHECK(h && ms10, FS_ERR_ARG, "null handle or output");
+  return h->impl.kernel_times(ms10);
 }
 
 extern "C" int fs_fcm_stage_times(fs_fcm *h, float *ms7) {

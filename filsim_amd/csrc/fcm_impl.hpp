@@ -49,6 +49,8 @@ struct FcmImpl {
   // [segment][component][tile slot]
   dbuf srec, sorg, tslab, tcnt, tovf, tpool, ipart;
   hipEvent_t ev[8] = {};
+  hipEvent_t kev[11] = {};  // kernel boundaries of a timed solve
+  bool kev_used[11] = {};
 
   int init(const fs_fcm_params &pp, int maxnp, int prec);
   int set_geometry();
@@ -58,6 +60,7 @@ struct FcmImpl {
   int friction(int np, const double *F, const double *Tq, double *V,
                double *W, hipStream_t st);
   int stage_times(float *ms7);
+  int kernel_times(float *ms10);
   int set_tuning(const fs_fcm_tuning &t);
   int autotune(int np, int reps, fs_fcm_tuning *best);
   template <typename T>

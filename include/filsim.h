@@ -78,6 +78,13 @@ int fs_fcm_friction(fs_fcm *h, int np, const double *F, const double *T,
  * fs_fcm_set_timing(h,1) before the solve. */
 int fs_fcm_set_timing(fs_fcm *h, int on);
 int fs_fcm_stage_times(fs_fcm *h, float *ms7);
+/* Per-kernel device time of the last solve in ms (no reference counterpart;
+ * finer than fs_fcm_stage_times): [0] tables and bin sort, [1] tile lists,
+ * [2] spread, [3] z R2C, [4] y forward, [5] x FFT + projection + x inverse,
+ * [6] y inverse, [7] z C2R, [8] interpolation, [9] interpolation sums (0
+ * for a kernel the solve's path does not have). Requires
+ * fs_fcm_set_timing(h,1) before the solve. */
+int fs_fcm_kernel_times(fs_fcm *h, float *ms10);
 /* Launch-shape tunables (tile/batch shapes); values <= 0 keep current. */
 typedef struct fs_fcm_tuning {
   int fft_batch;    /* columns per workgroup in the strided FFT passes */
