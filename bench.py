@@ -265,14 +265,16 @@ def batch_bench(args, torch, n=64, seed=4000):
     t_setup = time.perf_counter() - t0
     res, finals, t_capture = {}, {}, 0.0
     steps = max(2, args.steps // 2)
-    for name in ["batched_lanes%d" % k for k in (0, 4, 8)] + ["sequential"]:
+    for name in (["fused_lanes%d" % k for k in (1, 4)] +
+                 ["batched_lanes%d" % k for k in (0, 4, 8)] + ["sequential"]):
         grp = fresh(tunes)
         batch = None
-        if name.startswith("batched"):
+        if name != "sequential":
             t0 = time.perf_counter()
             batch = api.Batch(grp)
             t_capture = time.perf_counter() - t0
-            batch.set_lanes(int(name[len("batched_lanes"):]))
+            batch.set_fused(name.startswith("fused"))
+            batch.set_lanes(int(name.split("_lanes")[1]))
 
         def step(nt):
             if batch is not None:
@@ -300,10 +302,10 @@ def batch_bench(args, torch, n=64, seed=4000):
             batch.close()
         for s in grp:
             s.close()
-    best = max((k for k in res if k.startswith("batched")),
+    best = max((k for k in res if k != "sequential"),
                key=lambda k: res[k]["suspension_steps_per_s"])
     same = all(np.array_equal(a, b)
-               for k in res if k.startswith("batched")
+               for k in res if k != "sequential"
                for a, b in zip(finals[k], finals["sequential"]))
     res["batched"] = dict(res[best], mode=best)
     info = [p[5] for p in pts]
@@ -318,7 +320,8 @@ def batch_bench(args, torch, n=64, seed=4000):
                                    (n, sorted(set(i["grid"] for i in info)),
                                     sum(i["segments"] for i in info))},
             "batched": res["batched"], "sequential": res["sequential"],
-            "modes": {k: v for k, v in res.items() if k.startswith("batched_")},
+            "modes": {k: v for k, v in res.items()
+                      if k.startswith("batched_") or k.startswith("fused_")},
             "speedup": res["batched"]["suspension_steps_per_s"] /
                        res["sequential"]["suspension_steps_per_s"],
             "bitwise_equal_final_positions": bool(same),

@@ -186,6 +186,7 @@ PROTOS = {
     "fs_batch_size": [P, IP],
     "fs_batch_timestep": [P, C.c_int, IP],
     "fs_batch_set_lanes": [P, C.c_int],
+    "fs_batch_set_fused": [P, C.c_int],
 }
 
 _lib = None

@@ -433,6 +433,11 @@ class Batch:
         """0: one parent graph per phase; k: per-system graphs on k streams."""
         abi.check(self.lib.fs_batch_set_lanes(self.h, lanes))
 
+    def set_fused(self, on=True):
+        """Fused FCM stages: one launch per stage for each group of systems
+        with equal launch shapes (off by default)."""
+        abi.check(self.lib.fs_batch_set_fused(self.h, 1 if on else 0))
+
 
 def make_filaments(box, nw, dL=2.2, min_sep=2.2, seed=1, max_tries=100000,
                    gpu=False):

@@ -115,10 +115,10 @@ __global__ void k_prepare_tab(const int *__restrict__ act, int np, const double 
   ts[(6 + d) * g.ngd + i] = (T)(-xx / g.sigmadipsq);
 }
 
-template <typename T>
-__global__ void k_prepare(const int *__restrict__ act, int np, const double *__restrict__ Y,
+template <typename T, typename GC = GaussConst>
+__device__ __forceinline__ void d_prepare(const int *__restrict__ act, int np, const double *__restrict__ Y,
                           const double *__restrict__ F,
-                          const double *__restrict__ Tq, GaussConst g,
+                          const double *__restrict__ Tq, GC g,
                           int tx, int ty, int tz, int nbx, int nby, int nbz,
                           int *__restrict__ origin, T *__restrict__ coef,
                           int *__restrict__ bin, int *__restrict__ bin_count,
@@ -147,26 +147,99 @@ __global__ void k_prepare(const int *__restrict__ act, int np, const double *__r
   bin[s] = b;
   slot[s] = atomicAdd(&bin_count[b], 1);
 }
+template <typename T>
+__global__ void k_prepare(const int *__restrict__ act, int np, const double *__restrict__ Y,
+                          const double *__restrict__ F,
+                          const double *__restrict__ Tq, GaussConst g,
+                          int tx, int ty, int tz, int nbx, int nby, int nbz,
+                          int *__restrict__ origin, T *__restrict__ coef,
+                          int *__restrict__ bin, int *__restrict__ bin_count,
+                          int *__restrict__ slot) {
+  d_prepare<T>(act, np, Y, F, Tq, g, tx, ty, tz, nbx, nby, nbz, origin, coef, bin, bin_count, slot);
+}
+// arguments of one suspension for k_prepare_multi
+template <typename T>
+struct A_prepare {
+  const int *act;
+  int np;
+  const double *Y;
+  const double *F;
+  const double *Tq;
+  GaussConst g;
+  int tx;
+  int ty;
+  int tz;
+  int nbx;
+  int nby;
+  int nbz;
+  int *origin;
+  T *coef;
+  int *bin;
+  int *bin_count;
+  int *slot;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T>
+__global__ void k_prepare_multi(const A_prepare<T> *__restrict__ a) {
+  const A_prepare<T> &A = a[blockIdx.z];
+  d_prepare<T, const GaussConst &>(A.act, A.np, A.Y, A.F, A.Tq, A.g, A.tx, A.ty, A.tz, A.nbx, A.nby, A.nbz, A.origin, A.coef, A.bin, A.bin_count, A.slot);
+}
 
-__global__ void k_fill_int(const int *__restrict__ act, int *p, int n, int v) {
+__device__ __forceinline__ void d_fill_int(const int *__restrict__ act, int *p, int n, int v) {
   if (act && !*act) return;  // batched sweep: suspension converged
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_fill_int(const int *__restrict__ act, int *p, int n, int v) {
+  d_fill_int(act, p, n, v);
+}
+// arguments of one suspension for k_fill_int_multi
+struct A_fill_int {
+  const int *act;
+  int *p;
+  int n;
+  int v;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+__global__ void k_fill_int_multi(const A_fill_int *__restrict__ a) {
+  const A_fill_int &A = a[blockIdx.z];
+  d_fill_int(A.act, A.p, A.n, A.v);
+}
 
-__global__ void k_scatter(int np, const int *__restrict__ bin,
+__device__ __forceinline__ void d_scatter(int np, const int *__restrict__ bin,
                           const int *__restrict__ slot,
                           const int *__restrict__ start, int *__restrict__ ids) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= np) return;
   ids[start[bin[s]] + slot[s]] = s;
 }
+__global__ void k_scatter(int np, const int *__restrict__ bin,
+                          const int *__restrict__ slot,
+                          const int *__restrict__ start, int *__restrict__ ids) {
+  d_scatter(np, bin, slot, start, ids);
+}
+// arguments of one suspension for k_scatter_multi
+struct A_scatter {
+  int np;
+  const int *bin;
+  const int *slot;
+  const int *start;
+  int *ids;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+__global__ void k_scatter_multi(const A_scatter *__restrict__ a) {
+  const A_scatter &A = a[blockIdx.z];
+  d_scatter(A.np, A.bin, A.slot, A.start, A.ids);
+}
 
 // Ascending segment order inside each bin (atomic slots are unordered);
 // srec and sorg (the matrix-core path; may be null) receive each segment's
 // record and packed stencil origin in the same order.
 template <typename T>
-__global__ void k_sort_bins(const int *__restrict__ act, int nbins,
+__device__ __forceinline__ void d_sort_bins(const int *__restrict__ act, int nbins,
                             const int *__restrict__ start,
                             int *__restrict__ ids,
                             const int *__restrict__ origin,
@@ -204,6 +277,41 @@ __global__ void k_sort_bins(const int *__restrict__ act, int nbins,
       srec[i] = r;
       sorg[i] = r.org;
     }
+}
+template <typename T>
+__global__ void k_sort_bins(const int *__restrict__ act, int nbins,
+                            const int *__restrict__ start,
+                            int *__restrict__ ids,
+                            const int *__restrict__ origin,
+                            const double *__restrict__ Y,
+                            const double *__restrict__ F,
+                            const double *__restrict__ Tq, double dx,
+                            int ngdh, SegRec<T> *__restrict__ srec,
+                            int *__restrict__ sorg) {
+  d_sort_bins<T>(act, nbins, start, ids, origin, Y, F, Tq, dx, ngdh, srec, sorg);
+}
+// arguments of one suspension for k_sort_bins_multi
+template <typename T>
+struct A_sort_bins {
+  const int *act;
+  int nbins;
+  const int *start;
+  int *ids;
+  const int *origin;
+  const double *Y;
+  const double *F;
+  const double *Tq;
+  double dx;
+  int ngdh;
+  SegRec<T> *srec;
+  int *sorg;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T>
+__global__ void k_sort_bins_multi(const A_sort_bins<T> *__restrict__ a) {
+  const A_sort_bins<T> &A = a[blockIdx.z];
+  d_sort_bins<T>(A.act, A.nbins, A.start, A.ids, A.origin, A.Y, A.F, A.Tq, A.dx, A.ngdh, A.srec, A.sorg);
 }
 
 // ---------------------------------------------------------------------
@@ -414,9 +522,8 @@ __device__ __forceinline__ void tile_of_block(const SpreadGeom &G, int bid,
 // runs, the order inside it does not); tcnt = how many, tovf = the pool
 // block. The pool holds np times the tiles one stencil can meet, a bound on
 // the records of all tiles.
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_tile_lists(const int *__restrict__ act, SpreadGeom G,
+template <typename T, typename SG = SpreadGeom>
+__device__ __forceinline__ void d_tile_lists(const int *__restrict__ act, SG G,
                  const int *__restrict__ start, const int *__restrict__ sorg,
                  const SegRec<T> *__restrict__ srec, SegRec<T> *__restrict__ slab,
                  int *__restrict__ tcnt, int *__restrict__ tovf,
@@ -502,6 +609,37 @@ __global__ void __launch_bounds__(256)
     tcnt[tile] = kept;
     tovf[tile] = base;
   }
+}
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_tile_lists(const int *__restrict__ act, SpreadGeom G,
+                 const int *__restrict__ start, const int *__restrict__ sorg,
+                 const SegRec<T> *__restrict__ srec, SegRec<T> *__restrict__ slab,
+                 int *__restrict__ tcnt, int *__restrict__ tovf,
+                 SegRec<T> *__restrict__ pool, int *__restrict__ pool_top) {
+  d_tile_lists<T>(act, G, start, sorg, srec, slab, tcnt, tovf, pool, pool_top);
+}
+// arguments of one suspension for k_tile_lists_multi
+template <typename T>
+struct A_tile_lists {
+  const int *act;
+  SpreadGeom G;
+  const int *start;
+  const int *sorg;
+  const SegRec<T> *srec;
+  SegRec<T> *slab;
+  int *tcnt;
+  int *tovf;
+  SegRec<T> *pool;
+  int *pool_top;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_tile_lists_multi(const A_tile_lists<T> *__restrict__ a) {
+  const A_tile_lists<T> &A = a[blockIdx.z];
+  d_tile_lists<T, const SpreadGeom &>(A.act, A.G, A.start, A.sorg, A.srec, A.slab, A.tcnt, A.tovf, A.pool, A.pool_top);
 }
 
 // Stages records [r0, r0 + n) of tile `tile` into s_rec: r0 = 0 copies the
@@ -649,9 +787,8 @@ struct Mfma<double> {
   }
 };
 
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_spread_mfma(const int *__restrict__ act, SpreadGeom G, GaussConst g,
+template <typename T, typename SG = SpreadGeom, typename GC = GaussConst>
+__device__ __forceinline__ void d_spread_mfma(const int *__restrict__ act, SG G, GC g,
                   const SegRec<T> *__restrict__ slab,
                   const SegRec<T> *__restrict__ pool,
                   const int *__restrict__ tcnt, const int *__restrict__ tovf,
@@ -738,6 +875,35 @@ __global__ void __launch_bounds__(256)
     R[2 * N + o] = acc2[r];
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_spread_mfma(const int *__restrict__ act, SpreadGeom G, GaussConst g,
+                  const SegRec<T> *__restrict__ slab,
+                  const SegRec<T> *__restrict__ pool,
+                  const int *__restrict__ tcnt, const int *__restrict__ tovf,
+                  T *__restrict__ R) {
+  d_spread_mfma<T>(act, G, g, slab, pool, tcnt, tovf, R);
+}
+// arguments of one suspension for k_spread_mfma_multi
+template <typename T>
+struct A_spread_mfma {
+  const int *act;
+  SpreadGeom G;
+  GaussConst g;
+  const SegRec<T> *slab;
+  const SegRec<T> *pool;
+  const int *tcnt;
+  const int *tovf;
+  T *R;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_spread_mfma_multi(const A_spread_mfma<T> *__restrict__ a) {
+  const A_spread_mfma<T> &A = a[blockIdx.z];
+  d_spread_mfma<T, const SpreadGeom &, const GaussConst &>(A.act, A.G, A.g, A.slab, A.pool, A.tcnt, A.tovf, A.R);
+}
 // ---------------------------------------------------------------------
 // FFT passes. Half spectrum layout C[c][kz][y][x] (x fastest): the z pass
 // writes it transposed so the x pass reads contiguous lines and the y pass
@@ -803,9 +969,8 @@ __device__ __forceinline__ void st16c(cx<T> *p, const cx<T> *in) {
 
 // z forward: XB = 2P consecutive x rows at one y, as P complex sequences
 // (row 2p in the real part, row 2p+1 in the imaginary part).
-template <typename T, int E, int LOGN>
-__global__ void __launch_bounds__(1024)
-    k_fft_z_r2c(const int *__restrict__ act, const T *__restrict__ R, cx<T> *__restrict__ C, FftPlan pl,
+template <typename T, int E, int LOGN, typename PL = FftPlan>
+__device__ __forceinline__ void d_fft_z_r2c(const int *__restrict__ act, const T *__restrict__ R, cx<T> *__restrict__ C, PL pl,
                 const cx<T> *__restrict__ twg, int nx, int ny, int P,
                 FastDiv dXB, FastDiv dP) {
   if (act && !*act) return;  // batched sweep: suspension converged
@@ -871,11 +1036,39 @@ __global__ void __launch_bounds__(1024)
     }
   }
 }
-
-// z inverse: the mirror of k_fft_z_r2c.
 template <typename T, int E, int LOGN>
 __global__ void __launch_bounds__(1024)
-    k_fft_z_c2r(const int *__restrict__ act, const cx<T> *__restrict__ C, T *__restrict__ R, FftPlan pl,
+    k_fft_z_r2c(const int *__restrict__ act, const T *__restrict__ R, cx<T> *__restrict__ C, FftPlan pl,
+                const cx<T> *__restrict__ twg, int nx, int ny, int P,
+                FastDiv dXB, FastDiv dP) {
+  d_fft_z_r2c<T, E, LOGN>(act, R, C, pl, twg, nx, ny, P, dXB, dP);
+}
+// arguments of one suspension for k_fft_z_r2c_multi
+template <typename T, int E, int LOGN>
+struct A_fft_z_r2c {
+  const int *act;
+  const T *R;
+  cx<T> *C;
+  FftPlan pl;
+  const cx<T> *twg;
+  int nx;
+  int ny;
+  int P;
+  FastDiv dXB;
+  FastDiv dP;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T, int E, int LOGN>
+__global__ void __launch_bounds__(1024)
+    k_fft_z_r2c_multi(const A_fft_z_r2c<T, E, LOGN> *__restrict__ a) {
+  const A_fft_z_r2c<T, E, LOGN> &A = a[blockIdx.z];
+  d_fft_z_r2c<T, E, LOGN, const FftPlan &>(A.act, A.R, A.C, A.pl, A.twg, A.nx, A.ny, A.P, A.dXB, A.dP);
+}
+
+// z inverse: the mirror of k_fft_z_r2c.
+template <typename T, int E, int LOGN, typename PL = FftPlan>
+__device__ __forceinline__ void d_fft_z_c2r(const int *__restrict__ act, const cx<T> *__restrict__ C, T *__restrict__ R, PL pl,
                 const cx<T> *__restrict__ twg, int nx, int ny, int P,
                 FastDiv dP) {
   if (act && !*act) return;  // batched sweep: suspension converged
@@ -936,11 +1129,38 @@ __global__ void __launch_bounds__(1024)
     }
   }
 }
-
-// y pass: one workgroup per (comp, kz, run of B consecutive x).
 template <typename T, int E, int LOGN>
 __global__ void __launch_bounds__(1024)
-    k_fft_y(const int *__restrict__ act, cx<T> *__restrict__ C, FftPlan pl, const cx<T> *__restrict__ twg,
+    k_fft_z_c2r(const int *__restrict__ act, const cx<T> *__restrict__ C, T *__restrict__ R, FftPlan pl,
+                const cx<T> *__restrict__ twg, int nx, int ny, int P,
+                FastDiv dP) {
+  d_fft_z_c2r<T, E, LOGN>(act, C, R, pl, twg, nx, ny, P, dP);
+}
+// arguments of one suspension for k_fft_z_c2r_multi
+template <typename T, int E, int LOGN>
+struct A_fft_z_c2r {
+  const int *act;
+  const cx<T> *C;
+  T *R;
+  FftPlan pl;
+  const cx<T> *twg;
+  int nx;
+  int ny;
+  int P;
+  FastDiv dP;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T, int E, int LOGN>
+__global__ void __launch_bounds__(1024)
+    k_fft_z_c2r_multi(const A_fft_z_c2r<T, E, LOGN> *__restrict__ a) {
+  const A_fft_z_c2r<T, E, LOGN> &A = a[blockIdx.z];
+  d_fft_z_c2r<T, E, LOGN, const FftPlan &>(A.act, A.C, A.R, A.pl, A.twg, A.nx, A.ny, A.P, A.dP);
+}
+
+// y pass: one workgroup per (comp, kz, run of B consecutive x).
+template <typename T, int E, int LOGN, typename PL = FftPlan>
+__device__ __forceinline__ void d_fft_y(const int *__restrict__ act, cx<T> *__restrict__ C, PL pl, const cx<T> *__restrict__ twg,
             int nx, int nh, int B, int inverse, FastDiv dB) {
   if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
@@ -984,6 +1204,33 @@ __global__ void __launch_bounds__(1024)
     }
   }
 }
+template <typename T, int E, int LOGN>
+__global__ void __launch_bounds__(1024)
+    k_fft_y(const int *__restrict__ act, cx<T> *__restrict__ C, FftPlan pl, const cx<T> *__restrict__ twg,
+            int nx, int nh, int B, int inverse, FastDiv dB) {
+  d_fft_y<T, E, LOGN>(act, C, pl, twg, nx, nh, B, inverse, dB);
+}
+// arguments of one suspension for k_fft_y_multi
+template <typename T, int E, int LOGN>
+struct A_fft_y {
+  const int *act;
+  cx<T> *C;
+  FftPlan pl;
+  const cx<T> *twg;
+  int nx;
+  int nh;
+  int B;
+  int inverse;
+  FastDiv dB;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T, int E, int LOGN>
+__global__ void __launch_bounds__(1024)
+    k_fft_y_multi(const A_fft_y<T, E, LOGN> *__restrict__ a) {
+  const A_fft_y<T, E, LOGN> &A = a[blockIdx.z];
+  d_fft_y<T, E, LOGN, const FftPlan &>(A.act, A.C, A.pl, A.twg, A.nx, A.nh, A.B, A.inverse, A.dB);
+}
 
 struct ProjConst {
   int nx, ny, nz, nh;
@@ -993,10 +1240,9 @@ struct ProjConst {
 
 // x pass + Stokes projection + inverse x pass: one workgroup per (kz, Q
 // consecutive y), the three components' contiguous x lines in LDS.
-template <typename T, int E, int LOGN>
-__global__ void __launch_bounds__(1024)
-    k_fft_x_project(const int *__restrict__ act, cx<T> *__restrict__ C, FftPlan pl,
-                    const cx<T> *__restrict__ twg, ProjConst pc, int Q,
+template <typename T, int E, int LOGN, typename PL = FftPlan, typename PC = ProjConst>
+__device__ __forceinline__ void d_fft_x_project(const int *__restrict__ act, cx<T> *__restrict__ C, PL pl,
+                    const cx<T> *__restrict__ twg, PC pc, int Q,
                     FastDiv dQ) {
   if (act && !*act) return;  // batched sweep: suspension converged
   extern __shared__ unsigned char smem_raw[];
@@ -1069,6 +1315,32 @@ __global__ void __launch_bounds__(1024)
       base[c * comp_stride + (size_t)ql * n + x] = a[sq * pn + fft_pad(x)];
     }
   }
+}
+template <typename T, int E, int LOGN>
+__global__ void __launch_bounds__(1024)
+    k_fft_x_project(const int *__restrict__ act, cx<T> *__restrict__ C, FftPlan pl,
+                    const cx<T> *__restrict__ twg, ProjConst pc, int Q,
+                    FastDiv dQ) {
+  d_fft_x_project<T, E, LOGN>(act, C, pl, twg, pc, Q, dQ);
+}
+// arguments of one suspension for k_fft_x_project_multi
+template <typename T, int E, int LOGN>
+struct A_fft_x_project {
+  const int *act;
+  cx<T> *C;
+  FftPlan pl;
+  const cx<T> *twg;
+  ProjConst pc;
+  int Q;
+  FastDiv dQ;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T, int E, int LOGN>
+__global__ void __launch_bounds__(1024)
+    k_fft_x_project_multi(const A_fft_x_project<T, E, LOGN> *__restrict__ a) {
+  const A_fft_x_project<T, E, LOGN> &A = a[blockIdx.z];
+  d_fft_x_project<T, E, LOGN, const FftPlan &, const ProjConst &>(A.act, A.C, A.pl, A.twg, A.pc, A.Q, A.dQ);
 }
 
 // ---------------------------------------------------------------------
@@ -1240,9 +1512,8 @@ __device__ __forceinline__ void ld4(const T *p, T *out) {
   if constexpr (sizeof(T) == 8) ld16(p + 2, out + 2);
 }
 
-template <typename T, int NGD>
-__global__ void __launch_bounds__(256)
-    k_interp_mfma(const int *__restrict__ act, SpreadGeom G, GaussConst g,
+template <typename T, int NGD, typename SG = SpreadGeom, typename GC = GaussConst>
+__device__ __forceinline__ void d_interp_mfma(const int *__restrict__ act, SG G, GC g,
                   const SegRec<T> *__restrict__ slab,
                   const SegRec<T> *__restrict__ pool,
                   const int *__restrict__ tcnt, const int *__restrict__ tovf,
@@ -1399,13 +1670,43 @@ __global__ void __launch_bounds__(256)
     }
   }
 }
+template <typename T, int NGD>
+__global__ void __launch_bounds__(256)
+    k_interp_mfma(const int *__restrict__ act, SpreadGeom G, GaussConst g,
+                  const SegRec<T> *__restrict__ slab,
+                  const SegRec<T> *__restrict__ pool,
+                  const int *__restrict__ tcnt, const int *__restrict__ tovf,
+                  const T *__restrict__ R, T *__restrict__ part) {
+  d_interp_mfma<T, NGD>(act, G, g, slab, pool, tcnt, tovf, R, part);
+}
+// arguments of one suspension for k_interp_mfma_multi
+template <typename T, int NGD>
+struct A_interp_mfma {
+  const int *act;
+  SpreadGeom G;
+  GaussConst g;
+  const SegRec<T> *slab;
+  const SegRec<T> *pool;
+  const int *tcnt;
+  const int *tovf;
+  const T *R;
+  T *part;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T, int NGD>
+__global__ void __launch_bounds__(256)
+    k_interp_mfma_multi(const A_interp_mfma<T, NGD> *__restrict__ a) {
+  const A_interp_mfma<T, NGD> &A = a[blockIdx.z];
+  d_interp_mfma<T, NGD, const SpreadGeom &, const GaussConst &>(A.act, A.G, A.g, A.slab, A.pool, A.tcnt, A.tovf, A.R, A.part);
+}
 
 // One thread per (segment, component): adds the segment's tile partials in
 // slot order (only the tiles its stencil meets, the ones k_interp_mfma
 // wrote) and scales them: V = dx^3 sum, Omega = -dx^3 / 2 sum
 // (particle_velocities / particle_rotations, FCMfunctions.cpp:740-808).
-template <typename T, int NGD>
-__global__ void k_interp_sum(const int *__restrict__ act, int np, SpreadGeom G,
+template <typename T, int NGD, typename SG = SpreadGeom>
+__device__ __forceinline__ void d_interp_sum(const int *__restrict__ act, int np, SG G,
                              double dx3, const int *__restrict__ origin,
                              const T *__restrict__ part,
                              double *__restrict__ V, double *__restrict__ W) {
@@ -1445,6 +1746,32 @@ __global__ void k_interp_sum(const int *__restrict__ act, int np, SpreadGeom G,
     V[3 * s + m] = dx3 * acc;
   else
     W[3 * s + m - 3] = -0.5 * dx3 * acc;
+}
+template <typename T, int NGD>
+__global__ void k_interp_sum(const int *__restrict__ act, int np, SpreadGeom G,
+                             double dx3, const int *__restrict__ origin,
+                             const T *__restrict__ part,
+                             double *__restrict__ V, double *__restrict__ W) {
+  d_interp_sum<T, NGD>(act, np, G, dx3, origin, part, V, W);
+}
+// arguments of one suspension for k_interp_sum_multi
+template <typename T, int NGD>
+struct A_interp_sum {
+  const int *act;
+  int np;
+  SpreadGeom G;
+  double dx3;
+  const int *origin;
+  const T *part;
+  double *V;
+  double *W;
+};
+// one launch for a group of suspensions with equal launch shapes:
+// blockIdx.z selects the suspension
+template <typename T, int NGD>
+__global__ void k_interp_sum_multi(const A_interp_sum<T, NGD> *__restrict__ a) {
+  const A_interp_sum<T, NGD> &A = a[blockIdx.z];
+  d_interp_sum<T, NGD, const SpreadGeom &>(A.act, A.np, A.G, A.dx3, A.origin, A.part, A.V, A.W);
 }
 
 __global__ void k_friction(const int *__restrict__ act, int n3, const double *__restrict__ F,
@@ -1560,15 +1887,23 @@ static int set_fft_lds_attr(const FftPlan &plx, const FftPlan &ply,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
     (void)hipFuncSetAttribute((const void *)k_fft_z_c2r<T, EE, LL>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    (void)hipFuncSetAttribute((const void *)k_fft_z_r2c_multi<T, EE, LL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    (void)hipFuncSetAttribute((const void *)k_fft_z_c2r_multi<T, EE, LL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
   });
   ok &= with_EL(ply, [&](auto e, auto l) {
     constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
     (void)hipFuncSetAttribute((const void *)k_fft_y<T, EE, LL>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    (void)hipFuncSetAttribute((const void *)k_fft_y_multi<T, EE, LL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
   });
   ok &= with_EL(plx, [&](auto e, auto l) {
     constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
     (void)hipFuncSetAttribute((const void *)k_fft_x_project<T, EE, LL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    (void)hipFuncSetAttribute((const void *)k_fft_x_project_multi<T, EE, LL>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
   });
   FS_CHECK(ok, FS_ERR_ARG, "no FFT register tile for these sizes");
@@ -1809,9 +2144,8 @@ void FcmImpl::release() {
   prof_used = 0;
 }
 
-template <typename T>
-int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
-                 double *V, double *W, hipStream_t st) {
+// Gaussian constants of the spread, tables and interpolation.
+static GaussConst gauss_const(const fs_fcm_params &p) {
   GaussConst g;
   g.dx = p.dx;
   double sigmadipsq = p.sigma_dip * p.sigma_dip;
@@ -1829,6 +2163,27 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   g.nx = p.nx;
   g.ny = p.ny;
   g.nz = p.nz;
+  return g;
+}
+
+// Constants of the Stokes projection (flow_solve).
+static ProjConst proj_const(const fs_fcm_params &p) {
+  ProjConst pc;
+  pc.nx = p.nx;
+  pc.ny = p.ny;
+  pc.nz = p.nz;
+  pc.nh = p.nz / 2 + 1;
+  pc.kfx = 2.0 * kPi / (p.nx * p.dx);
+  pc.kfy = 2.0 * kPi / (p.ny * p.dx);
+  pc.kfz = 2.0 * kPi / (p.nz * p.dx);
+  pc.scale = 1.0 / ((double)p.nx * p.ny * p.nz * p.mu);
+  return pc;
+}
+
+template <typename T>
+int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
+                 double *V, double *W, hipStream_t st) {
+  const GaussConst g = gauss_const(p);
 
   hipEvent_t *pset = nullptr;  // this solve's profiling events
   if (prof) {
@@ -1923,15 +2278,7 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   FS_LAUNCHED();
   mark(3);
   kmark(5);
-  ProjConst pc;
-  pc.nx = p.nx;
-  pc.ny = p.ny;
-  pc.nz = p.nz;
-  pc.nh = nh;
-  pc.kfx = 2.0 * kPi / (p.nx * p.dx);
-  pc.kfy = 2.0 * kPi / (p.ny * p.dx);
-  pc.kfz = 2.0 * kPi / (p.nz * p.dx);
-  pc.scale = 1.0 / ((double)p.nx * p.ny * p.nz * p.mu);
+  const ProjConst pc = proj_const(p);
   ok = with_EL(plx, [&](auto e, auto l) {
     constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
     hipLaunchKernelGGL((k_fft_x_project<T, EE, LL>), dim3(nh * (p.ny / xQ)),
@@ -1988,6 +2335,275 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   mark(6);
   kmark(10);
   return FS_OK;
+}
+
+// ---------------------------------------------------------------------
+// Batched sweep: the mobility solves of a group of suspensions with one
+// launch per FCM stage. Members have equal launch shapes (same precision,
+// grid, Gaussian support and tile/FFT shapes), so every stage's grid is the
+// same for each member and blockIdx.z selects the member; the kernels are
+// the single-solve kernels' bodies (the k_*_multi wrappers), so each
+// member's results are bit-identical to its own solve. Segment counts may
+// differ: segment-indexed stages launch for the largest and return past a
+// member's own count. The argument tables are built once for fixed
+// Y/F/T/V/W and loop-control pointers.
+// ---------------------------------------------------------------------
+struct A_scan_small {
+  const int *act;
+  const int *cnt;
+  int n;
+  int *start;
+};
+// start[0..n] = exclusive scan of cnt[0..n) with one workgroup per member
+// (1024-element chunks with a running carry; the bin counts of the sweep's
+// grids are a few thousand)
+__global__ void __launch_bounds__(1024)
+    k_scan_small_multi(const A_scan_small *__restrict__ a) {
+  const A_scan_small &A = a[blockIdx.z];
+  if (A.act && !*A.act) return;  // batched sweep: suspension converged
+  __shared__ int s[1024];
+  const int t = threadIdx.x;
+  int carry = 0;
+  for (int base = 0; base < A.n; base += 1024) {
+    const int i = base + t;
+    s[t] = (i < A.n) ? A.cnt[i] : 0;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int v = (t >= off) ? s[t - off] : 0;
+      __syncthreads();
+      s[t] += v;
+      __syncthreads();
+    }
+    if (i < A.n) A.start[i] = carry + ((t == 0) ? 0 : s[t - 1]);
+    carry += s[1023];
+    __syncthreads();  // s is rewritten by the next chunk
+  }
+  if (t == 0) A.start[A.n] = carry;
+}
+
+bool fcm_same_shapes(const FcmImpl &a, const FcmImpl &b) {
+  return a.precision == b.precision && a.p.nx == b.p.nx && a.p.ny == b.p.ny &&
+         a.p.nz == b.p.nz && a.p.ngd == b.p.ngd && a.mfma_spread && b.mfma_spread &&
+         a.zP == b.zP && a.yB == b.yB && a.xQ == b.xQ &&
+         a.grid_interp == b.grid_interp && a.nbins == b.nbins;
+}
+
+namespace {
+
+// One pass over the stages: with `host` set, appends each stage's argument
+// table (256-byte aligned) and records its offset; otherwise launches each
+// stage with its table in G.tab.
+template <typename T>
+int group_stages(FcmGroup &G, std::vector<char> *host, hipStream_t st) {
+  const int S = (int)G.m.size();
+  const FcmImpl &f0 = *G.m[0].fcm;
+  const fs_fcm_params &p = f0.p;
+  int npmax = 0;
+  for (auto &m : G.m) npmax = std::max(npmax, m.np);
+  const int nbins = f0.nbins, ntiles = f0.nbins, nh = p.nz / 2 + 1;
+  const size_t csz = sizeof(cx<T>);
+  size_t k = 0;  // stage index
+  // table of stage k: build it (host) or return its device address
+  auto table = [&](auto make) -> const void * {
+    using A = decltype(make(G.m[0]));
+    if (host) {
+      size_t at = (host->size() + 255) & ~(size_t)255;
+      host->resize(at + sizeof(A) * S);
+      for (int i = 0; i < S; ++i) {
+        A v = make(G.m[i]);
+        std::memcpy(host->data() + at + sizeof(A) * i, &v, sizeof(A));
+      }
+      G.off.push_back(at);
+      ++k;
+      return nullptr;
+    }
+    return static_cast<const char *>(G.tab.p) + G.off[k++];
+  };
+  const GaussConst g = gauss_const(p);
+  const ProjConst pc = proj_const(p);
+  const double dx3 = p.dx * p.dx * p.dx;
+  {
+    auto t = table([&](const FcmGroupMember &m) {
+      return A_fill_int{m.act, m.fcm->bin_count.as<int>(), nbins + 1, 0};
+    });
+    if (!host)
+      hipLaunchKernelGGL(k_fill_int_multi, dim3(ceil_div(nbins + 1, 256), 1, S),
+                         dim3(256), 0, st, (const A_fill_int *)t);
+  }
+  {
+    auto t = table([&](const FcmGroupMember &m) {
+      const FcmImpl &f = *m.fcm;
+      return A_prepare<T>{m.act, m.np, m.Y, m.F, m.T, g, f.geom.tx, f.geom.ty,
+                          f.geom.tz, f.geom.nbx, f.geom.nby, f.geom.nbz,
+                          f.origin.as<int>(), f.coef.as<T>(), f.bin.as<int>(),
+                          f.bin_count.as<int>(), f.slot.as<int>()};
+    });
+    if (!host)
+      hipLaunchKernelGGL(k_prepare_multi<T>, dim3(ceil_div(npmax, 64), 1, S),
+                         dim3(64), 0, st, (const A_prepare<T> *)t);
+  }
+  {
+    auto t = table([&](const FcmGroupMember &m) {
+      return A_scan_small{m.act, m.fcm->bin_count.as<int>(), nbins,
+                          m.fcm->bin_start.as<int>()};
+    });
+    if (!host)
+      hipLaunchKernelGGL(k_scan_small_multi, dim3(1, 1, S), dim3(1024), 0, st,
+                         (const A_scan_small *)t);
+  }
+  {
+    auto t = table([&](const FcmGroupMember &m) {
+      const FcmImpl &f = *m.fcm;
+      return A_scatter{m.np, f.bin.as<int>(), f.slot.as<int>(),
+                       f.bin_start.as<int>(), f.ids.as<int>()};
+    });
+    if (!host)
+      hipLaunchKernelGGL(k_scatter_multi, dim3(ceil_div(npmax, 256), 1, S),
+                         dim3(256), 0, st, (const A_scatter *)t);
+  }
+  {
+    auto t = table([&](const FcmGroupMember &m) {
+      const FcmImpl &f = *m.fcm;
+      return A_sort_bins<T>{m.act, nbins, f.bin_start.as<int>(), f.ids.as<int>(),
+                            f.origin.as<int>(), m.Y, m.F, m.T, p.dx, p.ngd / 2,
+                            f.srec.as<SegRec<T>>(), f.sorg.as<int>()};
+    });
+    if (!host)
+      hipLaunchKernelGGL(k_sort_bins_multi<T>, dim3(ceil_div(nbins, 64), 1, S),
+                         dim3(64), 0, st, (const A_sort_bins<T> *)t);
+  }
+  {
+    auto t = table([&](const FcmGroupMember &m) {
+      const FcmImpl &f = *m.fcm;
+      return A_tile_lists<T>{m.act, f.geom, f.bin_start.as<int>(), f.sorg.as<int>(),
+                             f.srec.as<SegRec<T>>(), f.tslab.as<SegRec<T>>(),
+                             f.tcnt.as<int>(), f.tovf.as<int>(),
+                             f.tpool.as<SegRec<T>>(), f.bin_count.as<int>() + nbins};
+    });
+    if (!host)
+      hipLaunchKernelGGL(k_tile_lists_multi<T>, dim3(ceil_div(ntiles, 4), 1, S),
+                         dim3(256), 0, st, (const A_tile_lists<T> *)t);
+  }
+  {
+    auto t = table([&](const FcmGroupMember &m) {
+      const FcmImpl &f = *m.fcm;
+      return A_spread_mfma<T>{m.act, f.geom, g, f.tslab.as<SegRec<T>>(),
+                              f.tpool.as<SegRec<T>>(), f.tcnt.as<int>(),
+                              f.tovf.as<int>(), f.R.as<T>()};
+    });
+    if (!host)
+      hipLaunchKernelGGL(k_spread_mfma_multi<T>, dim3(ntiles, 1, S), dim3(256), 0,
+                         st, (const A_spread_mfma<T> *)t);
+  }
+  bool ok = with_EL(f0.plz, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    auto t = table([&](const FcmGroupMember &m) {
+      const FcmImpl &f = *m.fcm;
+      return A_fft_z_r2c<T, EE, LL>{m.act, f.R.as<T>(), f.C.as<cx<T>>(), f.plz,
+                                    f.twz.as<cx<T>>(), p.nx, p.ny, f.zP,
+                                    make_div(2 * f.zP), make_div(f.zP)};
+    });
+    if (!host)
+      hipLaunchKernelGGL((k_fft_z_r2c_multi<T, EE, LL>),
+                         dim3(p.ny * (p.nx / (2 * f0.zP)), 3, S),
+                         dim3(p.nz / EE * f0.zP),
+                         csz * (p.nz + (size_t)f0.zP * fft_padded_len(p.nz)), st,
+                         (const A_fft_z_r2c<T, EE, LL> *)t);
+  });
+  for (int inv = 0; inv < 2 && ok; ++inv) {
+    ok &= with_EL(f0.ply, [&](auto e, auto l) {
+      constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+      auto t = table([&](const FcmGroupMember &m) {
+        const FcmImpl &f = *m.fcm;
+        return A_fft_y<T, EE, LL>{m.act, f.C.as<cx<T>>(), f.ply, f.twy.as<cx<T>>(),
+                                  p.nx, nh, f.yB, inv, make_div(f.yB)};
+      });
+      if (!host)
+        hipLaunchKernelGGL((k_fft_y_multi<T, EE, LL>),
+                           dim3(nh * (p.nx / f0.yB), 3, S), dim3(p.ny / EE * f0.yB),
+                           csz * (p.ny + (size_t)f0.yB * fft_rowpad_len(p.ny)), st,
+                           (const A_fft_y<T, EE, LL> *)t);
+    });
+    if (inv == 0)
+      ok &= with_EL(f0.plx, [&](auto e, auto l) {
+        constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+        auto t = table([&](const FcmGroupMember &m) {
+          const FcmImpl &f = *m.fcm;
+          return A_fft_x_project<T, EE, LL>{m.act, f.C.as<cx<T>>(), f.plx,
+                                            f.twx.as<cx<T>>(), pc, f.xQ,
+                                            make_div(f.xQ)};
+        });
+        if (!host)
+          hipLaunchKernelGGL((k_fft_x_project_multi<T, EE, LL>),
+                             dim3(nh * (p.ny / f0.xQ), 1, S),
+                             dim3(p.nx / EE * 3 * f0.xQ),
+                             csz * (p.nx + 3 * (size_t)f0.xQ * fft_padded_len(p.nx)),
+                             st, (const A_fft_x_project<T, EE, LL> *)t);
+      });
+  }
+  ok &= with_EL(f0.plz, [&](auto e, auto l) {
+    constexpr int EE = decltype(e)::value, LL = decltype(l)::value;
+    auto t = table([&](const FcmGroupMember &m) {
+      const FcmImpl &f = *m.fcm;
+      return A_fft_z_c2r<T, EE, LL>{m.act, f.C.as<cx<T>>(), f.R.as<T>(), f.plz,
+                                    f.twz.as<cx<T>>(), p.nx, p.ny, f.zP,
+                                    make_div(f.zP)};
+    });
+    if (!host)
+      hipLaunchKernelGGL((k_fft_z_c2r_multi<T, EE, LL>),
+                         dim3(p.ny * (p.nx / (2 * f0.zP)), 3, S),
+                         dim3(p.nz / EE * f0.zP),
+                         csz * (p.nz + (size_t)f0.zP * fft_padded_len(p.nz)), st,
+                         (const A_fft_z_c2r<T, EE, LL> *)t);
+  });
+  ok &= with_NGD(p.ngd, [&](auto ng) {
+    constexpr int NG = decltype(ng)::value;
+    auto t = table([&](const FcmGroupMember &m) {
+      const FcmImpl &f = *m.fcm;
+      return A_interp_mfma<T, NG>{m.act, f.geom, g, f.tslab.as<SegRec<T>>(),
+                                  f.tpool.as<SegRec<T>>(), f.tcnt.as<int>(),
+                                  f.tovf.as<int>(), f.R.as<T>(), f.ipart.as<T>()};
+    });
+    if (!host)
+      hipLaunchKernelGGL((k_interp_mfma_multi<T, NG>), dim3(f0.grid_interp, 1, S),
+                         dim3(256), 0, st, (const A_interp_mfma<T, NG> *)t);
+    auto t2 = table([&](const FcmGroupMember &m) {
+      const FcmImpl &f = *m.fcm;
+      return A_interp_sum<T, NG>{m.act, m.np, f.geom, dx3, f.origin.as<int>(),
+                                 f.ipart.as<T>(), m.V, m.W};
+    });
+    if (!host)
+      hipLaunchKernelGGL((k_interp_sum_multi<T, NG>),
+                         dim3(ceil_div(6L * npmax, 256), 1, S), dim3(256), 0, st,
+                         (const A_interp_sum<T, NG> *)t2);
+  });
+  FS_CHECK(ok, FS_ERR_ARG, "group FCM dispatch failed");
+  if (!host) FS_LAUNCHED();
+  return FS_OK;
+}
+
+}  // namespace
+
+int FcmGroup::build() {
+  FS_CHECK(!m.empty(), FS_ERR_ARG, "empty FCM group");
+  for (auto &x : m)
+    FS_CHECK(fcm_same_shapes(*m[0].fcm, *x.fcm) && x.np >= 1 &&
+                 x.np <= x.fcm->max_np,
+             FS_ERR_ARG, "FCM group members differ in launch shapes or sizes");
+  std::vector<char> host;
+  off.clear();
+  int rc = m[0].fcm->precision == FS_FP64 ? group_stages<double>(*this, &host, nullptr)
+                                          : group_stages<float>(*this, &host, nullptr);
+  FS_TRY(rc);
+  FS_TRY(tab.alloc(host.size()));
+  FS_HIP(hipMemcpy(tab.p, host.data(), host.size(), hipMemcpyHostToDevice));
+  return FS_OK;
+}
+
+int FcmGroup::solve(hipStream_t st) {
+  FS_CHECK(tab.p && !off.empty(), FS_ERR_STATE, "FCM group not built");
+  return m[0].fcm->precision == FS_FP64 ? group_stages<double>(*this, nullptr, st)
+                                        : group_stages<float>(*this, nullptr, st);
 }
 
 // add the stage times of every solve since the last call into acc7

@@ -68,4 +68,26 @@ struct FcmImpl {
           double *V, double *W, hipStream_t st);
 };
 
+// Mobility solves of several suspensions with one launch per FCM stage
+// (batched sweep). Members need equal launch shapes (fcm_same_shapes) and
+// the matrix-core tile path; build() fills the per-member argument tables
+// in device memory once (fixed Y/F/T/V/W and loop-control pointers), then
+// each solve() launches every stage once over the whole group.
+struct FcmGroupMember {
+  FcmImpl *fcm = nullptr;
+  const int *act = nullptr;  // loop-control flag (nullptr: always run)
+  int np = 0;
+  const double *Y = nullptr, *F = nullptr, *T = nullptr;
+  double *V = nullptr, *W = nullptr;
+};
+struct FcmGroup {
+  std::vector<FcmGroupMember> m;
+  dbuf tab;                 // argument tables
+  std::vector<size_t> off;  // offset of each stage's table
+  int build();
+  int solve(hipStream_t st);
+  void release() { tab.release(); }
+};
+bool fcm_same_shapes(const FcmImpl &a, const FcmImpl &b);
+
 }  // namespace fs

@@ -1980,15 +1980,21 @@ void SystemImpl::set_device_control(const int *act, const int *ntp) {
   cb.act = act;
 }
 
-int SystemImpl::begin_step_dev(int *status) {
+// part: -1 the whole sequence; 0 up to the mobility solve; 1 after it (a
+// batched sweep with fused FCM stages solves every suspension's mobility
+// between the two).
+int SystemImpl::begin_step_dev(int *status, int part) {
   const int nt = 1;  // read from d.ntp on the device
   const int *ntp = d.ntp;
   set_device_control(nullptr, ntp);
-  FS_TRY(initial_guess());
-  FS_TRY(forces(nt));
-  FS_TRY(build_jacobian_dev(nt));
-  FS_TRY(barrier(nt, 1, nullptr));
-  FS_TRY(mobility(nt));
+  if (part != 1) {
+    FS_TRY(initial_guess());
+    FS_TRY(forces(nt));
+    FS_TRY(build_jacobian_dev(nt));
+    FS_TRY(barrier(nt, 1, nullptr));
+  }
+  if (part == -1) FS_TRY(mobility(nt));
+  if (part == 0) return FS_OK;
   unsigned long long *mb = flags.as<unsigned long long>();
   hipLaunchKernelGGL(k_resid_reset, dim3(1), dim3(1), 0, st,
                      (const int *)nullptr, mb);
@@ -2008,16 +2014,22 @@ int SystemImpl::begin_step_dev(int *status) {
   return FS_OK;
 }
 
-int SystemImpl::iteration_dev(int *status) {
+int SystemImpl::iteration_dev(int *status, int part) {
   const int nt = 1;  // read from d.ntp on the device
   const int *ntp = d.ntp;
   const int *act = ctl.as<int>() + 1;
   const size_t n6 = 6 * (size_t)np;
   set_device_control(act, ntp);
-  FS_TRY(update(Upd));
-  FS_TRY(forces(nt));
-  FS_TRY(barrier(nt, 0, nullptr));
-  FS_TRY(mobility(nt));
+  if (part != 1) {
+    FS_TRY(update(Upd));
+    FS_TRY(forces(nt));
+    FS_TRY(barrier(nt, 0, nullptr));
+  }
+  if (part == -1) FS_TRY(mobility(nt));
+  if (part == 0) {
+    set_device_control(nullptr, ntp);
+    return FS_OK;
+  }
   unsigned long long *mb = flags.as<unsigned long long>();
   hipLaunchKernelGGL(k_resid_reset, dim3(1), dim3(1), 0, st, act, mb);
   hipLaunchKernelGGL(k_residual, dim3(ceil_div(np, 256)), dim3(256), 0, st, d,

@@ -558,6 +558,15 @@ struct fs_batch {
   int *dnt = nullptr, *dstat = nullptr;  // device: step number, 3 ints each
   int *hnt = nullptr, *hstat = nullptr;  // pinned host mirrors
   int maxiter = 0;
+  // fused mode (begin and iteration phases): each suspension's phase is
+  // captured in two halves around its mobility solve, and the solves of
+  // all suspensions run in between as one launch per FCM stage per group
+  // of equal launch shapes (FcmGroup), captured in one graph per phase
+  bool fusable = false, fused = false;
+  std::vector<hipGraphExec_t> spre[2], spost[2];
+  hipGraphExec_t gfcm[2] = {nullptr, nullptr};
+  std::vector<fs::FcmGroup> groups[2];
+  hipEvent_t ev1 = nullptr;
 };
 
 namespace {
@@ -568,6 +577,13 @@ void batch_release(fs_batch *b) {
     if (b->g[k]) (void)hipGraphDestroy(b->g[k]);
     for (hipGraphExec_t e : b->sge[k]) (void)hipGraphExecDestroy(e);
   }
+  for (int k = 0; k < 2; ++k) {
+    for (hipGraphExec_t e : b->spre[k]) (void)hipGraphExecDestroy(e);
+    for (hipGraphExec_t e : b->spost[k]) (void)hipGraphExecDestroy(e);
+    if (b->gfcm[k]) (void)hipGraphExecDestroy(b->gfcm[k]);
+    for (fs::FcmGroup &g : b->groups[k]) g.release();
+  }
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
   for (hipStream_t s2 : b->lst) {
     (void)hipStreamSynchronize(s2);
     (void)hipStreamDestroy(s2);
@@ -619,6 +635,93 @@ int batch_capture(fs_batch *b, int kind) {
   return FS_OK;
 }
 
+// captures one half (part 0: before the mobility solve, 1: after it) of
+// phase `kind` (0 begin, 1 iteration) of every suspension
+int batch_capture_half(fs_batch *b, int kind, int part) {
+  auto &out = part == 0 ? b->spre[kind] : b->spost[kind];
+  for (size_t i = 0; i < b->sys.size(); ++i) {
+    SystemImpl &S = b->sys[i]->impl;
+    FS_HIP(hipStreamSynchronize(S.st));
+    S.set_device_control(nullptr, b->dnt);
+    FS_HIP(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal));
+    int *status = b->dstat + 3 * i;
+    int rc = kind == 0 ? S.begin_step_dev(status, part)
+                       : S.iteration_dev(status, part);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(S.st, &gr);
+    S.set_device_control(nullptr, nullptr);
+    if (rc != FS_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    FS_CHECK(e == hipSuccess && gr, FS_ERR_HIP, "graph capture failed: %s",
+             hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e2 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    FS_CHECK(e2 == hipSuccess, FS_ERR_HIP, "graph instantiation failed: %s",
+             hipGetErrorString(e2));
+    out.push_back(ex);
+  }
+  return FS_OK;
+}
+
+// groups of equal FCM launch shapes for phase `kind` (loop control: none
+// in the begin phase, each suspension's iterating flag in the iteration
+// phase), and the graph that solves them all on b->st
+int batch_build_fcm(fs_batch *b, int kind) {
+  auto &groups = b->groups[kind];
+  for (fs_system *h : b->sys) {
+    SystemImpl &S = h->impl;
+    fs::FcmGroupMember m;
+    m.fcm = &S.fcm;
+    m.act = kind == 0 ? nullptr : S.ctl.as<int>() + 1;
+    m.np = S.np;
+    m.Y = S.d.Ya;
+    m.F = S.d.Fa;
+    m.T = S.d.Ta;
+    m.V = S.d.V;
+    m.W = S.d.W;
+    fs::FcmGroup *g = nullptr;
+    for (fs::FcmGroup &x : groups)
+      if (fs::fcm_same_shapes(*x.m[0].fcm, S.fcm)) g = &x;
+    if (!g) {
+      groups.emplace_back();
+      g = &groups.back();
+    }
+    g->m.push_back(m);
+  }
+  for (fs::FcmGroup &g : groups) FS_TRY(g.build());
+  FS_HIP(hipStreamSynchronize(b->st));
+  FS_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  int rc = FS_OK;
+  for (fs::FcmGroup &g : groups)
+    if (rc == FS_OK) rc = g.solve(b->st);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(b->st, &gr);
+  if (rc != FS_OK) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  FS_CHECK(e == hipSuccess && gr, FS_ERR_HIP, "graph capture failed: %s",
+           hipGetErrorString(e));
+  hipError_t e2 = hipGraphInstantiate(&b->gfcm[kind], gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  FS_CHECK(e2 == hipSuccess, FS_ERR_HIP, "graph instantiation failed: %s",
+           hipGetErrorString(e2));
+  return FS_OK;
+}
+
+// whether every suspension can join a fused FCM group: the matrix-core
+// tile path, and the FCM mobility (not the friction start-up) in every step
+bool batch_fusable(fs_batch *b) {
+  for (fs_system *h : b->sys) {
+    const SystemImpl &S = h->impl;
+    if (!S.fcm.mfma_spread || S.p.initial_friction_steps != 0) return false;
+  }
+  return true;
+}
+
 int batch_status(fs_batch *b) {
   FS_HIP(hipMemcpyAsync(b->hstat, b->dstat, 3 * sizeof(int) * b->sys.size(),
                         hipMemcpyDeviceToHost, b->st));
@@ -662,6 +765,15 @@ extern "C" int fs_batch_create(fs_system **sys, int n, fs_batch **out) {
     b->maxiter = std::max(b->maxiter, sys[i]->impl.p.broyden_maxiter);
   for (int k = 0; k < 3 && rc == FS_OK; ++k) rc = batch_capture(b, k);
   if (rc != FS_OK) return fail(rc);
+  b->fusable = batch_fusable(b);
+  if (b->fusable) {
+    for (int k = 0; k < 2 && rc == FS_OK; ++k) {
+      rc = batch_capture_half(b, k, 0);
+      if (rc == FS_OK) rc = batch_capture_half(b, k, 1);
+      if (rc == FS_OK) rc = batch_build_fcm(b, k);
+    }
+    if (rc != FS_OK) return fail(rc);
+  }
   // measured on a 64-suspension sweep: 4 streams 2.4x sequential stepping,
   // one parent graph 0.8x (its child graphs ran one after another)
   rc = fs_batch_set_lanes(b, std::min(n, 4));
@@ -683,6 +795,29 @@ namespace {
 // one phase for every suspension, ordered after everything already on b->st
 // and before what follows on it
 int batch_phase(fs_batch *b, int kind) {
+  if (b->fused && kind < 2) {
+    // halves before the solves on the lanes, the fused FCM graph on b->st,
+    // halves after the solves on the lanes
+    const int lanes = std::max(1, b->lanes);
+    FS_HIP(hipEventRecord(b->ev0, b->st));
+    for (int l = 0; l < lanes; ++l) FS_HIP(hipStreamWaitEvent(b->lst[l], b->ev0, 0));
+    for (size_t i = 0; i < b->sys.size(); ++i)
+      FS_HIP(hipGraphLaunch(b->spre[kind][i], b->lst[i % lanes]));
+    for (int l = 0; l < lanes; ++l) {
+      FS_HIP(hipEventRecord(b->lev[l], b->lst[l]));
+      FS_HIP(hipStreamWaitEvent(b->st, b->lev[l], 0));
+    }
+    FS_HIP(hipGraphLaunch(b->gfcm[kind], b->st));
+    FS_HIP(hipEventRecord(b->ev1, b->st));
+    for (int l = 0; l < lanes; ++l) FS_HIP(hipStreamWaitEvent(b->lst[l], b->ev1, 0));
+    for (size_t i = 0; i < b->sys.size(); ++i)
+      FS_HIP(hipGraphLaunch(b->spost[kind][i], b->lst[i % lanes]));
+    for (int l = 0; l < lanes; ++l) {
+      FS_HIP(hipEventRecord(b->lev[l], b->lst[l]));
+      FS_HIP(hipStreamWaitEvent(b->st, b->lev[l], 0));
+    }
+    return FS_OK;
+  }
   if (b->lanes == 0) {
     FS_HIP(hipGraphLaunch(b->ge[kind], b->st));
     return FS_OK;
@@ -701,11 +836,22 @@ int batch_phase(fs_batch *b, int kind) {
 
 }  // namespace
 
+extern "C" int fs_batch_set_fused(fs_batch *b, int on) {
+  FS_CHECK(b, FS_ERR_ARG, "null batch");
+  FS_CHECK(!on || b->fusable, FS_ERR_ARG,
+           "fused FCM stages need every suspension on the matrix-core tile "
+           "path and with initial_friction_steps == 0");
+  FS_HIP(hipStreamSynchronize(b->st));
+  b->fused = on != 0;
+  return FS_OK;
+}
+
 extern "C" int fs_batch_set_lanes(fs_batch *b, int lanes) {
   FS_CHECK(b && lanes >= 0 && lanes <= 16, FS_ERR_ARG,
            "lanes must be in [0, 16]");
   FS_HIP(hipStreamSynchronize(b->st));
-  while ((int)b->lst.size() < lanes) {
+  const int want = std::max(lanes, 1);  // the fused mode uses at least one
+  while ((int)b->lst.size() < want) {
     hipStream_t s2 = nullptr;
     hipEvent_t e = nullptr;
     FS_HIP(hipStreamCreate(&s2));
@@ -714,6 +860,7 @@ extern "C" int fs_batch_set_lanes(fs_batch *b, int lanes) {
     b->lev.push_back(e);
   }
   if (!b->ev0) FS_HIP(hipEventCreateWithFlags(&b->ev0, hipEventDisableTiming));
+  if (!b->ev1) FS_HIP(hipEventCreateWithFlags(&b->ev1, hipEventDisableTiming));
   b->lanes = lanes;
   return FS_OK;
 }

@@ -113,8 +113,8 @@ struct SystemImpl {
                   int noff);
   int build_jacobian_dev(int nt);
   void set_device_control(const int *act, const int *ntp);
-  int begin_step_dev(int *status);  // status: where to publish ctl
-  int iteration_dev(int *status);
+  int begin_step_dev(int *status, int part = -1);  // status: where to publish ctl
+  int iteration_dev(int *status, int part = -1);
   int end_step_dev(int *status);
 };
 

@@ -507,6 +507,15 @@ int fs_batch_timestep(fs_batch *b, int nt, int *iters);
  * hardware queues (default min(n, 4)); 0 launches one parent graph per
  * phase. Results do not depend on the mode. */
 int fs_batch_set_lanes(fs_batch *b, int lanes);
+/* Fused FCM stages (off by default; available when every system runs the
+ * matrix-core tile path and initial_friction_steps == 0): in the begin and
+ * iteration phases the mobility solves of all systems run as one launch
+ * per FCM stage for each group of systems with equal launch shapes
+ * (precision, grid, support, tile and FFT shapes), between the systems'
+ * own graphs for the work before and after the solve (on the lanes).
+ * Results do not depend on it; FS_ERR_ARG if on and a system cannot
+ * join. */
+int fs_batch_set_fused(fs_batch *b, int on);
 
 #ifdef __cplusplus
 }

@@ -38,7 +38,10 @@ def test_batched_equals_unbatched(cuda, precision):
             group.append(s)
     batch = api.Batch(bat)
     for nt in range(1, 5):
-        batch.set_lanes(0 if nt % 2 else 3)  # parent graph / 3 streams
+        # fused FCM stages on 1 stream, per-system graphs on 3 streams, one
+        # parent graph, fused on 3 streams
+        batch.set_fused(nt in (1, 4))
+        batch.set_lanes(0 if nt in (1, 3) else 3)
         ib = batch.timestep(nt)
         isolo = [s.timestep(nt) for s in solo]
         print("step", nt, "batched iters", ib, "solo", isolo)
@@ -52,6 +55,39 @@ def test_batched_equals_unbatched(cuda, precision):
     for a, b in zip(solo, bat):
         assert a.timestep(5) == b.timestep(5)
         assert np.array_equal(a.get_state()["X"], b.get_state()["X"])
+
+
+@pytest.mark.parametrize("precision", [abi.FS_FP32, abi.FS_FP64])
+def test_fused_groups_equal_unbatched(cuda, precision):
+    """Fused FCM stages with groups of several members: three suspensions
+    on 48^3 grids (run-time FFT plans) and two on 64^3 (compile-time plans),
+    different filament and segment counts, stepped as one batch with fused
+    stages and one after another: same iteration counts, same bits."""
+    pts = []
+    for k, grid in enumerate((48, 64, 48, 64, 48)):
+        fp, sp, nws, props, st, info = params.sweep_point(
+            300 + k, grids=(grid,), nfil_range=(3, 14), nw_range=(6, 20),
+            precision=precision, tol=1e-9, dt=0.01)
+        pts.append((sp, nws, props, st, info))
+    print([(p[4]["grid"], p[4]["segments"]) for p in pts])
+    solo, bat = [], []
+    for sp, nws, props, st, info in pts:
+        for group in (solo, bat):
+            s = api.System(sp, nws, props)
+            s.set_state(**st)
+            group.append(s)
+    batch = api.Batch(bat)
+    batch.set_fused(True)
+    for nt in range(1, 4):
+        ib = batch.timestep(nt)
+        isolo = [s.timestep(nt) for s in solo]
+        print("step", nt, "fused iters", ib, "solo", isolo)
+        assert ib == isolo
+    for a, b in zip(solo, bat):
+        sa, sb = a.get_state(), b.get_state()
+        for k in FIELDS:
+            assert np.array_equal(sa[k], sb[k]), k
+    batch.close()
 
 
 def test_batch_rejects_friction_steps(cuda):

@@ -73,8 +73,10 @@ def test_batched_sweep_64_suspensions(cuda):
     """The 64 sweep points of the batched benchmark (grids 32^3-128^3,
     8-200 filaments of 15-40 segments capped by box size, KB log-uniform in
     [0.1, 10], gravity N(0, 0.2^2), FP32 FCM + FP64 Broyden, TOL 1e-6):
-    two steps as one batch and two steps one suspension after another from
-    the same states give the same iteration counts and the same bits."""
+    two steps as one batch (per-suspension graphs, and with the FCM stages
+    fused across suspensions) and two steps one suspension after another
+    from the same states give the same iteration counts and the same
+    bits."""
     pts = [params.sweep_point(4000 + k, precision=abi.FS_FP32, tol=1e-6)
            for k in range(64)]
 
@@ -99,7 +101,19 @@ def test_batched_sweep_64_suspensions(cuda):
         for k in ("X", "q", "lam"):
             assert np.array_equal(sa[k], sb[k]), k
     batch.close()
-    for s in seq + bat:
+    # the same two steps with the FCM stages fused across the suspensions
+    # (one launch per stage for each group of equal grids)
+    fus = fresh()
+    batch = api.Batch(fus)
+    batch.set_fused(True)
+    its_fus = [batch.timestep(nt) for nt in (1, 2)]
+    assert its_fus == its_seq
+    for a, b in zip(seq, fus):
+        sa, sb = a.get_state(), b.get_state()
+        for k in ("X", "q", "lam"):
+            assert np.array_equal(sa[k], sb[k]), k
+    batch.close()
+    for s in seq + bat + fus:
         s.close()
 
 
