@@ -947,6 +947,18 @@ __device__ __forceinline__ void ld16(const T *p, T *out) {
   const V v = *reinterpret_cast<const V *>(p);
   __builtin_memcpy(out, &v, 16);
 }
+// Non-temporal 16-byte load, for data read once that the solve does not
+// read again before overwriting it (the z passes' inputs): the hint keeps it
+// from displacing the grid the next pass reads from the Infinity Cache
+// (measured at 256^3 FP32: spread, z passes and forward y pass 0.039 ms
+// faster per solve; the interpolation, which shares 128-byte lines between
+// neighbouring tiles through L2, loads normally).
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ void ld16_nt(const T *p, T *out) {
+  const i32x4 v = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(p));
+  __builtin_memcpy(out, &v, 16);
+}
 template <typename T>
 __device__ __forceinline__ void st16(T *p, const T *in) {
   using V = typename Vec16<T>::type;
@@ -994,8 +1006,8 @@ __device__ __forceinline__ void d_fft_z_r2c(const int *__restrict__ act, const T
       const int p = e >> LV, z = (e & ((1 << LV) - 1)) * VW;
       const T *r0 = Rc + ((size_t)(x0 + 2 * p) * ny + y) * n + z;
       T u0[VW], u1[VW];
-      ld16(r0, u0);
-      ld16(r0 + (size_t)ny * n, u1);
+      ld16_nt(r0, u0);
+      ld16_nt(r0 + (size_t)ny * n, u1);
       cx<T> *dst = a + p * pn + fft_pad(z);
 #pragma unroll
       for (int j = 0; j < VW; ++j) dst[j] = cx<T>{u0[j], u1[j]};
@@ -1089,8 +1101,8 @@ __device__ __forceinline__ void d_fft_z_c2r(const int *__restrict__ act, const c
     cx<T> A, B;
     if constexpr (LOGN > 0) {  // x0 + 2p even: the pair is 16-byte aligned
       T u[4];
-      ld16(reinterpret_cast<const T *>(src), u);
-      if constexpr (Vec16<T>::n == 2) ld16(reinterpret_cast<const T *>(src) + 2, u + 2);
+      ld16_nt(reinterpret_cast<const T *>(src), u);
+      if constexpr (Vec16<T>::n == 2) ld16_nt(reinterpret_cast<const T *>(src) + 2, u + 2);
       A = {u[0], u[1]};
       B = {u[2], u[3]};
     } else {
