@@ -2102,6 +2102,7 @@ int FcmImpl::set_geometry() {
   }
   // one more counter after the bins: the tile lists' pool top
   FS_TRY(bin_count.alloc((size_t)(nbins + 1) * sizeof(int)));
+  FS_HIP(hipMemset(bin_count.p, 0, (size_t)(nbins + 1) * sizeof(int)));
   FS_TRY(bin_start.alloc((size_t)(nbins + 1) * sizeof(int)));
   FS_TRY(scan_tmp.alloc((size_t)(nbins / kScanBlock + 4) * sizeof(int)));
   // FFT pass geometry: z rows per workgroup (even divisor of nx), y-pass run
@@ -2222,9 +2223,14 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
   };
   mark(0);
   kmark(0);
-  // bin counting sort
-  hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbins + 1, 256)), dim3(256), 0, st,
-                     act, bin_count.as<int>(), nbins + 1, 0);
+  // bin counting sort. Up to kScanOneMax bins the counts are scanned by one
+  // workgroup that also zeroes them (and the pool counter after them) for
+  // the next solve: they start zeroed (set_geometry) and every solve leaves
+  // them so, so no fill pass is needed.
+  const bool scan_one = nbins <= kScanOneMax;
+  if (!scan_one)
+    hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbins + 1, 256)), dim3(256), 0, st,
+                       act, bin_count.as<int>(), nbins + 1, 0);
   if (!mfma_spread)  // the matrix-core kernels form their weights per tile
     hipLaunchKernelGGL(k_prepare_tab<T>, dim3(ceil_div((long)np * 3 * p.ngd, 256)),
                        dim3(256), 0, st, act, np, Y, g, tab.as<T>());
@@ -2232,8 +2238,11 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
                      Y, F, Tq, g, geom.tx, geom.ty, geom.tz, geom.nbx, geom.nby,
                      geom.nbz, origin.as<int>(), coef.as<T>(), bin.as<int>(),
                      bin_count.as<int>(), slot.as<int>());
-  FS_TRY(device_scan(bin_count.as<int>(), nbins, bin_start.as<int>(),
-                     scan_tmp.as<int>(), st));
+  if (scan_one)
+    FS_TRY(device_scan_one(act, bin_count.as<int>(), nbins, bin_start.as<int>(), 1, st));
+  else
+    FS_TRY(device_scan(bin_count.as<int>(), nbins, bin_start.as<int>(),
+                       scan_tmp.as<int>(), st));
   hipLaunchKernelGGL(k_scatter, dim3(ceil_div(np, 256)), dim3(256), 0, st, np,
                      bin.as<int>(), slot.as<int>(), bin_start.as<int>(),
                      ids.as<int>());
@@ -2362,16 +2371,21 @@ int FcmImpl::run(int np, const double *Y, const double *F, const double *Tq,
 // ---------------------------------------------------------------------
 struct A_scan_small {
   const int *act;
-  const int *cnt;
+  int *cnt;
   int n;
   int *start;
 };
-// start[0..n] = exclusive scan of cnt[0..n) with one workgroup per member
-// (1024-element chunks with a running carry; the bin counts of the sweep's
-// grids are a few thousand)
+// start[0..n] = exclusive scan of cnt[0..n) with one workgroup per member;
+// up to kScanOneMax bins the single-solve scan, which leaves cnt[0..n]
+// zeroed as the single-solve path expects, else 1024-element chunks with a
+// running carry
 __global__ void __launch_bounds__(1024)
     k_scan_small_multi(const A_scan_small *__restrict__ a) {
   const A_scan_small &A = a[blockIdx.z];
+  if (A.n <= kScanOneMax) {
+    d_scan_one(A.act, A.cnt, A.n, A.start, 1);
+    return;
+  }
   if (A.act && !*A.act) return;  // batched sweep: suspension converged
   __shared__ int s[1024];
   const int t = threadIdx.x;

@@ -172,23 +172,49 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// column sums (bias gradients), fixed order
-__global__ void k_colsum(const float *__restrict__ g, int rows, int cols,
-                         float *__restrict__ out) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// column sums (bias gradients) in a fixed order: a workgroup of 16
+// wavefronts per 64 columns, wavefront w adds the w-th contiguous sixteenth
+// of the rows (one coalesced row segment per load), then the 16 partial
+// sums are added in wavefront order
+__global__ void __launch_bounds__(1024)
+    k_colsum(const float *__restrict__ g, int rows, int cols,
+             float *__restrict__ out) {
+  __shared__ float part[16][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const int per = (rows + 15) / 16, r0 = min(rows, w * per),
+            r1 = min(rows, r0 + per);
   float s = 0.f;
-  for (int r = 0; r < rows; ++r) s += g[(long)r * cols + c];
-  out[c] = s;
+  if (c < cols) {
+#pragma unroll 8
+    for (int r = r0; r < r1; ++r) s += g[(long)r * cols + c];
+  }
+  part[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && c < cols) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += part[k][lane];
+    out[c] = t;
+  }
 }
 
-__global__ void k_sum(const float *__restrict__ p, int n,
-                      float *__restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    float s = 0.f;
-    for (int i = 0; i < n; ++i) s += p[i];
-    *out = s / 1.0f;
+// sum of n partials in a fixed order: thread t adds the t-th contiguous
+// run of ceil(n / 256), then a tree over the 256 run sums
+__global__ void __launch_bounds__(256)
+    k_sum(const float *__restrict__ p, int n, float *__restrict__ out) {
+  __shared__ float red[256];
+  const int t = threadIdx.x, per = (n + 255) / 256;
+  const int lo = min(n, t * per), hi = min(n, lo + per);
+  float s = 0.f;
+  for (int i = lo; i < hi; ++i) s += p[i];
+  red[t] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
   }
+  if (t == 0) *out = red[0];
 }
 
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g,
@@ -378,7 +404,8 @@ struct MlpImpl {
     const int nblk = ceil_div(n, 256);
     hipLaunchKernelGGL(k_mse_grad, dim3(nblk), dim3(256), 0, st, Y, T, tau(),
                        B, nout, G, Gb, partial);
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1), 0, st, partial, nblk, lossd);
+    if (loss_host)
+      hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, st, partial, nblk, lossd);
     float *g = grads.as<float>();
     const uint16_t *W = wbf.as<uint16_t>();
     // layer 3: dW3 = Gb^T H3 (out x width), db3 = colsum(G), dH3 = Gb W3
@@ -392,7 +419,7 @@ struct MlpImpl {
       // dW_l(o, i) = sum_b g(b, o) h(b, i)
       gemm(out, in, B, gb, 1, gcols, hin, ldin, 1, EPI_STORE, nullptr,
            g + woff[l], in, nullptr, 0, nullptr, 0);
-      hipLaunchKernelGGL(k_colsum, dim3(ceil_div(out, 64)), dim3(64), 0, st,
+      hipLaunchKernelGGL(k_colsum, dim3(ceil_div(out, 64)), dim3(1024), 0, st,
                          gcur, B, out, g + boff[l]);
       if (l > 0) {
         // dh(b, i) = sum_o g(b, o) W_l[o][i]; dz = dh * (z_{l-1} > 0)
