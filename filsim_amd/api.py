@@ -47,7 +47,7 @@ class Fcm:
     def close(self):
         if self.h:
             self.lib.fs_fcm_destroy(self.h)
-            self.h = C.c_void_p()
+            self.h = None
 
     __del__ = close
 
@@ -137,7 +137,7 @@ class Colbar:
     def close(self):
         if self.h:
             self.lib.fs_colbar_destroy(self.h)
-            self.h = C.c_void_p()
+            self.h = None
 
     __del__ = close
 
@@ -216,7 +216,7 @@ class System:
         it, since the batch's captured launches use its buffers."""
         if self.h:
             abi.check(self.lib.fs_system_destroy(self.h))
-            self.h = C.c_void_p()
+            self.h = None
 
     def __del__(self):
         try:
@@ -420,7 +420,7 @@ class Batch:
     def close(self):
         if self.h:
             self.lib.fs_batch_destroy(self.h)
-            self.h = C.c_void_p()
+            self.h = None
 
     __del__ = close
 
@@ -481,7 +481,7 @@ class Mlp:
     def close(self):
         if self.h:
             self.lib.fs_mlp_destroy(self.h)
-            self.h = C.c_void_p()
+            self.h = None
 
     __del__ = close
 

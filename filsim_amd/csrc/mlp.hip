@@ -41,51 +41,64 @@ __device__ __forceinline__ float bf2f(uint16_t b) {
 enum { EPI_STORE = 0, EPI_HIDDEN = 1, EPI_OUT = 2, EPI_DRELU = 3, EPI_ADD = 4 };
 
 // C(m,n) = sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) =
-// B[k*sbk + n*sbn] (BF16). One wavefront per 32x32 output tile (2x2 MFMA
-// 16x16 blocks), four wavefronts per workgroup. MFMA 16x16x16 lane layout:
-// lane l feeds A[l%16][4(l/16)..+3] and B[4(l/16)..+3][l%16] and receives
-// D[4(l/16)+i][l%16], i < 4.
+// B[k*sbk + n*sbn] (BF16). One workgroup per 32x32 output tile (2x2 MFMA
+// 16x16 blocks): wavefront w takes the w-th quarter of the K steps of 16,
+// and the four partial tiles are added through LDS in wavefront order
+// before the epilogue (a fixed order: results do not depend on timing).
+// MFMA 16x16x16 lane layout: lane l feeds A[l%16][4(l/16)..+3] and
+// B[4(l/16)..+3][l%16] and receives D[4(l/16)+i][l%16], i < 4. AV / BV: A
+// (B) is contiguous along k with 8-byte aligned rows (columns), so a lane's
+// four k values are one 8-byte load.
 // Epilogues: STORE  Cf = acc
 //            HIDDEN Zf = acc + bias, Hb = BF16(relu(Zf))
 //            OUT    Cf = acc + bias
 //            DRELU  g = acc * (Zf > 0): Cf = g, Hb = BF16(g)
 //            ADD    Cf += acc
+template <bool AV, bool BV>
 __global__ void __launch_bounds__(256)
     k_gemm_bf16(int M, int N, int K, const uint16_t *__restrict__ A, long sam,
                 long sak, const uint16_t *__restrict__ B, long sbk, long sbn,
                 int epi, const float *__restrict__ bias, float *__restrict__ Cf,
                 long ldc, uint16_t *__restrict__ Hb, long ldh,
                 const float *__restrict__ Zin, long ldz) {
+  __shared__ float red[4][32 * 33];  // the waves' partial tiles
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tilesN = (N + 31) / 32;
-  const int tile = blockIdx.x * 4 + wave;
-  const int tm = tile / tilesN, tn = tile - tm * tilesN;
-  if (tm * 32 >= M) return;  // whole wavefront leaves together
+  const int tm = blockIdx.x / tilesN, tn = blockIdx.x - tm * tilesN;
   const int m0 = tm * 32, n0 = tn * 32;
   const int r = lane & 15, kq = (lane >> 4) * 4;
+  const int ks = (K + 15) / 16;
+  const int s0 = ks * wave / 4, s1 = ks * (wave + 1) / 4;
   f4v acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  for (int s = s0; s < s1; ++s) {
+    const int k = 16 * s + kq;
     s4v a[2], b[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int m = m0 + 16 * i + r;
+      if (AV && m < M && k + 4 <= K) {
+        a[i] = *reinterpret_cast<const s4v *>(A + (long)m * sam + k);
+      } else {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int k = k0 + kq + q;
-        a[i][q] = (m < M && k < K) ? (short)A[m * sam + k * sak] : (short)0;
+        for (int q = 0; q < 4; ++q)
+          a[i][q] = (m < M && k + q < K) ? (short)A[(long)m * sam + (long)(k + q) * sak]
+                                         : (short)0;
       }
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int n = n0 + 16 * j + r;
+      if (BV && n < N && k + 4 <= K) {
+        b[j] = *reinterpret_cast<const s4v *>(B + (long)n * sbn + k);
+      } else {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int k = k0 + kq + q;
-        b[j][q] = (n < N && k < K) ? (short)B[k * sbk + n * sbn] : (short)0;
+        for (int q = 0; q < 4; ++q)
+          b[j][q] = (n < N && k + q < K) ? (short)B[(long)(k + q) * sbk + (long)n * sbn]
+                                         : (short)0;
       }
     }
 #pragma unroll
@@ -100,26 +113,31 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int m = m0 + 16 * i + kq + q, n = n0 + 16 * j + r;
-        if (m >= M || n >= N) continue;
-        float v = acc[i][j][q];
-        if (epi == EPI_HIDDEN) {
-          v += bias[n];
-          Cf[(long)m * ldc + n] = v;
-          Hb[(long)m * ldh + n] = f2bf(v > 0.f ? v : 0.f);
-        } else if (epi == EPI_OUT) {
-          Cf[(long)m * ldc + n] = v + bias[n];
-        } else if (epi == EPI_ADD) {
-          Cf[(long)m * ldc + n] += v;
-        } else if (epi == EPI_DRELU) {
-          float g = Zin[(long)m * ldz + n] > 0.f ? v : 0.f;
-          Cf[(long)m * ldc + n] = g;
-          Hb[(long)m * ldh + n] = f2bf(g);
-        } else {
-          Cf[(long)m * ldc + n] = v;
-        }
-      }
+      for (int q = 0; q < 4; ++q)
+        red[wave][(16 * i + kq + q) * 33 + 16 * j + r] = acc[i][j][q];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int lm = e >> 5, ln = e & 31;
+    const int m = m0 + lm, n = n0 + ln;
+    if (m >= M || n >= N) continue;
+    const int o = lm * 33 + ln;
+    const float v = ((red[0][o] + red[1][o]) + red[2][o]) + red[3][o];
+    if (epi == EPI_HIDDEN) {
+      const float z = v + bias[n];
+      Cf[(long)m * ldc + n] = z;
+      Hb[(long)m * ldh + n] = f2bf(z > 0.f ? z : 0.f);
+    } else if (epi == EPI_OUT) {
+      Cf[(long)m * ldc + n] = v + bias[n];
+    } else if (epi == EPI_ADD) {
+      Cf[(long)m * ldc + n] += v;
+    } else if (epi == EPI_DRELU) {
+      const float g = Zin[(long)m * ldz + n] > 0.f ? v : 0.f;
+      Cf[(long)m * ldc + n] = g;
+      Hb[(long)m * ldh + n] = f2bf(g);
+    } else {
+      Cf[(long)m * ldc + n] = v;
+    }
+  }
 }
 
 __global__ void k_to_bf16(const float *__restrict__ x, long n,
@@ -356,10 +374,17 @@ struct MlpImpl {
             const uint16_t *B, long sbk, long sbn, int epi, const float *bias,
             float *Cf, long ldc, uint16_t *Hbo, long ldh, const float *Z,
             long ldz) {
-    long tiles = (long)((M + 31) / 32) * ((N + 31) / 32);
-    hipLaunchKernelGGL(k_gemm_bf16, dim3(ceil_div(tiles, 4)), dim3(256), 0, st,
-                       M, N, K, A, sam, sak, B, sbk, sbn, epi, bias, Cf, ldc,
-                       Hbo, ldh, Z, ldz);
+    const long tiles = (long)((M + 31) / 32) * ((N + 31) / 32);
+    const bool av = sak == 1 && sam % 4 == 0 && ((uintptr_t)A & 7) == 0;
+    const bool bv = sbk == 1 && sbn % 4 == 0 && ((uintptr_t)B & 7) == 0;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(tiles), dim3(256), 0, st, M, N, K, A, sam,
+                         sak, B, sbk, sbn, epi, bias, Cf, ldc, Hbo, ldh, Z, ldz);
+    };
+    if (av && bv) go(k_gemm_bf16<true, true>);
+    else if (av) go(k_gemm_bf16<true, false>);
+    else if (bv) go(k_gemm_bf16<false, true>);
+    else go(k_gemm_bf16<false, false>);
   }
   const float *mu() { return norm.as<float>(); }
   const float *sig() { return norm.as<float>() + nin; }
