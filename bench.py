@@ -487,6 +487,10 @@ def main():
     ap.add_argument("--refresh", type=int, default=100,
                     help="learned path: fine-tune on its own last REFRESH "
                          "steps every REFRESH steps (0: never)")
+    ap.add_argument("--ws-check", type=int, default=10,
+                    help="learned path: solve the extrapolated guess beside "
+                         "the network's every WS_CHECK steps (and when the "
+                         "network's guess loses); 1: every step")
     ap.add_argument("--c3-steps", type=int, default=2000,
                     help="steps of the mixed-vs-FP64 swimming run (--only c3)")
     ap.add_argument("--skip", type=int, default=10,
@@ -584,6 +588,7 @@ def main():
         keep.update(Xs=state["X"], Us=state["U"], qs=state["q"])
         cpu_state = keep
         api.system_set_warmstart(s_l, mlp)
+        api.system_set_warmstart_check(s_l, args.ws_check)
         info["train"] = {"samples": args.record * 1000, "skip": args.skip,
                          "record_s": t_rec,
                          "train_s": t_train, "epochs": args.epochs,
@@ -605,7 +610,7 @@ def main():
             else:
                 s, _ = build_system(prec)
                 s.set_state(**keep)
-            its, used, wits = [], 0, []
+            its, used, solved, wits = [], 0, 0, []
             nt = nt0
             for _ in range(args.warmup):
                 wits.append(s.timestep(nt))
@@ -620,7 +625,9 @@ def main():
             for _ in range(args.steps):
                 its.append(s.timestep(nt))
                 if name == "learned":
-                    used += api.system_last_step_info(s)[0]
+                    u, rp, _ = api.system_last_step_info(s)
+                    used += u
+                    solved += int(np.isfinite(rp))
                     if ref is not None:
                         ref.after_step()
                 nt += 1
@@ -644,10 +651,13 @@ def main():
             s.set_profiling(False)
             log("%s: %d steps in %.3f s -> %.3f steps/s, iters %s%s" %
                 (name, args.steps, el, args.steps / el, its,
-                 " (network guess used %d/%d)" % (used, args.steps)
+                 " (network guess used %d/%d, extrapolated guess solved "
+                 "on %d)" % (used, args.steps, solved)
                  if name == "learned" else ""))
             if name == "learned":
                 info["learned_used"] = used
+                info["learned_extrapolated_solves"] = solved
+                info["warmstart_check_every"] = args.ws_check
             s.close()
         ref = finals["fp64"]
         for name in ("learned", "mixed"):

@@ -548,6 +548,14 @@ def system_set_warmstart(system, mlp):
         system.h, mlp.h if mlp is not None else None))
 
 
+def system_set_warmstart_check(system, every):
+    """Solve the extrapolated guess beside the network's on every
+    `every`-th step only (1: every step); in between the network's guess is
+    kept while its residual beats the last measured extrapolation residual
+    (fs_system_set_warmstart_check)."""
+    abi.check(system.lib.fs_system_set_warmstart_check(system.h, every))
+
+
 def system_record(system, X, T, capacity):
     abi.check(system.lib.fs_system_ws_record(
         system.h, _ptr(X) if X is not None else None,

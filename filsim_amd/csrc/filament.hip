@@ -1910,16 +1910,24 @@ int SystemImpl::set_barrier(int on, int mask) {
 }
 
 // Initial guess, forces, Jacobian, link + barrier, mobility, residual E.
-int SystemImpl::begin_step(int nt) {
+int SystemImpl::begin_guess(int nt) {
   step_exploded = 0;
   step_nonfinite = 0;
   step_check = 1;
   FS_TRY(initial_guess());
   FS_TRY(forces(nt));
   FS_TRY(build_jacobian(nt));
-  FS_TRY(barrier(nt, 1, nullptr));
+  return barrier(nt, 1, nullptr);
+}
+
+int SystemImpl::eval_guess(int nt) {
   FS_TRY(mobility(nt));
   return residual(nt, d.V, d.W, E, &check0, &last_res0);
+}
+
+int SystemImpl::begin_step(int nt) {
+  FS_TRY(begin_guess(nt));
+  return eval_guess(nt);
 }
 
 int SystemImpl::broyden_loop(int nt, int *iters) {

@@ -289,6 +289,15 @@ extern "C" int fs_system_set_warmstart(fs_system *h, fs_mlp *m) {
   return h->ws.attach(h->impl, fs::mlp_impl(m));
 }
 
+extern "C" int fs_system_set_warmstart_check(fs_system *h, int every) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(every >= 1, FS_ERR_ARG, "check interval must be >= 1");
+  h->ws.check_every = every;
+  h->ws.since_check = 0;
+  h->ws.ref_plain = -1;
+  return FS_OK;
+}
+
 extern "C" int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
   FS_CHECK(h->batches == 0, FS_ERR_STATE,

@@ -73,7 +73,11 @@ struct SystemImpl {
   int set_barrier(int on, int mask);
   int timestep(int nt, int *iters);
   int timestep_trial(int nt, int *iters);
-  int begin_step(int nt);
+  int begin_step(int nt);  // begin_guess + eval_guess
+  // the step's extrapolated guess, forces, Jacobian and cell list
+  int begin_guess(int nt);
+  // mobility and residual of the current guess into E, check0, last_res0
+  int eval_guess(int nt);
   int broyden_loop(int nt, int *iters);
   int check0 = 0;
   // outcome of the last step: the final residual still above TOL, a

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "common.hpp"
 #include "warmstart.hpp"
@@ -183,6 +184,8 @@ int WarmStart::attach(SystemImpl &S, MlpImpl *m) {
            "MLP is %d -> %d, the system needs %d -> %d", a, b, nin, nout);
   mlp = m;
   tracking = true;
+  since_check = 0;
+  ref_plain = -1;  // the first step solves both guesses
   return FS_OK;
 }
 
@@ -221,8 +224,17 @@ int WarmStart::timestep(SystemImpl &S, int nt, int *iters) {
   hipStream_t st = S.st;
   const size_t n3 = 3 * (size_t)S.np, n4 = 4 * (size_t)S.np,
                l3 = 3 * (size_t)S.nlam, n6 = 6 * (size_t)S.np;
-  FS_TRY(S.begin_step(nt));
-  res_plain = S.last_res0;
+  // the extrapolated guess is solved on check steps (every one by default)
+  // and, on the others, only when the network's guess is rejected
+  const bool check = !mlp || check_every <= 1 || ref_plain < 0 ||
+                     since_check + 1 >= check_every;
+  if (check) {
+    FS_TRY(S.begin_step(nt));
+    res_plain = S.last_res0;
+  } else {
+    FS_TRY(S.begin_guess(nt));
+    res_plain = std::nan("");
+  }
   res_mlp = 0;
   used_mlp = 0;
   hipLaunchKernelGGL(k_ws_features, dim3(S.nf), dim3(64), 0, st, S.d, nt,
@@ -235,7 +247,7 @@ int WarmStart::timestep(SystemImpl &S, int nt, int *iters) {
   FS_HIP(hipMemcpyAsync(sqs.p, S.d.qs, sizeof(double) * n4, hipMemcpyDeviceToDevice, st));
   if (l3)
     FS_HIP(hipMemcpyAsync(slam.p, S.d.lam, sizeof(double) * l3, hipMemcpyDeviceToDevice, st));
-  if (mlp && S.check0) {
+  if (mlp && (!check || S.check0)) {
     FS_TRY(mlp_forward(mlp, S.nf, feat.as<float>(), pred.as<float>(), st));
     hipLaunchKernelGGL(k_ws_apply, dim3(S.nf), dim3(64), 0, st, S.d, nw,
                        pred.as<float>(), nout, u.as<double>());
@@ -246,7 +258,7 @@ int WarmStart::timestep(SystemImpl &S, int nt, int *iters) {
     FS_TRY(S.mobility(nt));
     int chk = 0;
     FS_TRY(S.residual(nt, S.d.V, S.d.W, S.NewE, &chk, &res_mlp));
-    if (res_mlp < res_plain) {
+    if (check ? res_mlp < res_plain : res_mlp < ref_plain) {
       std::swap(S.E, S.NewE);
       S.check0 = chk;
       S.last_res0 = res_mlp;
@@ -257,7 +269,22 @@ int WarmStart::timestep(SystemImpl &S, int nt, int *iters) {
       FS_HIP(hipMemcpyAsync(S.d.qs, sqs.p, sizeof(double) * n4, hipMemcpyDeviceToDevice, st));
       if (l3)
         FS_HIP(hipMemcpyAsync(S.d.lam, slam.p, sizeof(double) * l3, hipMemcpyDeviceToDevice, st));
+      if (!check) {
+        // not solved yet: the forces, barrier, mobility and residual of the
+        // restored guess, as begin_step computes them (same state, same
+        // cell list)
+        FS_TRY(S.forces(nt));
+        FS_TRY(S.barrier(nt, 0, nullptr));
+        FS_TRY(S.eval_guess(nt));
+        res_plain = S.last_res0;
+      }
     }
+  }
+  if (std::isnan(res_plain)) {
+    ++since_check;
+  } else {
+    ref_plain = res_plain;
+    since_check = 0;
   }
   FS_TRY(S.broyden_loop(nt, iters));
   if (recording && rec_count + S.nf <= rec_cap) {

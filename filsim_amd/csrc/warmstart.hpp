@@ -25,7 +25,12 @@ struct WarmStart {
   float *recX = nullptr, *recT = nullptr;
   int rec_cap = 0, rec_count = 0;
   int used_mlp = 0;
-  double res_plain = 0, res_mlp = 0;
+  double res_plain = 0, res_mlp = 0;  // res_plain NaN: not evaluated
+  // periodic check (check_every > 1): the extrapolated guess is solved on
+  // every check_every-th step and whenever the network's guess does not
+  // beat ref_plain, the last extrapolation residual measured
+  int check_every = 1, since_check = 0;
+  double ref_plain = -1;
 
   bool attached() const { return tracking; }
   int setup(SystemImpl &S);  // sizes + buffers (uniform segment count)

@@ -388,8 +388,19 @@ int fs_system_warmstart_dims(fs_system *h, int *n_in, int *n_out);
  * filament per step, until capacity rows are filled. NULL stops. */
 int fs_system_ws_record(fs_system *h, float *X, float *T, int capacity);
 int fs_system_ws_count(fs_system *h, int *count);
+/* How often the extrapolated guess is solved beside the network's (no
+ * reference counterpart). every = 1 (the default): on every step, and the
+ * guess with the smaller residual infinity norm starts the Broyden
+ * iteration. every = k > 1: on every k-th step (and the first after this
+ * call or after attaching a network); on the other steps only the network's
+ * guess is solved, and it is kept when its residual is below the last
+ * measured residual of the extrapolated guess, else the extrapolated guess
+ * is solved and used. Saves one mobility solve per step while the network
+ * keeps winning. */
+int fs_system_set_warmstart_check(fs_system *h, int every);
 /* Whether the last timestep started from the network's guess, and the
- * infinity norms of the residuals of both candidate guesses. */
+ * infinity norms of the residuals of both candidate guesses (res_plain is
+ * NaN when the extrapolated guess was not solved). */
 int fs_system_last_step_info(fs_system *h, int *used_mlp, double *res_plain,
                              double *res_mlp);
 /* Per-timestep profile in the reference profiler's buckets

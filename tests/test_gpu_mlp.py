@@ -154,3 +154,72 @@ def test_trained_warmstart_gain(cuda):
     assert hist[-1][2] <= mlp.ridge_holdout_net
     assert mlp.ridge_holdout < 0.5 * mlp.zero_holdout
     assert np.mean(il) <= np.mean(ib) - 2.0
+
+
+def test_check_mode_fallback_is_exact_plain_path(cuda):
+    """With the extrapolated guess solved only on every third step, a
+    network whose guess is far off is still rejected on every step (then the
+    extrapolated guess is solved after all), and the trajectory is
+    bit-identical to the plain path."""
+    sp, nws, props, st = _swimmers()
+    plain = api.System(sp, nws, props)
+    plain.set_state(**st)
+    ws = api.System(sp, nws, props)
+    ws.set_state(**st)
+    n_in, n_out = api.system_warmstart_dims(ws)
+    bad = api.Mlp(n_in, n_out, width=64, seed=9)
+    bad.set_norm(np.zeros(n_in), np.ones(n_in), 1e3 * np.ones(n_out))
+    api.system_set_warmstart(ws, bad)
+    api.system_set_warmstart_check(ws, 3)
+    for nt in range(1, 6):
+        a = plain.timestep(nt)
+        b = ws.timestep(nt)
+        used, rp, rm = api.system_last_step_info(ws)
+        assert a == b and used == 0 and np.isfinite(rp) and rm > rp
+    sa, sb = plain.get_state(), ws.get_state()
+    assert np.array_equal(sa["X"], sb["X"]) and np.array_equal(sa["q"], sb["q"])
+
+
+def test_check_mode_matches_every_step_mode(cuda):
+    """Two recorders with the same trajectory and history continue with the
+    same trained network, one solving the extrapolated guess every step, the
+    other only every 4th step (or when the network's guess does not beat the
+    last measured extrapolation residual). Skipping that solve changes
+    nothing else: while the two runs have made the same choices their
+    network residuals, iteration counts and states agree bit for bit, and
+    the check mode solves the extrapolated guess on fewer steps."""
+    torch = cuda
+    sp, nws, props, st = _swimmers()
+    runs = []
+    for _ in range(2):
+        s = api.System(sp, nws, props)
+        s.set_state(**st)
+        X, T, _ = warmstart.record(s, 60, torch=torch, skip=5)
+        runs.append(s)
+    mlp, _ = warmstart.train(X, T, torch, epochs=40, batch=128, lr=1e-3,
+                             linear=True)
+    a, b = runs
+    api.system_set_warmstart(a, mlp)
+    api.system_set_warmstart(b, mlp)
+    api.system_set_warmstart_check(b, 4)
+    same, agreed, solved, used_b = True, 0, 0, 0
+    for nt in range(66, 96):
+        ia = a.timestep(nt)
+        ua, pa, ma = api.system_last_step_info(a)
+        ib = b.timestep(nt)
+        ub, pb, mb = api.system_last_step_info(b)
+        assert np.isfinite(pa)
+        solved += int(np.isfinite(pb))
+        used_b += ub
+        if same:
+            assert ma == mb  # same state: same network guess and residual
+            if ua != ub:
+                same = False
+                continue
+            assert ia == ib
+            assert np.array_equal(a.get_state()["X"], b.get_state()["X"])
+            agreed += 1
+    print("steps agreeing", agreed, "extrapolated solves", solved,
+          "network used", used_b)
+    assert agreed >= 5
+    assert solved <= 20 and used_b >= 15
