@@ -877,6 +877,19 @@ __global__ void __launch_bounds__(1024)
 constexpr int kBlk = 12;
 constexpr int kBlkSz = kBlk * kBlk;
 constexpr int kBlkRec = 3 * kBlkSz;  // D^-1, G, A_K,K-1 per block row
+// a's value in lane l ^ X (X = 1 or 2, inside each group of four lanes)
+// through a quad permutation of the data-parallel primitives: a few cycles,
+// against a round trip through the LDS crossbar for a general shuffle
+template <int X>
+__device__ __forceinline__ double quad_xor(double a) {
+  constexpr int ctrl = X == 1 ? (1 | 0 << 2 | 3 << 4 | 2 << 6)   // [1,0,3,2]
+                              : (2 | 3 << 2 | 0 << 4 | 1 << 6);  // [2,3,0,1]
+  static_assert(X == 1 || X == 2, "quad lanes");
+  const long long v = __double_as_longlong(a);
+  const int lo = __builtin_amdgcn_mov_dpp((int)v, ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), ctrl, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 constexpr int kBandMaxNwBlk = 32;  // block rows of the longest filament (Nw <= 64)
 
 // Original unknown (column of J) of permuted unknown j, and original
@@ -1056,8 +1069,8 @@ __global__ void __launch_bounds__(64)
     double a = 0.0;
 #pragma unroll
     for (int m = 0; m < 3; ++m) a += o[m] * v[3 * part + m];
-    a += __shfl_xor(a, 1, 64);
-    a += __shfl_xor(a, 2, 64);
+    a += quad_xor<1>(a);
+    a += quad_xor<2>(a);
     return a;
   };
   // forward: y_K = D_K^-1 (b_K - A_K,K-1 y_K-1)
