@@ -1696,16 +1696,22 @@ int SystemImpl::build_jacobian_dev(int nt) {
   return FS_OK;
 }
 
-int SystemImpl::build_jacobian(int nt) {
+int SystemImpl::build_jacobian(int nt, bool clear_history) {
   ProfScope ps_(*this, P_BLDJ);
   FS_TRY(build_jacobian_dev(nt));
-  // history reset (FilamentJacobianSolver.hpp:40-41)
-  size_t nh = 6 * (size_t)np * p.broyden_maxiter;
-  hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nh + 255) / 256)),
-                     dim3(256), 0, st, Cm.as<double>(), nh, 0.0);
-  hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nh + 255) / 256)),
-                     dim3(256), 0, st, Dm.as<double>(), nh, 0.0);
-  FS_LAUNCHED();
+  // history reset (FilamentJacobianSolver.hpp:40-41). The history kernels
+  // read only columns below the count they are given, and a timestep's
+  // Broyden iteration k reads columns 0 .. k-2, written by its own earlier
+  // iterations: the timestep needs no clearing (neither does the batched
+  // sweep's device-controlled step)
+  if (clear_history) {
+    size_t nh = 6 * (size_t)np * p.broyden_maxiter;
+    hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nh + 255) / 256)),
+                       dim3(256), 0, st, Cm.as<double>(), nh, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(std::min<size_t>(4096, (nh + 255) / 256)),
+                       dim3(256), 0, st, Dm.as<double>(), nh, 0.0);
+    FS_LAUNCHED();
+  }
   int *sing = flags.as<int>() + 4;
   FS_HIP(hipMemcpyAsync((int *)hflags + 4, sing, sizeof(int),
                         hipMemcpyDeviceToHost, st));
@@ -1929,7 +1935,7 @@ int SystemImpl::begin_guess(int nt) {
   step_check = 1;
   FS_TRY(initial_guess());
   FS_TRY(forces(nt));
-  FS_TRY(build_jacobian(nt));
+  FS_TRY(build_jacobian(nt, false));
   return barrier(nt, 1, nullptr);
 }
 

@@ -57,7 +57,9 @@ struct SystemImpl {
   int initial_guess();
   int forces(int nt);
   int update(const double *u);
-  int build_jacobian(int nt);
+  // clear_history: zero the Broyden history (the API's reset); the
+  // timestep reads only the columns its own iterations wrote and skips it
+  int build_jacobian(int nt, bool clear_history = true);
   int jsolve(const double *b, double *x, int iter);
   int jinv_dense(int f, double *out);
   int broyden_update(const double *Eo, const double *En, const double *upd,
