@@ -179,11 +179,25 @@ __global__ void __launch_bounds__(256)
     const int ifil = sfil[i], iseg = sseg[i];
     const int ci = cell[i];
     const int cx = ci % g.MX, cy = (ci / g.MX) % g.MY, cz = ci / (g.MX * g.MY);
-    for (int k = q; k < 27; k += kCbLanes) {
-      const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
-      int nc = cb_wrap(cx + dx, g.MX) +
-               g.MX * (cb_wrap(cy + dy, g.MY) + g.MY * cb_wrap(cz + dz, g.MZ));
-      for (int e = start[nc]; e < start[nc + 1]; ++e) {
+    // the lane's cell ranges first, all loads in flight together (most
+    // cells are empty), then the pairs in the same cell order
+    constexpr int kCells = (27 + kCbLanes - 1) / kCbLanes;
+    int lo[kCells], hi[kCells];
+#pragma unroll
+    for (int t = 0; t < kCells; ++t) {
+      const int k = q + kCbLanes * t;
+      lo[t] = hi[t] = 0;
+      if (k < 27) {
+        const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+        const int nc = cb_wrap(cx + dx, g.MX) +
+                       g.MX * (cb_wrap(cy + dy, g.MY) + g.MY * cb_wrap(cz + dz, g.MZ));
+        lo[t] = start[nc];
+        hi[t] = start[nc + 1];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kCells; ++t) {
+      for (int e = lo[t]; e < hi[t]; ++e) {
         int j = ids[e];
         if (j == i) continue;
         const int jfil = sfil[j];
