@@ -491,6 +491,10 @@ def main():
                     help="learned path: solve the extrapolated guess beside "
                          "the network's every WS_CHECK steps (and when the "
                          "network's guess loses); 1: every step")
+    ap.add_argument("--hist-carry", type=int, default=96,
+                    help="learned path: carry the Broyden history across "
+                         "steps while it holds at most this many columns "
+                         "(0: restart every step)")
     ap.add_argument("--c3-steps", type=int, default=2000,
                     help="steps of the mixed-vs-FP64 swimming run (--only c3)")
     ap.add_argument("--skip", type=int, default=10,
@@ -589,6 +593,7 @@ def main():
         cpu_state = keep
         api.system_set_warmstart(s_l, mlp)
         api.system_set_warmstart_check(s_l, args.ws_check)
+        api.system_set_history_carry(s_l, args.hist_carry)
         info["train"] = {"samples": args.record * 1000, "skip": args.skip,
                          "record_s": t_rec,
                          "train_s": t_train, "epochs": args.epochs,
@@ -658,6 +663,7 @@ def main():
                 info["learned_used"] = used
                 info["learned_extrapolated_solves"] = solved
                 info["warmstart_check_every"] = args.ws_check
+                info["broyden_history_carry_cols"] = args.hist_carry
             s.close()
         ref = finals["fp64"]
         for name in ("learned", "mixed"):

@@ -141,6 +141,7 @@ PROTOS = {
     "fs_mlp_set_params": [P, FP, I],
     "fs_system_set_warmstart": [P, P],
     "fs_system_set_warmstart_check": [P, C.c_int],
+    "fs_system_set_history_carry": [P, C.c_int],
     "fs_system_warmstart_dims": [P, IP, IP],
     "fs_system_ws_record": [P, P, P, I],
     "fs_system_ws_count": [P, IP],

@@ -556,6 +556,13 @@ def system_set_warmstart_check(system, every):
     abi.check(system.lib.fs_system_set_warmstart_check(system.h, every))
 
 
+def system_set_history_carry(system, max_cols):
+    """Carry the Broyden history across timesteps while it holds at most
+    max_cols columns (0: restart every step, the reference's behaviour;
+    fs_system_set_history_carry)."""
+    abi.check(system.lib.fs_system_set_history_carry(system.h, max_cols))
+
+
 def system_record(system, X, T, capacity):
     abi.check(system.lib.fs_system_ws_record(
         system.h, _ptr(X) if X is not None else None,

@@ -1458,13 +1458,17 @@ int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
     FS_TRY(JTb.alloc(sizeof(double) * (size_t)hjoff[nf]));
     FS_TRY(Mb.alloc(sizeof(double) * (size_t)hjoff[nf]));
   }
-  FS_TRY(Cm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
-  FS_TRY(Dm.alloc(sizeof(double) * n6 * p.broyden_maxiter));
+  // room for the carried history beyond the step's own columns
+  FS_TRY(Cm.alloc(sizeof(double) * n6 * (p.broyden_maxiter + kHistCarryMax)));
+  FS_TRY(Dm.alloc(sizeof(double) * n6 * (p.broyden_maxiter + kHistCarryMax)));
   dot_chunk = 2048;
   dot_blocks = ceil_div((long)n6, dot_chunk);
-  FS_TRY(partial.alloc(sizeof(double) * (size_t)dot_blocks *
-                       std::max(2, p.broyden_maxiter)));
-  FS_TRY(coefb.alloc(sizeof(double) * (p.broyden_maxiter + 2)));
+  // history column capacity (the step's own columns and the carried ones):
+  // the dot partials and coefficients are sized for it, the 2 Broyden dot
+  // scalars follow the coefficients
+  hcols = p.broyden_maxiter + kHistCarryMax;
+  FS_TRY(partial.alloc(sizeof(double) * (size_t)dot_blocks * std::max(2, hcols)));
+  FS_TRY(coefb.alloc(sizeof(double) * (hcols + 2)));
   FS_TRY(flags.alloc(64));
   FS_TRY(ctl.alloc(4 * sizeof(int)));
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(4), 0, st, ctl.as<int>(), 4, 0);
@@ -1764,8 +1768,7 @@ int SystemImpl::jinv_dense(int f, double *out) {
 }
 
 int SystemImpl::jsolve(const double *b, double *x, int iter) {
-  FS_CHECK(iter >= 0 && iter <= p.broyden_maxiter, FS_ERR_ARG, "bad iter %d",
-           iter);
+  FS_CHECK(iter >= 0 && iter < hcols, FS_ERR_ARG, "bad iter %d", iter);
   FS_CHECK(b != x, FS_ERR_ARG, "jsolve needs distinct b and x");
   return jsolve_impl(b, x, iter, nullptr, 0);
 }
@@ -1813,7 +1816,7 @@ int SystemImpl::broyden_update(const double *Eo, const double *En,
   hipLaunchKernelGGL(k_dots_partial, dim3(dot_blocks, 1), dim3(256), 0, st,
                      (const int *)nullptr, dE, n6, 1, (const int *)nullptr, 0,
                      1, dE, n6, dot_chunk, partial.as<double>());
-  double *nrm2 = coefb.as<double>() + p.broyden_maxiter;
+  double *nrm2 = coefb.as<double>() + hcols;
   hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st,
                      (const int *)nullptr, partial.as<double>(), dot_blocks, 1,
                      1, (const int *)nullptr, 0, nrm2);
@@ -1954,7 +1957,9 @@ int SystemImpl::broyden_loop(int nt, int *iters) {
   int check = check0;
   int iter = 0;
   last_res = last_res0;
-  if (check) FS_TRY(jsolve(E, Upd, 0));
+  // history columns carried in from the last step (0 without carrying)
+  const int h0 = hist_cap > 0 ? hist_cols : 0;
+  if (check) FS_TRY(jsolve(E, Upd, h0));
   while (check && iter < p.broyden_maxiter) {
     FS_TRY(update(Upd));
     FS_TRY(forces(nt));
@@ -1964,7 +1969,7 @@ int SystemImpl::broyden_loop(int nt, int *iters) {
     ++iter;
     if (!check || iter >= p.broyden_maxiter) break;
     // dE = NewE - E; |dE|^2 and dE.NewE; g = H_{iter-1} NewE
-    double *dots = coefb.as<double>() + p.broyden_maxiter;  // 2 scalars
+    double *dots = coefb.as<double>() + hcols;  // 2 scalars
     {
       ProfScope ps_(*this, P_SLVJ);
       hipLaunchKernelGGL(k_diff_dots, dim3(dot_blocks), dim3(256), 0, st,
@@ -1975,16 +1980,35 @@ int SystemImpl::broyden_loop(int nt, int *iters) {
                          2, 2, (const int *)nullptr, 0, dots);
       FS_LAUNCHED();
     }
-    FS_TRY(jsolve(NewE, tmp, iter - 1));
+    FS_TRY(jsolve(NewE, tmp, h0 + iter - 1));
     {
       ProfScope ps_(*this, P_SLVJ);
       hipLaunchKernelGGL(k_broyden_fused, dim3(ceil_div((long)n6, 256)),
                          dim3(256), 0, st, (const int *)nullptr, tmp, dE, dots,
-                         n6, Cm.as<double>(), Dm.as<double>(), iter - 1,
+                         n6, Cm.as<double>(), Dm.as<double>(), h0 + iter - 1,
                          (const int *)nullptr, Upd);
       FS_LAUNCHED();
     }
     std::swap(E, NewE);
+  }
+  if (hist_cap > 0) {
+    // columns now held: the carried ones and one per iteration that did
+    // not end the loop. Beyond hist_cap the newest hist_cap / 2 move to the
+    // front and the older ones are dropped
+    const int held = h0 + std::max(0, iter - 1);
+    if (held <= hist_cap) {
+      hist_cols = held;
+    } else {
+      const int keep = hist_cap / 2;  // held > 2 keep: the ranges are disjoint
+      const size_t cb = sizeof(double) * n6 * keep;
+      if (keep > 0) {
+        FS_HIP(hipMemcpyAsync(Cm.as<double>(), Cm.as<double>() + n6 * (held - keep), cb,
+                              hipMemcpyDeviceToDevice, st));
+        FS_HIP(hipMemcpyAsync(Dm.as<double>(), Dm.as<double>() + n6 * (held - keep), cb,
+                              hipMemcpyDeviceToDevice, st));
+      }
+      hist_cols = keep;
+    }
   }
   step_check = check;
   if (iters) *iters = iter;
@@ -2063,7 +2087,7 @@ int SystemImpl::iteration_dev(int *status, int part) {
                      nt, d.V, d.W, NewE, mb);
   hipLaunchKernelGGL(k_ctl_after, dim3(1), dim3(1), 0, st, ctl.as<int>(), mb,
                      p.tol, p.broyden_maxiter);
-  double *dots = coefb.as<double>() + p.broyden_maxiter;  // 2 scalars
+  double *dots = coefb.as<double>() + hcols;  // 2 scalars
   hipLaunchKernelGGL(k_diff_dots, dim3(dot_blocks), dim3(256), 0, st, act, NewE,
                      E, dE, n6, dot_chunk, partial.as<double>());
   hipLaunchKernelGGL(k_dots_final, dim3(2), dim3(64), 0, st, act,

@@ -83,6 +83,7 @@ extern "C" int fs_system_set_state(fs_system *h, const fs_state_ptrs *s) {
   FS_TRY(copy_field(d.lam2, s->lam2, l3, K));
   FS_TRY(copy_field(d.F, s->F, n3, K));
   FS_TRY(copy_field(d.TAU, s->TAU, n3, K));
+  h->impl.hist_cols = 0;  // a new state: no history carried in
   return FS_OK;
 }
 
@@ -295,6 +296,16 @@ extern "C" int fs_system_set_warmstart_check(fs_system *h, int every) {
   h->ws.check_every = every;
   h->ws.since_check = 0;
   h->ws.ref_plain = -1;
+  return FS_OK;
+}
+
+extern "C" int fs_system_set_history_carry(fs_system *h, int max_cols) {
+  FS_CHECK(h, FS_ERR_ARG, "null handle");
+  FS_CHECK(max_cols >= 0 && max_cols <= fs::kHistCarryMax, FS_ERR_ARG,
+           "history carry must be in [0, %d]", fs::kHistCarryMax);
+  FS_HIP(hipStreamSynchronize(h->impl.st));
+  h->impl.hist_cap = max_cols;
+  h->impl.hist_cols = 0;
   return FS_OK;
 }
 

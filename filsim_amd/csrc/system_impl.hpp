@@ -9,6 +9,8 @@
 
 namespace fs {
 
+constexpr int kHistCarryMax = 128;  // Broyden columns a timestep may carry in
+
 // Plain struct of device pointers and scalars passed by value to kernels.
 struct SysDev {
   double *X, *Xs, *Xt, *U, *Us, *Ut, *q, *qs, *qt, *F, *TAU;
@@ -46,6 +48,11 @@ struct SystemImpl {
   int dot_chunk = 0, dot_blocks = 0;
   int max_nw = 0;
   double last_res0 = 0, last_res = 0;
+  // Broyden history carried into the next timestep (no reference
+  // counterpart; the reference restarts it each step): up to hist_cap
+  // columns, hist_cols of them carried into the current step; 0 = restart
+  int hist_cap = 0, hist_cols = 0;
+  int hcols = 0;  // history column capacity: broyden_maxiter + kHistCarryMax
   FcmImpl fcm;
   ColbarImpl cb;
   hipStream_t st = nullptr;

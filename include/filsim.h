@@ -398,6 +398,15 @@ int fs_system_ws_count(fs_system *h, int *count);
  * is solved and used. Saves one mobility solve per step while the network
  * keeps winning. */
 int fs_system_set_warmstart_check(fs_system *h, int every);
+/* Broyden history carried across timesteps (no reference counterpart; the
+ * reference restarts the history with each step's new Jacobian,
+ * FilamentJacobianSolver.hpp:40-41). max_cols = 0 (the default): restart
+ * every step. max_cols > 0 (at most 128): a step starts from the new
+ * approximate inverse plus the rank-one corrections the previous steps
+ * accumulated; when they exceed max_cols columns the newest max_cols / 2
+ * are kept. Every step still ends at the same residual tolerance. Setting
+ * the state clears the carried history. */
+int fs_system_set_history_carry(fs_system *h, int max_cols);
 /* Whether the last timestep started from the network's guess, and the
  * infinity norms of the residuals of both candidate guesses (res_plain is
  * NaN when the extrapolated guess was not solved). */
