@@ -877,18 +877,29 @@ __global__ void __launch_bounds__(1024)
 constexpr int kBlk = 12;
 constexpr int kBlkSz = kBlk * kBlk;
 constexpr int kBlkRec = 3 * kBlkSz;  // D^-1, G, A_K,K-1 per block row
+// Lane permutations of the data-parallel primitives (DPP): a few cycles,
+// against a round trip through the LDS crossbar for a general shuffle.
+// Quad permutations [1,0,3,2] and [2,3,0,1] (lane l ^ 1, l ^ 2), and the
+// mirrors inside each group of 8 and of 16 lanes.
+constexpr int kDppXor1 = 1 | 0 << 2 | 3 << 4 | 2 << 6;
+constexpr int kDppXor2 = 2 | 3 << 2 | 0 << 4 | 1 << 6;
+constexpr int kDppMirror = 0x140, kDppHalfMirror = 0x141;
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double a) {
+  const long long v = __double_as_longlong(a);
+  const int lo = dpp_i32<CTRL>((int)v);
+  const int hi = dpp_i32<CTRL>((int)(v >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 // a's value in lane l ^ X (X = 1 or 2, inside each group of four lanes)
-// through a quad permutation of the data-parallel primitives: a few cycles,
-// against a round trip through the LDS crossbar for a general shuffle
 template <int X>
 __device__ __forceinline__ double quad_xor(double a) {
-  constexpr int ctrl = X == 1 ? (1 | 0 << 2 | 3 << 4 | 2 << 6)   // [1,0,3,2]
-                              : (2 | 3 << 2 | 0 << 4 | 1 << 6);  // [2,3,0,1]
   static_assert(X == 1 || X == 2, "quad lanes");
-  const long long v = __double_as_longlong(a);
-  const int lo = __builtin_amdgcn_mov_dpp((int)v, ctrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), ctrl, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  return dpp_f64<X == 1 ? kDppXor1 : kDppXor2>(a);
 }
 constexpr int kBandMaxNwBlk = 32;  // block rows of the longest filament (Nw <= 64)
 
@@ -920,10 +931,13 @@ __device__ __forceinline__ size_t blk_offset(const SysDev &S, int f) {
   return ((size_t)S.boff[f] + 2 * (size_t)f) * (kBlkRec / 2);
 }
 
-// One workgroup of 256 threads per filament: block rows in order, the
-// 12 x 12 operations spread over the threads (padding rows past n carry an
-// identity diagonal).
-__global__ void __launch_bounds__(256)
+// One wavefront per filament (a one-wave workgroup: its barriers are
+// cheap, and the pivot search needs no second wave): block rows in order,
+// the 12 x 12 operations spread over the lanes (padding rows past n carry
+// an identity diagonal).
+constexpr int kFacT = 64;
+constexpr int kFacUpd = (kBlk * 2 * kBlk + kFacT - 1) / kFacT;  // updates per lane
+__global__ void __launch_bounds__(kFacT)
     k_blk_factor(SysDev S, const double *__restrict__ Jall,
                  double *__restrict__ Fall, int *__restrict__ singular) {
   if (S.act && !*S.act) return;
@@ -936,11 +950,11 @@ __global__ void __launch_bounds__(256)
   double *F = Fall + blk_offset(S, f);
   const int tid = threadIdx.x;
   bool bad = false;
-  for (int e = tid; e < kBlkSz; e += 256) sG[e] = 0.0;
+  for (int e = tid; e < kBlkSz; e += kFacT) sG[e] = 0.0;
   for (int K = 0; K < nb; ++K) {
     const int r0 = K * kBlk;
     // A_K,K-1 and A_K,K+1 of this block row; D = A_KK (identity padding)
-    for (int e = tid; e < kBlkSz; e += 256) {
+    for (int e = tid; e < kBlkSz; e += kFacT) {
       const int i = e / kBlk, j = e - i * kBlk;
       sL[e] = band_entry(J, n, nw, r0 + i, r0 - kBlk + j);
       sU[e] = band_entry(J, n, nw, r0 + i, r0 + kBlk + j);
@@ -952,7 +966,7 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     // D -= A_K,K-1 G_K-1
     if (K > 0)
-      for (int e = tid; e < kBlkSz; e += 256) {
+      for (int e = tid; e < kBlkSz; e += kFacT) {
         const int i = e / kBlk, j = e - i * kBlk;
         double acc = 0.0;
 #pragma unroll
@@ -970,15 +984,21 @@ __global__ void __launch_bounds__(256)
           a = x == x ? x : -0.5;
           t = tid;
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const double oa = __shfl_xor(a, off, 64);
-          const int ot = __shfl_xor(t, off, 64);
+        // the candidates are lanes c..11: a max (ties to the lower row)
+        // over each group of 16 lanes through four data-parallel lane
+        // permutations (pairs 1 and 2 apart, then mirrored within 8 and 16
+        // lanes); the combination is associative and commutative, so any
+        // pairing gives the same winner
+        auto take = [&](double oa, int ot) {
           if (oa > a || (oa == a && ot < t)) {
             a = oa;
             t = ot;
           }
-        }
+        };
+        take(dpp_f64<kDppXor1>(a), dpp_i32<kDppXor1>(t));
+        take(dpp_f64<kDppXor2>(a), dpp_i32<kDppXor2>(t));
+        take(dpp_f64<kDppHalfMirror>(a), dpp_i32<kDppHalfMirror>(t));
+        take(dpp_f64<kDppMirror>(a), dpp_i32<kDppMirror>(t));
         if (tid == 0) {
           bad = bad || !(a > 0.0) || !(a <= DBL_MAX);
           s_piv = t < kBlk ? t : c;
@@ -993,20 +1013,20 @@ __global__ void __launch_bounds__(256)
       }
       __syncthreads();
       const double inv = 1.0 / sA[c][c];
-      // eliminate column c from the other rows (12 x 24 entries, two per
-      // thread at most), then scale row c
-      double upd[2];
+      // eliminate column c from the other rows (12 x 24 entries, five per
+      // lane at most), then scale row c
+      double upd[kFacUpd];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int e = tid + 256 * h, ui = e / (2 * kBlk), uj = e - ui * 2 * kBlk;
+      for (int h = 0; h < kFacUpd; ++h) {
+        const int e = tid + kFacT * h, ui = e / (2 * kBlk), uj = e - ui * 2 * kBlk;
         upd[h] = (e < kBlk * 2 * kBlk && ui != c)
                      ? sA[ui][uj] - sA[ui][c] * inv * sA[c][uj]
                      : 0.0;
       }
       __syncthreads();
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int e = tid + 256 * h, ui = e / (2 * kBlk), uj = e - ui * 2 * kBlk;
+      for (int h = 0; h < kFacUpd; ++h) {
+        const int e = tid + kFacT * h, ui = e / (2 * kBlk), uj = e - ui * 2 * kBlk;
         if (e < kBlk * 2 * kBlk && ui != c) sA[ui][uj] = upd[h];
       }
       __syncthreads();
@@ -1015,7 +1035,7 @@ __global__ void __launch_bounds__(256)
     }
     // G_K = D^-1 A_K,K+1; store D^-1, G_K, A_K,K-1
     double *FK = F + (size_t)K * kBlkRec;
-    for (int e = tid; e < kBlkSz; e += 256) {
+    for (int e = tid; e < kBlkSz; e += kFacT) {
       const int i = e / kBlk, j = e - i * kBlk;
       double acc = 0.0;
 #pragma unroll
@@ -1688,7 +1708,7 @@ int SystemImpl::build_jacobian_dev(int nt) {
   int *sing = flags.as<int>() + 4;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, sing, 1, 0);
   if (banded()) {  // block-tridiagonal factorisation, one workgroup each
-    hipLaunchKernelGGL(k_blk_factor, dim3(nf), dim3(256), 0, st, d,
+    hipLaunchKernelGGL(k_blk_factor, dim3(nf), dim3(kFacT), 0, st, d,
                        Jb.as<double>(), Fb.as<double>(), sing);
   } else {  // longer filaments: dense Gauss-Jordan in global memory (L2)
     FS_HIP(hipMemcpyAsync(Mb.p, Jb.p, sizeof(double) * nJ,
