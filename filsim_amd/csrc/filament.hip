@@ -1478,17 +1478,9 @@ int SystemImpl::init(const fs_sim_params &pp, int nfil, const int *nwv,
     FS_TRY(JTb.alloc(sizeof(double) * (size_t)hjoff[nf]));
     FS_TRY(Mb.alloc(sizeof(double) * (size_t)hjoff[nf]));
   }
-  // room for the carried history beyond the step's own columns
-  FS_TRY(Cm.alloc(sizeof(double) * n6 * (p.broyden_maxiter + kHistCarryMax)));
-  FS_TRY(Dm.alloc(sizeof(double) * n6 * (p.broyden_maxiter + kHistCarryMax)));
   dot_chunk = 2048;
   dot_blocks = ceil_div((long)n6, dot_chunk);
-  // history column capacity (the step's own columns and the carried ones):
-  // the dot partials and coefficients are sized for it, the 2 Broyden dot
-  // scalars follow the coefficients
-  hcols = p.broyden_maxiter + kHistCarryMax;
-  FS_TRY(partial.alloc(sizeof(double) * (size_t)dot_blocks * std::max(2, hcols)));
-  FS_TRY(coefb.alloc(sizeof(double) * (hcols + 2)));
+  FS_TRY(alloc_history(p.broyden_maxiter));
   FS_TRY(flags.alloc(64));
   FS_TRY(ctl.alloc(4 * sizeof(int)));
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(4), 0, st, ctl.as<int>(), 4, 0);
@@ -1717,6 +1709,34 @@ int SystemImpl::build_jacobian_dev(int nt) {
                        Mb.as<double>(), Jinv_cm(), sing);
   }
   FS_LAUNCHED();
+  return FS_OK;
+}
+
+// Broyden history of `cols` columns (the step's own, plus carried ones):
+// the dot partials and coefficients are sized for it, the 2 Broyden dot
+// scalars follow the coefficients
+int SystemImpl::alloc_history(int cols) {
+  const size_t n6 = 6 * (size_t)np;
+  Cm.release();
+  Dm.release();
+  partial.release();
+  coefb.release();
+  hcols = 0;
+  FS_TRY(Cm.alloc(sizeof(double) * n6 * cols));
+  FS_TRY(Dm.alloc(sizeof(double) * n6 * cols));
+  FS_TRY(partial.alloc(sizeof(double) * (size_t)dot_blocks * std::max(2, cols)));
+  FS_TRY(coefb.alloc(sizeof(double) * (cols + 2)));
+  hcols = cols;
+  return FS_OK;
+}
+
+int SystemImpl::set_history_carry(int cap) {
+  FS_CHECK(cap >= 0 && cap <= kHistCarryMax, FS_ERR_ARG,
+           "history carry must be in [0, %d]", kHistCarryMax);
+  FS_HIP(hipStreamSynchronize(st));
+  if (p.broyden_maxiter + cap > hcols) FS_TRY(alloc_history(p.broyden_maxiter + cap));
+  hist_cap = cap;
+  hist_cols = 0;
   return FS_OK;
 }
 

@@ -301,12 +301,9 @@ extern "C" int fs_system_set_warmstart_check(fs_system *h, int every) {
 
 extern "C" int fs_system_set_history_carry(fs_system *h, int max_cols) {
   FS_CHECK(h, FS_ERR_ARG, "null handle");
-  FS_CHECK(max_cols >= 0 && max_cols <= fs::kHistCarryMax, FS_ERR_ARG,
-           "history carry must be in [0, %d]", fs::kHistCarryMax);
-  FS_HIP(hipStreamSynchronize(h->impl.st));
-  h->impl.hist_cap = max_cols;
-  h->impl.hist_cols = 0;
-  return FS_OK;
+  FS_CHECK(h->batches == 0, FS_ERR_STATE,
+           "the system belongs to a batch (its launches are captured)");
+  return h->impl.set_history_carry(max_cols);
 }
 
 extern "C" int fs_system_autotune(fs_system *h, int reps, fs_fcm_tuning *best) {

@@ -9,7 +9,7 @@
 
 namespace fs {
 
-constexpr int kHistCarryMax = 128;  // Broyden columns a timestep may carry in
+constexpr int kHistCarryMax = 192;  // Broyden columns a timestep may carry in
 
 // Plain struct of device pointers and scalars passed by value to kernels.
 struct SysDev {
@@ -52,7 +52,9 @@ struct SystemImpl {
   // counterpart; the reference restarts it each step): up to hist_cap
   // columns, hist_cols of them carried into the current step; 0 = restart
   int hist_cap = 0, hist_cols = 0;
-  int hcols = 0;  // history column capacity: broyden_maxiter + kHistCarryMax
+  int hcols = 0;  // history column capacity (broyden_maxiter + the carry cap)
+  int alloc_history(int cols);
+  int set_history_carry(int cap);
   FcmImpl fcm;
   ColbarImpl cb;
   hipStream_t st = nullptr;

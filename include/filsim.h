@@ -401,11 +401,13 @@ int fs_system_set_warmstart_check(fs_system *h, int every);
 /* Broyden history carried across timesteps (no reference counterpart; the
  * reference restarts the history with each step's new Jacobian,
  * FilamentJacobianSolver.hpp:40-41). max_cols = 0 (the default): restart
- * every step. max_cols > 0 (at most 128): a step starts from the new
+ * every step. max_cols > 0 (at most 192): a step starts from the new
  * approximate inverse plus the rank-one corrections the previous steps
  * accumulated; when they exceed max_cols columns the newest max_cols / 2
  * are kept. Every step still ends at the same residual tolerance. Setting
- * the state clears the carried history. */
+ * the state clears the carried history. The history buffers grow to
+ * broyden_maxiter + max_cols columns of 6 Np doubles (two of them) on the
+ * first call that needs it; not while the system belongs to a batch. */
 int fs_system_set_history_carry(fs_system *h, int max_cols);
 /* Whether the last timestep started from the network's guess, and the
  * infinity norms of the residuals of both candidate guesses (res_plain is

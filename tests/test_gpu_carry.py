@@ -67,4 +67,4 @@ def test_carry_bounds():
     with pytest.raises(api.abi.FsError):
         api.system_set_history_carry(s, -1)
     with pytest.raises(api.abi.FsError):
-        api.system_set_history_carry(s, 129)
+        api.system_set_history_carry(s, 193)
