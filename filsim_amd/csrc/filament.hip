@@ -1056,7 +1056,18 @@ __global__ void __launch_bounds__(kFacT)
 // block sweeps: lane 4 row + part (row < 12) sums 3 of the 12 products of
 // a row of a 12 x 12 factor, the four parts combine by lane exchange; the
 // right-hand side lives in LDS, each lane's factor entries in registers,
-// loaded one block row ahead of their use.
+// loaded kSolveAhead block rows ahead of their use (a sweep step is much
+// shorter than a global load's latency).
+constexpr int kSolveAhead = 6;
+// Ordering of LDS accesses between the lanes of a one-wavefront workgroup:
+// a wavefront-scope fence pair around a wave barrier. Unlike a workgroup
+// barrier it does not wait for the wave's outstanding global loads, so
+// loads issued rows ahead stay in flight across it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __global__ void __launch_bounds__(64)
     k_blk_solve(SysDev S, const double *__restrict__ Fall,
                 const double *__restrict__ b, double *__restrict__ x) {
@@ -1081,9 +1092,15 @@ __global__ void __launch_bounds__(64)
   const int row = lane >> 2, part = lane & 3;
   const bool act = row < kBlk;
   const int mo = act ? row * kBlk + 3 * part : 0;  // this lane's 3 entries
+  // every lane loads (lanes past row 11 read row 0's entries and zero
+  // them), so no load sits behind a branch and the waits before a slot's
+  // use count only the loads issued after it
   auto load3 = [&](const double *M, double *o) {
 #pragma unroll
-    for (int m = 0; m < 3; ++m) o[m] = act ? M[mo + m] : 0.0;
+    for (int m = 0; m < 3; ++m) {
+      const double v = M[mo + m];
+      o[m] = act ? v : 0.0;
+    }
   };
   auto dot3 = [&](const double *o, const double *v) {
     double a = 0.0;
@@ -1093,40 +1110,56 @@ __global__ void __launch_bounds__(64)
     a += quad_xor<2>(a);
     return a;
   };
-  // forward: y_K = D_K^-1 (b_K - A_K,K-1 y_K-1)
-  double dc[3], lc[3], dn[3], ln[3];
-  load3(F, dc);
-  load3(F + 2 * kBlkSz, lc);
-  __syncthreads();
-  for (int K = 0; K < nb; ++K) {
-    if (K + 1 < nb) {
-      load3(F + (size_t)(K + 1) * kBlkRec, dn);
-      load3(F + (size_t)(K + 1) * kBlkRec + 2 * kBlkSz, ln);
-    }
-    double *yK = sy + K * kBlk;
-    const double t = K > 0 ? dot3(lc, yK - kBlk) : 0.0;
-    if (act && part == 0) sr[row] = yK[row] - t;
-    __syncthreads();
-    const double y = dot3(dc, sr);
-    if (act && part == 0) yK[row] = y;
-    __syncthreads();
+  // forward: y_K = D_K^-1 (b_K - A_K,K-1 y_K-1). Register slot j holds the
+  // factor entries of the rows K = j (mod kSolveAhead): each row's slot is
+  // refilled, right after its use, with the row kSolveAhead further on (the
+  // last row again past the end: every load is issued, so the wait before
+  // a slot's use counts only the loads issued after it)
+  constexpr int P = kSolveAhead;
+  double dq[P][3], lq[P][3];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      dc[m] = dn[m];
-      lc[m] = ln[m];
+  for (int j = 0; j < P; ++j) {
+    const int r = min(j, nb - 1);
+    load3(F + (size_t)r * kBlkRec, dq[j]);
+    load3(F + (size_t)r * kBlkRec + 2 * kBlkSz, lq[j]);
+  }
+  wave_sync();
+  for (int K0 = 0; K0 < nb; K0 += P) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int K = K0 + j;
+      if (K < nb) {
+        double *yK = sy + K * kBlk;
+        const double t = K > 0 ? dot3(lq[j], yK - kBlk) : 0.0;
+        if (act && part == 0) sr[row] = yK[row] - t;
+        wave_sync();
+        const double y = dot3(dq[j], sr);
+        if (act && part == 0) yK[row] = y;
+        wave_sync();
+      }
+      const int r = min(K + P, nb - 1);
+      load3(F + (size_t)r * kBlkRec, dq[j]);
+      load3(F + (size_t)r * kBlkRec + 2 * kBlkSz, lq[j]);
     }
   }
-  // backward: x_K = y_K - G_K x_K+1
-  double gc[3], gn[3];
-  if (nb >= 2) load3(F + (size_t)(nb - 2) * kBlkRec + kBlkSz, gc);
-  for (int K = nb - 2; K >= 0; --K) {
-    if (K >= 1) load3(F + (size_t)(K - 1) * kBlkRec + kBlkSz, gn);
-    double *yK = sy + K * kBlk;
-    const double t = dot3(gc, yK + kBlk);
-    if (act && part == 0) yK[row] -= t;
-    __syncthreads();
+  // backward: x_K = y_K - G_K x_K+1 over K = nb - 2 .. 0, the same slots
+  // for the G of rows nb - 2 - j (mod kSolveAhead)
+  double gq[P][3];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) gc[m] = gn[m];
+  for (int j = 0; j < P; ++j)
+    load3(F + (size_t)max(nb - 2 - j, 0) * kBlkRec + kBlkSz, gq[j]);
+  for (int s0 = 0; s0 < nb - 1; s0 += P) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int K = nb - 2 - (s0 + j);
+      if (K >= 0) {
+        double *yK = sy + K * kBlk;
+        const double t = dot3(gq[j], yK + kBlk);
+        if (act && part == 0) yK[row] -= t;
+        wave_sync();
+      }
+      load3(F + (size_t)max(K - P, 0) * kBlkRec + kBlkSz, gq[j]);
+    }
   }
   const double idt = 1.0 / S.dt;
   for (int i = lane; i < n; i += 64) {
