@@ -378,6 +378,8 @@ def c5_bench(args, torch):
                                   "lam1", "lam2")}
     keep.update(Xs=state["X"], Us=state["U"], qs=state["q"])
     api.system_set_warmstart(s_l, mlp)
+    api.system_set_warmstart_check(s_l, args.ws_check)
+    api.system_set_history_carry(s_l, args.c5_hist_carry)
     nt0 = 1 + skip + rec
     res = {}
     for name in ("learned", "mixed"):
@@ -410,7 +412,9 @@ def c5_bench(args, torch):
             "unit": "timesteps/s", "n_gpus": 1, "higher_is_better": True,
             "dtype": "f32 FCM + f64 Broyden + bf16 MLP", "data": "synthetic",
             "config": {"workload": "8000 swimming filaments x 30 segments, "
-                                   "512^3, KB=1000, dt=0.005, TOL=1e-6"},
+                                   "512^3, KB=1000, dt=0.005, TOL=1e-6",
+                       "warmstart_check_every": args.ws_check,
+                       "broyden_history_carry_cols": args.c5_hist_carry},
             "learned": res["learned"], "mixed": res["mixed"],
             "speedup_vs_mixed": res["learned"]["timesteps_per_s"] /
                                 res["mixed"]["timesteps_per_s"],
@@ -495,6 +499,10 @@ def main():
                     help="learned path: carry the Broyden history across "
                          "steps while it holds at most this many columns "
                          "(0: restart every step)")
+    ap.add_argument("--c5-hist-carry", type=int, default=0,
+                    help="--only c5: Broyden history carried across steps "
+                         "(measured slower there: 6.0 vs 7.7 timesteps/s "
+                         "with 96 columns)")
     ap.add_argument("--c3-steps", type=int, default=2000,
                     help="steps of the mixed-vs-FP64 swimming run (--only c3)")
     ap.add_argument("--skip", type=int, default=10,
