@@ -936,14 +936,12 @@ __device__ __forceinline__ size_t blk_offset(const SysDev &S, int f) {
 // the 12 x 12 operations spread over the lanes (padding rows past n carry
 // an identity diagonal).
 constexpr int kFacT = 64;
-constexpr int kFacUpd = (kBlk * 2 * kBlk + kFacT - 1) / kFacT;  // updates per lane
 __global__ void __launch_bounds__(kFacT)
     k_blk_factor(SysDev S, const double *__restrict__ Jall,
                  double *__restrict__ Fall, int *__restrict__ singular) {
   if (S.act && !*S.act) return;
   __shared__ double sA[kBlk][2 * kBlk];  // [D | I] -> [I | D^-1]
   __shared__ double sL[kBlkSz], sU[kBlkSz], sG[kBlkSz];
-  __shared__ int s_piv;
   const int f = blockIdx.x;
   const int nw = S.nw[f], n = 6 * nw, nb = (n + kBlk - 1) / kBlk;
   const double *J = Jall + S.joff[f];
@@ -974,63 +972,77 @@ __global__ void __launch_bounds__(kFacT)
         sA[i][j] -= acc;
       }
     __syncthreads();
-    // [D | I] -> [I | D^-1], partial pivoting over the rows below
-    for (int c = 0; c < kBlk; ++c) {
-      if (tid < 64) {
-        double a = -1.0;
-        int t = kBlk;
-        if (tid >= c && tid < kBlk) {
-          const double x = fabs(sA[tid][c]);
-          a = x == x ? x : -0.5;
-          t = tid;
-        }
-        // the candidates are lanes c..11: a max (ties to the lower row)
-        // over each group of 16 lanes through four data-parallel lane
-        // permutations (pairs 1 and 2 apart, then mirrored within 8 and 16
-        // lanes); the combination is associative and commutative, so any
-        // pairing gives the same winner
-        auto take = [&](double oa, int ot) {
-          if (oa > a || (oa == a && ot < t)) {
-            a = oa;
-            t = ot;
+    // [D | I] -> [I | D^-1], partial pivoting over the rows below, in
+    // registers: lane 4 r + q (r < 12) holds columns 6q .. 6q + 5 of row r
+    {
+      const int gr = tid >> 2, gq = tid & 3;
+      const bool own = tid < 4 * kBlk;
+      double a[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) a[k] = own ? sA[gr][6 * gq + k] : 0.0;
+#pragma unroll
+      for (int c = 0; c < kBlk; ++c) {
+        const int cq = c / 6, ck = c % 6;
+        // pivot: the largest |A[r][c]| over r >= c (NaN below every number,
+        // ties to the lower row), compared as bit patterns of the absolute
+        // values in uniform registers
+        unsigned long long best = 0, wbits = 0;
+        int p = c;
+#pragma unroll
+        for (int r = c; r < kBlk; ++r) {
+          const long long v = __double_as_longlong(a[ck]);
+          const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)v, 4 * r + cq);
+          const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), 4 * r + cq);
+          const unsigned long long bits =
+              (((unsigned long long)hi << 32) | lo) & 0x7FFFFFFFFFFFFFFFull;
+          const unsigned long long key =
+              bits > 0x7FF0000000000000ull ? 1ull : bits + 2ull;
+          if (key > best) {
+            best = key;
+            wbits = bits;
+            p = r;
           }
-        };
-        take(dpp_f64<kDppXor1>(a), dpp_i32<kDppXor1>(t));
-        take(dpp_f64<kDppXor2>(a), dpp_i32<kDppXor2>(t));
-        take(dpp_f64<kDppHalfMirror>(a), dpp_i32<kDppHalfMirror>(t));
-        take(dpp_f64<kDppMirror>(a), dpp_i32<kDppMirror>(t));
-        if (tid == 0) {
-          bad = bad || !(a > 0.0) || !(a <= DBL_MAX);
-          s_piv = t < kBlk ? t : c;
+        }
+        // zero, NaN or infinite pivot: singular (the row order still holds)
+        bad = bad || best < 2 || wbits == 0 || wbits >= 0x7FF0000000000000ull;
+        if (p != c) {  // uniform: rows c and p trade their registers
+          const int d = 4 * (p - c);
+          const int src = gr == c ? tid + d : (gr == p ? tid - d : tid);
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            const long long v = __double_as_longlong(a[k]);
+            const int lo = __shfl((int)v, src, 64);
+            const int hi = __shfl((int)(v >> 32), src, 64);
+            a[k] = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+          }
+        }
+        double pv;
+        {
+          const long long v = __double_as_longlong(a[ck]);
+          const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)v, 4 * c + cq);
+          const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), 4 * c + cq);
+          pv = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+        }
+        const double inv = 1.0 / pv;
+        // eliminate column c from the other rows: A[r][j] -= A[r][c] inv A[c][j]
+        // (A[r][c] from the lane of this row holding column c, A[c][j] from
+        // the lane of row c holding column j), then scale row c
+        const double arc = cq == 0 ? dpp_f64<0x00>(a[ck])
+                                   : (cq == 1 ? dpp_f64<0x55>(a[ck])
+                                              : (cq == 2 ? dpp_f64<0xAA>(a[ck])
+                                                         : dpp_f64<0xFF>(a[ck])));
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const long long v = __double_as_longlong(a[k]);
+          const int lo = __shfl((int)v, 4 * c + gq, 64);
+          const int hi = __shfl((int)(v >> 32), 4 * c + gq, 64);
+          const double acj = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+          a[k] = gr == c ? a[k] * inv : a[k] - arc * inv * acj;
         }
       }
-      __syncthreads();
-      const int p = s_piv;
-      if (p != c && tid < 2 * kBlk) {
-        const double x = sA[c][tid];
-        sA[c][tid] = sA[p][tid];
-        sA[p][tid] = x;
-      }
-      __syncthreads();
-      const double inv = 1.0 / sA[c][c];
-      // eliminate column c from the other rows (12 x 24 entries, five per
-      // lane at most), then scale row c
-      double upd[kFacUpd];
+      if (own)
 #pragma unroll
-      for (int h = 0; h < kFacUpd; ++h) {
-        const int e = tid + kFacT * h, ui = e / (2 * kBlk), uj = e - ui * 2 * kBlk;
-        upd[h] = (e < kBlk * 2 * kBlk && ui != c)
-                     ? sA[ui][uj] - sA[ui][c] * inv * sA[c][uj]
-                     : 0.0;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int h = 0; h < kFacUpd; ++h) {
-        const int e = tid + kFacT * h, ui = e / (2 * kBlk), uj = e - ui * 2 * kBlk;
-        if (e < kBlk * 2 * kBlk && ui != c) sA[ui][uj] = upd[h];
-      }
-      __syncthreads();
-      if (tid < 2 * kBlk) sA[c][tid] *= inv;
+        for (int k = 0; k < 6; ++k) sA[gr][6 * gq + k] = a[k];
       __syncthreads();
     }
     // G_K = D^-1 A_K,K+1; store D^-1, G_K, A_K,K-1
