@@ -759,6 +759,9 @@ extern "C" int fs_batch_create(fs_system **sys, int n, fs_batch **out) {
              "system %d has a warm start attached or recording", i);
     FS_CHECK(sys[i]->impl.p.initial_friction_steps == 0, FS_ERR_ARG,
              "system %d: batched sweeps need initial_friction_steps == 0", i);
+    FS_CHECK(sys[i]->impl.hist_cap == 0, FS_ERR_STATE,
+             "system %d carries its Broyden history across steps (batched "
+             "sweeps restart it every step)", i);
     for (int j = 0; j < i; ++j)
       FS_CHECK(sys[j] != sys[i], FS_ERR_ARG, "system %d listed twice", i);
   }

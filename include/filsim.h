@@ -515,8 +515,9 @@ int fs_system_write_forces(fs_system *h, const char *path, int nt);
  * captured once as hipGraphs (begin, one Broyden iteration, end) with the
  * loop state on the device, and one parent graph per phase launches every
  * system. A batched system reproduces its unbatched fs_system_timestep bit
- * for bit. Systems need initial_friction_steps == 0 and no warm start; they
- * must outlive the batch and must not be stepped on their own meanwhile. */
+ * for bit. Systems need initial_friction_steps == 0, no warm start and no
+ * carried Broyden history; they must outlive the batch and must not be
+ * stepped on their own meanwhile. */
 typedef struct fs_batch fs_batch;
 int fs_batch_create(fs_system **systems, int n, fs_batch **out);
 int fs_batch_destroy(fs_batch *b);

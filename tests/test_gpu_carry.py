@@ -68,3 +68,15 @@ def test_carry_bounds():
         api.system_set_history_carry(s, -1)
     with pytest.raises(api.abi.FsError):
         api.system_set_history_carry(s, 193)
+
+
+def test_batch_refuses_carried_history(cuda):
+    sp, nws, props, st = _swimmers()
+    s = api.System(sp, nws, props)
+    s.set_state(**st)
+    api.system_set_history_carry(s, 16)
+    with pytest.raises(api.abi.FsError):
+        api.Batch([s])
+    api.system_set_history_carry(s, 0)
+    b = api.Batch([s])
+    b.close()
