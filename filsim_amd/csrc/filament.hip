@@ -202,13 +202,14 @@ __device__ void unrobotarmify(const SysDev &S, int b0, int nw) {
   for (int i = threadIdx.x; i < nw; i += blockDim.x)
     q_tangent(S.qs + 4 * (b0 + i), s_tan[i]);
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  double Xsum[3] = {S.Xs[3 * b0], S.Xs[3 * b0 + 1], S.Xs[3 * b0 + 2]};
-  for (int i = 1; i < nw; ++i)
-    for (int d = 0; d < 3; ++d) {
-      Xsum[d] += (s_tan[i - 1][d] + s_tan[i][d]) * (0.5 * S.dL);
-      S.Xs[3 * (b0 + i) + d] = Xsum[d];
-    }
+  if (threadIdx.x == 0) {  // no early return: callers go on after it
+    double Xsum[3] = {S.Xs[3 * b0], S.Xs[3 * b0 + 1], S.Xs[3 * b0 + 2]};
+    for (int i = 1; i < nw; ++i)
+      for (int d = 0; d < 3; ++d) {
+        Xsum[d] += (s_tan[i - 1][d] + s_tan[i][d]) * (0.5 * S.dL);
+        S.Xs[3 * (b0 + i) + d] = Xsum[d];
+      }
+  }
 }
 
 // initialGuess (Filament.hpp:227-237, Bead.hpp:202-217) + unRobotArmify
@@ -231,8 +232,7 @@ __global__ void k_guess(SysDev S) {
 }
 
 // Filament::update (Filament.hpp:425-452, Bead.hpp:236-256) + unRobotArmify
-__global__ void k_update(SysDev S, const double *__restrict__ u) {
-  if (S.act && !*S.act) return;
+__device__ __forceinline__ void d_update(const SysDev &S, const double *__restrict__ u) {
   const int f = blockIdx.x;
   const int b0 = S.boff[f], l0 = S.loff[f], nw = S.nw[f];
   const size_t so = 6 * (size_t)b0;
@@ -256,12 +256,15 @@ __global__ void k_update(SysDev S, const double *__restrict__ u) {
   __syncthreads();
   unrobotarmify(S, b0, nw);
 }
+__global__ void k_update(SysDev S, const double *__restrict__ u) {
+  if (S.act && !*S.act) return;
+  d_update(S, u);
+}
 
 // setZeroForcesTorques + applyElasticTorques + applyConstraintForcesTorques
 // + applyExternalForcesTorques (Filament.hpp:271-402) + assign_filament_data
 // (FCMfunctions.cpp:882-918). One workgroup per filament.
-__global__ void k_forces(SysDev S, int nt) {
-  if (S.act && !*S.act) return;
+__device__ __forceinline__ void d_forces(const SysDev &S, int nt) {
   if (S.ntp) nt = *S.ntp;
   __shared__ double s_v[kMaxNw][3];  // elastic torque of link l
   __shared__ double s_t[kMaxNw][3];  // tangent of bead i
@@ -341,6 +344,18 @@ __global__ void k_forces(SysDev S, int nt) {
       S.Ta[3 * g + d] = tau[d];
     }
   }
+}
+__global__ void k_forces(SysDev S, int nt) {
+  if (S.act && !*S.act) return;
+  d_forces(S, nt);
+}
+// Filament::update then the forces of the updated state, one workgroup per
+// filament: both read and write only their own filament's beads
+__global__ void k_update_forces(SysDev S, const double *__restrict__ u, int nt) {
+  if (S.act && !*S.act) return;
+  d_update(S, u);
+  __syncthreads();
+  d_forces(S, nt);
 }
 
 // Spring links (spring_links::apply_spring_link_forces,
@@ -1654,9 +1669,23 @@ int SystemImpl::initial_guess() {
   return FS_OK;
 }
 
+// update(u) then forces(nt) in one launch
+int SystemImpl::update_forces(const double *u, int nt) {
+  ProfScope ps_(*this, P_ASSIGN);
+  hipLaunchKernelGGL(k_update_forces, dim3(nf), dim3(64), 0, st, d, u, nt);
+  FS_LAUNCHED();
+  return springs();
+}
+
 int SystemImpl::forces(int nt) {
   ProfScope ps_(*this, P_ASSIGN);
   hipLaunchKernelGGL(k_forces, dim3(nf), dim3(64), 0, st, d, nt);
+  FS_LAUNCHED();
+  return springs();
+}
+
+// spring link forces (after the filament forces)
+int SystemImpl::springs() {
   if (spring_nb > 0) {
     const int *sb = springb.as<int>();
     hipLaunchKernelGGL(k_springs, dim3(ceil_div(spring_nb, 64)), dim3(64), 0,
@@ -2046,8 +2075,7 @@ int SystemImpl::broyden_loop(int nt, int *iters) {
   const int h0 = hist_cap > 0 ? hist_cols : 0;
   if (check) FS_TRY(jsolve(E, Upd, h0));
   while (check && iter < p.broyden_maxiter) {
-    FS_TRY(update(Upd));
-    FS_TRY(forces(nt));
+    FS_TRY(update_forces(Upd, nt));
     FS_TRY(barrier(nt, 0, nullptr));
     FS_TRY(mobility(nt));
     FS_TRY(residual(nt, d.V, d.W, NewE, &check, &last_res));
@@ -2157,8 +2185,7 @@ int SystemImpl::iteration_dev(int *status, int part) {
   const size_t n6 = 6 * (size_t)np;
   set_device_control(act, ntp);
   if (part != 1) {
-    FS_TRY(update(Upd));
-    FS_TRY(forces(nt));
+    FS_TRY(update_forces(Upd, nt));
     FS_TRY(barrier(nt, 0, nullptr));
   }
   if (part == -1) FS_TRY(mobility(nt));

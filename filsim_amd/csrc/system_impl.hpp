@@ -66,6 +66,8 @@ struct SystemImpl {
   int initial_guess();
   int forces(int nt);
   int update(const double *u);
+  int update_forces(const double *u, int nt);  // update + forces, one launch
+  int springs();
   // clear_history: zero the Broyden history (the API's reset); the
   // timestep reads only the columns its own iterations wrote and skips it
   int build_jacobian(int nt, bool clear_history = true);

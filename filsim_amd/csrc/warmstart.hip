@@ -252,8 +252,7 @@ int WarmStart::timestep(SystemImpl &S, int nt, int *iters) {
     hipLaunchKernelGGL(k_ws_apply, dim3(S.nf), dim3(64), 0, st, S.d, nw,
                        pred.as<float>(), nout, u.as<double>());
     FS_LAUNCHED();
-    FS_TRY(S.update(u.as<double>()));
-    FS_TRY(S.forces(nt));
+    FS_TRY(S.update_forces(u.as<double>(), nt));
     FS_TRY(S.barrier(nt, 0, nullptr));
     FS_TRY(S.mobility(nt));
     int chk = 0;
